@@ -1,0 +1,429 @@
+"""Pins the physics oracle with closed forms, invariants and continuum limits.
+
+None of these checks re-types an oracle formula: each compares the oracle with
+a mathematical fact (an integral, a conservation law, a Taylor expansion of a
+smooth field, an exact Shepard identity, an analytic decay rate).
+"""
+import math
+
+import numpy as np
+import pytest
+from scipy import integrate
+
+from oracle import nl_oracle as nlo
+from oracle import sph_oracle as so
+from tests import helpers as H
+
+ALL_KERNELS = ["CSK", "QSK", "WC2K", "WC4K", "WC6K", "GK", "SGK"]
+
+
+# --------------------------------------------------------------------- kernels
+@pytest.mark.parametrize("name", ALL_KERNELS)
+@pytest.mark.parametrize("dim", [2, 3])
+def test_kernel_normalisation(name, dim):
+    k = so.Kernel(name, 0.7, dim, np.float64)
+    shell = (lambda r: 2 * math.pi * r) if dim == 2 else (lambda r: 4 * math.pi * r * r)
+    val, _ = integrate.quad(lambda r: float(k.w(np.array([r]))[0]) * shell(r), 0, k.cutoff,
+                            points=[0.7, 1.4], limit=200)
+    # the reference's 3D quintic constant 3/(359 pi) integrates to 360/359; the
+    # Gaussians are truncated at 3h, whose closed forms follow from erf and e^-9
+    e9, sp = math.exp(-9.0), math.sqrt(math.pi)
+    expect = {("QSK", 3): 360.0 / 359.0, ("GK", 2): 1 - e9,
+              ("GK", 3): math.erf(3.0) - 6 * e9 / sp, ("SGK", 2): 1 + 8 * e9,
+              ("SGK", 3): math.erf(3.0) + 48 * e9 / sp}.get((name, dim), 1.0)
+    assert abs(val - expect) < 1e-9, (name, dim, val, expect)
+
+
+@pytest.mark.parametrize("name,fac", [("CSK", 1), ("QSK", 1), ("WC2K", 1.3), ("WC4K", 1.3),
+                                      ("WC6K", 1.3), ("GK", 1)])
+def test_kernel_1d_reference_checks(name, fac):
+    """tests/test_kernel.py:17-44 of the reference, restated."""
+    N = 500
+    for h in (1, 0.1, 0.01):
+        k = so.Kernel(name, fac * h, 1, np.float64)
+        dx = k.cutoff / N
+        x = np.linspace(dx / 2, k.cutoff + dx / 2, N)
+        w, g = k.w(x), k.grad_w(x)
+        assert np.isclose(dx * w.sum(), 0.5, atol=1e-2)
+        assert (w >= 0).all()
+        assert (g <= 1e-12 * np.abs(g).max()).all()
+
+
+@pytest.mark.parametrize("name", ALL_KERNELS)
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_kernel_gradient_matches_finite_differences(name, dim):
+    k = so.Kernel(name, 0.37, dim, np.float64)
+    r = np.linspace(0.013, k.cutoff * 0.999, 400)
+    knots = np.array([0.37, 0.74, 1.11])
+    r = r[np.min(np.abs(r[:, None] - knots[None]), axis=1) > 1e-4]
+    e = 1e-6
+    fd = (k.w(r + e) - k.w(r - e)) / (2 * e)
+    np.testing.assert_allclose(k.grad_w(r), fd, rtol=1e-5, atol=1e-6 * np.abs(fd).max())
+
+
+def test_quintic_support_and_sign():
+    for dim in (2, 3):
+        k = so.Kernel("QSK", 0.02, dim, np.float32)
+        assert k.w(np.float32(k.cutoff)) == 0 and k.grad_w(np.float32(k.cutoff)) == 0
+        assert k.grad_w(np.float32(0.0)) == 0  # self pair: no force
+
+
+# ------------------------------------------------------------------------- EOS
+def test_eos_round_trip_and_reference_point():
+    for gamma in (1.0, 7.0):
+        e = H.tait(gamma=gamma, p_bg_factor=0.01, dtype=np.float64)
+        rho = np.linspace(0.9, 1.1, 11)
+        np.testing.assert_allclose(e.rho_fn(e.p_fn(rho)), rho, rtol=1e-13)
+        assert abs(e.p_fn(np.array([1.0]))[0] - e.p_bg) < 1e-12
+    r = so.RIEMANNEoS(1.0, 0.5, 2 ** 0.5, np.float64)
+    rho = np.linspace(0.9, 1.1, 11)
+    np.testing.assert_allclose(r.rho_fn(r.p_fn(rho)), rho, rtol=1e-13)
+    np.testing.assert_allclose(r.p_fn(rho), 200 * (rho - 1) + 0.5, rtol=1e-13)
+
+
+# -------------------------------------------------------------- helpers for RHS
+def _solver(box, dx, dim, dtype=np.float64, **kw):
+    eos = kw.pop("eos", None) or H.tait(dtype=dtype)
+    g = kw.pop("g", H.zero_g)
+    return so.WCSPH(box, eos, g, dx, dim, kw.pop("dt", 1e-4), 10.0, dtype=dtype, **kw)
+
+
+def _periodic_case(n, dim, dtype=np.float64, jitter=0.0, seed=0):
+    dx = 1.0 / n
+    r = H.lattice(n, dx, dim, dtype)
+    if jitter:
+        rng = np.random.default_rng(seed)
+        r = nlo.periodic_shift(np.ones(dim, dtype), r,
+                               (rng.normal(size=r.shape) * jitter * dx).astype(dtype))
+    box = np.ones(dim)
+    return r, box, dx
+
+
+# ------------------------------------------------------------- density summation
+@pytest.mark.parametrize("dim,n", [(2, 20), (3, 10)])
+def test_lattice_density_matches_kernel_integral(dim, n):
+    r, box, dx = _periodic_case(n, dim)
+    s = _solver(box, dx, dim)
+    st = H.base_state(r, dx=dx, dtype=np.float64)
+    idx = H.pairs(r, box, s.kernel.cutoff, np.float64)
+    out = s.forward(st, idx)
+    expect = 360.0 / 359.0 if dim == 3 else 1.0
+    # lattice sum of a smooth compact kernel converges to its integral (1e-4 level)
+    np.testing.assert_allclose(out["rho"], expect, rtol=1e-4)
+    # and the pressure is the EOS of it; a uniform lattice at rest feels no force
+    assert np.abs(out["dudt"]).max() < 1e-10
+    assert np.abs(out["dvdt"]).max() < 1e-10
+
+
+def test_wendland_lattice_density_vs_direct_lattice_sum():
+    """WC2K h=dx has a coarse support; compare with a direct sum over integer offsets."""
+    n, dim = 20, 2
+    r, box, dx = _periodic_case(n, dim)
+    for hf in (1.0, 1.3):
+        s = _solver(box, dx, dim, kernel="WC2K", h_fac=hf)
+        out = s.forward(H.base_state(r, dx=dx, dtype=np.float64),
+                        H.pairs(r, box, s.kernel.cutoff, np.float64))
+        off = np.arange(-4, 5)
+        ox, oy = np.meshgrid(off, off)
+        d = np.hypot(ox, oy).ravel() * dx
+        direct = dx ** 2 * so.Kernel("WC2K", hf * dx, 2, np.float64).w(
+            d[d < 2 * hf * dx]).sum()
+        np.testing.assert_allclose(out["rho"], direct, rtol=1e-12)
+
+
+# ---------------------------------------------------------- momentum conservation
+@pytest.mark.parametrize("solver,rho_evol,alpha", [("SPH", False, 0.0), ("SPH", True, 0.0),
+                                                    ("RIE", True, 0.0), ("SPH", False, 0.1),
+                                                    ("RIE", True, 0.1)])
+def test_momentum_conservation(solver, rho_evol, alpha):
+    r, box, dx = _periodic_case(16, 2, jitter=0.2, seed=1)
+    N = len(r)
+    rng = np.random.default_rng(2)
+    eos = so.RIEMANNEoS(1.0, 0.0, 1.0, np.float64) if solver == "RIE" else H.tait(
+        dtype=np.float64)
+    s = _solver(box, dx, 2, solver=solver, is_rho_evol=rho_evol, artificial_alpha=alpha,
+                eos=eos)
+    st = H.base_state(r, dx=dx, dtype=np.float64)
+    st["u"] = rng.normal(size=r.shape)
+    st["v"] = st["u"] + 0.1 * rng.normal(size=r.shape)
+    st["rho"] = 1.0 + 0.01 * rng.normal(size=N)
+    st["p"] = eos.p_fn(st["rho"])
+    st["mass"] = st["mass"] * (1 + 0.1 * rng.random(N))
+    st["eta"] = 0.01 * (1 + rng.random(N))
+    out = s.forward(st, H.pairs(r, box, s.kernel.cutoff, np.float64))
+    m = st["mass"][:, None]
+    for key in ("dudt", "dvdt"):
+        tot = (m * out[key]).sum(axis=0)
+        scale = np.abs(m * out[key]).sum(axis=0)
+        assert np.all(np.abs(tot) <= 1e-12 * scale + 1e-300), (key, tot, scale)
+
+
+def test_heat_conduction_conserves_energy():
+    r, box, dx = _periodic_case(16, 2, jitter=0.2, seed=3)
+    N = len(r)
+    rng = np.random.default_rng(4)
+    s = _solver(box, dx, 2, is_heat_conduction=True)
+    st = H.base_state(r, dx=dx, dtype=np.float64)
+    st["T"] = rng.random(N)
+    st["kappa"] = 0.5 + rng.random(N)
+    st["Cp"] = 1.0 + rng.random(N)
+    out = s.forward(st, H.pairs(r, box, s.kernel.cutoff, np.float64))
+    e = st["mass"] * st["Cp"] * out["dTdt"]
+    assert abs(e.sum()) < 1e-12 * np.abs(e).sum()
+    # heat flows from hot to cold: the hottest particle cools
+    assert out["dTdt"][np.argmax(out["T"])] < 0
+
+
+# ------------------------------------------------------ continuum limits (sign + size)
+def _interior(r, margin):
+    return np.all((r > margin) & (r < 1 - margin), axis=1)
+
+
+@pytest.mark.parametrize("solver", ["SPH", "RIE"])
+def test_linear_pressure_gives_minus_grad_p(solver):
+    """p = G x: dudt -> -G / rho (sign and magnitude).
+
+    p comes from rho through the EOS, so rho varies by G/100 across the box; the
+    Adami form picks up an O(p * grad rho) term, kept below 1e-3 by a small G."""
+    n, G = 40, 0.05
+    r, box, dx = _periodic_case(n, 2)
+    if solver == "SPH":
+        eos = H.tait(dtype=np.float64)            # p = 100 (rho - 1)
+    else:
+        eos = so.RIEMANNEoS(1.0, 0.0, 1.0, np.float64)  # p = 100 (rho - 1)
+    s = _solver(box, dx, 2, solver=solver, is_rho_evol=True, eos=eos)
+    st = H.base_state(r, dx=dx, dtype=np.float64)
+    st["rho"] = 1.0 + G * r[:, 0] / 100.0
+    st["p"] = eos.p_fn(st["rho"])
+    out = s.forward(st, H.pairs(r, box, s.kernel.cutoff, np.float64))
+    m = _interior(r, 4 * dx)
+    np.testing.assert_allclose(out["dudt"][m, 0], -G / st["rho"][m], rtol=2e-3)
+    assert np.abs(out["dudt"][m, 1]).max() < 1e-9
+
+
+@pytest.mark.parametrize("solver", ["SPH", "RIE"])
+def test_shear_wave_viscous_term(solver):
+    """u = (sin 2 pi y, 0), p = 0: dudt -> nu * lap u = -nu (2 pi)^2 sin 2 pi y."""
+    n, eta = 50, 0.01
+    r, box, dx = _periodic_case(n, 2)
+    eos = H.tait(dtype=np.float64) if solver == "SPH" else so.RIEMANNEoS(
+        1.0, 0.0, 1.0, np.float64)
+    # eta_limiter=0 switches the Riemann dissipation off (beta = 0), isolating viscosity
+    s = _solver(box, dx, 2, solver=solver, is_rho_evol=True, eos=eos, eta_limiter=0)
+    st = H.base_state(r, dx=dx, eta=eta, dtype=np.float64)
+    st["u"][:, 0] = np.sin(2 * np.pi * r[:, 1])
+    st["v"] = st["u"].copy()
+    out = s.forward(st, H.pairs(r, box, s.kernel.cutoff, np.float64))
+    expect = -eta * (2 * np.pi) ** 2 * np.sin(2 * np.pi * r[:, 1])
+    np.testing.assert_allclose(out["dudt"][:, 0], expect, atol=2e-3 * np.abs(expect).max())
+
+
+def test_velocity_divergence_gives_continuity():
+    """u = (k sin 2 pi x, 0): drho/dt -> -rho div u (SPH); RIE equals SPH at uniform p."""
+    n, k = 50, 0.3
+    r, box, dx = _periodic_case(n, 2)
+    st = H.base_state(r, dx=dx, dtype=np.float64)
+    st["u"][:, 0] = k * np.sin(2 * np.pi * r[:, 0])
+    st["v"] = st["u"].copy()
+    sph = _solver(box, dx, 2, is_rho_evol=True)
+    idx = H.pairs(r, box, sph.kernel.cutoff, np.float64)
+    d_sph = sph.forward(st, idx)["drhodt"]
+    expect = -k * 2 * np.pi * np.cos(2 * np.pi * r[:, 0])
+    # kernel smoothing damps a sine's derivative by O((k h)^2) ~ 3e-3 here
+    np.testing.assert_allclose(d_sph, expect, atol=6e-3 * np.abs(expect).max())
+    rie = _solver(box, dx, 2, solver="RIE", is_rho_evol=True,
+                  eos=so.RIEMANNEoS(1.0, 0.0, 1.0, np.float64))
+    d_rie = rie.forward(st, idx)["drhodt"]
+    np.testing.assert_allclose(d_rie, d_sph, rtol=1e-10, atol=1e-12)
+
+
+def test_tvf_stress_divergence():
+    """u = (U, 0), v - u = (0, k y): the A term gives dudt_x -> d/dy(rho U k y)/rho = U k."""
+    n, U, k = 40, 0.7, 0.5
+    r, box, dx = _periodic_case(n, 2)
+    s = _solver(box, dx, 2, is_rho_evol=True)
+    st = H.base_state(r, dx=dx, eta=0.0, dtype=np.float64)
+    st["u"][:, 0] = U
+    st["v"][:, 0] = U
+    st["v"][:, 1] = k * r[:, 1]
+    out = s.forward(st, H.pairs(r, box, s.kernel.cutoff, np.float64))
+    m = _interior(r, 4 * dx)
+    np.testing.assert_allclose(out["dudt"][m, 0], U * k, rtol=2e-3)
+
+
+def test_tvf_background_matches_pressure_term():
+    """With uniform p = P, the pressure acceleration equals -(P / p_b) * dvdt."""
+    r, box, dx = _periodic_case(16, 2, jitter=0.25, seed=8)
+    s = _solver(box, dx, 2, is_rho_evol=True)
+    st = H.base_state(r, dx=dx, eta=0.0, dtype=np.float64)
+    st["rho"][:] = 1.03
+    out = s.forward(st, H.pairs(r, box, s.kernel.cutoff, np.float64))
+    P = out["p"][0]
+    p_b = s.eos.p_fn(np.zeros(1))[0]
+    assert P != 0 and p_b == -100.0
+    np.testing.assert_allclose(out["dudt"], -(P / p_b) * out["dvdt"], rtol=1e-9, atol=1e-12)
+
+
+def test_riemann_limiter_adds_dissipation_only_in_compression():
+    """beta = min(eta_lim * max(u_L - u_R, 0), c): acts only on approaching pairs and
+    pushes them apart; compared against eta_limiter = 0 (beta = 0)."""
+    r, box, dx = _periodic_case(16, 2)
+    eos = so.RIEMANNEoS(1.0, 0.0, 1.0, np.float64)
+    st = H.base_state(r, dx=dx, eta=0.0, dtype=np.float64)
+    st["u"][:, 0] = -0.2 * np.sin(2 * np.pi * r[:, 0])   # converging at x = 0, diverging at 0.5
+    idx = H.pairs(r, box, 3 * dx, np.float64)
+    a = {}
+    for lim in (3, 0):
+        s = _solver(box, dx, 2, solver="RIE", is_rho_evol=True, eos=eos, eta_limiter=lim)
+        a[lim] = s.forward(st, idx)["dudt"][:, 0]
+    diss = a[3] - a[0]
+    x = r[:, 0]
+    div = np.abs(x - 0.5) < 1.5 * dx
+    left = (x > 1 - 1.5 * dx)           # just left of the x = 0 (= 1) convergence line
+    right = (x < 1.5 * dx)
+    assert np.abs(diss[div]).max() < 1e-12
+    assert diss[left].max() < 0 and diss[right].min() > 0
+
+
+# ---------------------------------------------------------------- wall conditions
+def _channel(n_fluid=12, n_walls=3, dx=0.05, dtype=np.float64):
+    """Fluid rows y in (3dx, 3dx+H) between two wall slabs, periodic in x."""
+    nx = 16
+    fl = H.lattice((nx, n_fluid), dx, 2, dtype) + np.array([0.0, n_walls * dx])
+    wb = H.lattice((nx, n_walls), dx, 2, dtype)
+    wt = wb + np.array([0.0, (n_fluid + n_walls) * dx])
+    r = np.concatenate([wb, wt, fl]).astype(dtype)
+    tag = np.concatenate([np.ones(len(wb) + len(wt), np.int64), np.zeros(len(fl), np.int64)])
+    box = np.array([nx * dx, (n_fluid + 2 * n_walls) * dx])
+    return r, tag, box, dx
+
+
+def test_gwbc_no_slip_and_hydrostatic_pressure():
+    r, tag, box, dx = _channel()
+    N = len(r)
+    gmag, H_top = 2.0, 0.8
+    # a stiff EOS keeps rho_j = 1 + O(1e-6) so the extrapolation identity is sharp
+    s = _solver(box, dx, 2, is_bc_trick=True, is_rho_evol=True, dt=1e-12,
+                eos=H.tait(c_ref=1000.0, dtype=np.float64),
+                g=lambda x: np.tile([0.0, -gmag], (len(x), 1)))
+    st = H.base_state(r, dx=dx, dtype=np.float64)
+    st["tag"] = tag
+    U = np.array([0.3, -0.1])
+    st["u"][tag == 0] = U
+    st["v"][tag == 0] = U
+    # hydrostatic column p = rho g (H - y) in the fluid; rho from the EOS inverse
+    p_f = 1.0 * gmag * (H_top - r[:, 1])
+    st["rho"] = np.where(tag == 0, s.eos.rho_fn(p_f), 1.0)
+    out = s.forward(st, H.pairs(r, box, s.kernel.cutoff, np.float64))
+    w = (tag == 1)
+    # walls that see fluid: u_wall = 2*0 - U exactly (Shepard of a constant)
+    near = w & (np.minimum(np.abs(r[:, 1] - 3 * dx), np.abs(r[:, 1] - 15 * dx)) < 2 * dx)
+    np.testing.assert_allclose(out["u"][near], np.tile(-U, (near.sum(), 1)), rtol=1e-12)
+    # wall pressure = Shepard(p_j + rho_j g.r_ij) = the linear hydrostatic field
+    # evaluated at the wall position
+    p_expect = 1.0 * gmag * (H_top - r[near, 1])
+    np.testing.assert_allclose(out["p"][near], p_expect, atol=1e-5 * gmag)
+    # every particle's density is the EOS inverse of its (new) pressure
+    np.testing.assert_allclose(out["rho"], s.eos.rho_fn(out["p"]), rtol=1e-14)
+
+
+def test_gwbc_free_slip_projects_on_normal():
+    r, tag, box, dx = _channel()
+    s = _solver(box, dx, 2, is_bc_trick=True, is_rho_evol=True, is_free_slip=True)
+    st = H.base_state(r, dx=dx, dtype=np.float64)
+    st["tag"] = tag
+    U = np.array([0.3, -0.1])
+    st["u"][tag == 0] = U
+    st["v"][tag == 0] = U
+    nw = np.zeros_like(r)
+    nw[(tag == 1) & (r[:, 1] < box[1] / 2), 1] = 1.0     # bottom wall normal points up
+    nw[(tag == 1) & (r[:, 1] > box[1] / 2), 1] = -1.0
+    st["nw"] = nw
+    out = s.forward(st, H.pairs(r, box, s.kernel.cutoff, np.float64))
+    near = (tag == 1) & (np.minimum(np.abs(r[:, 1] - 3 * dx), np.abs(r[:, 1] - 15 * dx))
+                         < 2 * dx)
+    # only the normal component is mirrored: u_wall = 2*0 - (U.n) n
+    np.testing.assert_allclose(out["u"][near, 0], 0.0, atol=1e-14)
+    np.testing.assert_allclose(out["u"][near, 1], 0.1, rtol=1e-12)
+
+
+def test_riemann_wall_velocity_is_shepard_average():
+    r, tag, box, dx = _channel()
+    eos = so.RIEMANNEoS(1.0, 0.0, 1.0, np.float64)
+    s = _solver(box, dx, 2, solver="RIE", is_bc_trick=True, is_rho_evol=True, eos=eos)
+    st = H.base_state(r, dx=dx, eta=0.01, dtype=np.float64)
+    st["tag"] = tag
+    U = np.array([0.3, 0.0])
+    st["u"][tag == 0] = U
+    st["v"][tag == 0] = U
+    nw = np.zeros_like(r)
+    nw[(tag == 1) & (r[:, 1] < box[1] / 2), 1] = 1.0
+    nw[(tag == 1) & (r[:, 1] > box[1] / 2), 1] = -1.0
+    st["nw"] = nw
+    idx = H.pairs(r, box, s.kernel.cutoff, np.float64)
+    out = s.forward(st, idx)
+    # uniform plug flow between no-slip walls: u_d = 2*0 - U mirrors the fluid, so the
+    # viscous force on the fluid next to the wall is a drag -x, and zero in the core
+    a = out["dudt"][:, 0]
+    fl = tag == 0
+    y = r[:, 1]
+    core = fl & (np.abs(y - box[1] / 2) < 2 * dx)
+    edge = fl & ((y < 4 * dx) | (y > box[1] - 4 * dx))
+    assert np.abs(a[core]).max() < 1e-12
+    assert a[edge].max() < 0
+    # wall neighbours carry no pressure jump here: p_R = p_L (g = 0) so density is flat
+    np.testing.assert_allclose(out["drhodt"][fl], 0.0, atol=1e-12)
+
+
+# ------------------------------------------------------------------- si_euler
+def test_si_euler_free_streaming_and_ordering():
+    r, box, dx = _periodic_case(8, 2)
+    st = H.base_state(r, dx=dx, dtype=np.float64)
+    rng = np.random.default_rng(0)
+    st["u"] = rng.normal(size=r.shape)
+    st["dudt"] = rng.normal(size=r.shape)
+    st["dvdt"] = rng.normal(size=r.shape)
+    seen = {}
+
+    def model(state, idx):
+        seen.update(state)
+        out = dict(state)
+        out["dudt"] = np.zeros_like(state["dudt"])
+        return out
+
+    ns = nlo.NeighborSearch(box, 0.3, mask_self=False, dtype=np.float64)
+    adv = so.si_euler(1.0, model, box, None, ns.update, np.float64)
+    nb = ns.allocate(st["r"])
+    dt = 0.01
+    out, _ = adv(dt, st, nb)
+    u1 = st["u"] + dt * st["dudt"]                       # kick with the incoming dudt
+    v1 = u1 + 0.5 * dt * st["dvdt"]                      # transport velocity
+    np.testing.assert_allclose(seen["u"], u1, rtol=1e-14)
+    np.testing.assert_allclose(seen["v"], v1, rtol=1e-14)
+    np.testing.assert_allclose(seen["r"], np.mod(r + dt * v1, 1.0), atol=1e-14)
+
+
+def test_tgv2d_kinetic_energy_decay():
+    """2D TGV, Re=100: E_kin(t) -> 0.25 exp(-16 pi^2 t / Re) (validate.py:181-194)."""
+    dx, dt, steps = 0.02, 4e-4, 250
+    n = 50
+    r = H.lattice(n, dx, 2, np.float64)
+    st = H.base_state(r, dx=dx, dtype=np.float64)
+    x, y = r[:, 0], r[:, 1]
+    st["u"][:, 0] = -np.cos(2 * np.pi * x) * np.sin(2 * np.pi * y)
+    st["u"][:, 1] = np.sin(2 * np.pi * x) * np.cos(2 * np.pi * y)
+    st["v"] = st["u"].copy()
+    box = np.ones(2)
+    s = _solver(box, dx, 2, dt=dt)
+    ns = nlo.NeighborSearch(box, s.kernel.cutoff, mask_self=False, dtype=np.float64)
+    adv = so.si_euler(0.0, s.forward, box, None, ns.update, np.float64)
+    nb = ns.allocate(st["r"])
+    e0 = so.get_ekin(st, dx)
+    for _ in range(steps):
+        st, nb = adv(dt, st, nb)
+    t = steps * dt
+    rate = -math.log(so.get_ekin(st, dx) / e0) / t
+    # weakly compressible SPH on a lattice adds acoustic/numerical dissipation on top
+    # of the physical decay (about +12% in rate here); the viscous operator itself is
+    # pinned sharply by test_shear_wave_viscous_term
+    assert abs(rate / (16 * math.pi ** 2 / 100.0) - 1) < 0.15, rate
