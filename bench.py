@@ -1,0 +1,175 @@
+"""Throughput of the SPH step on one MI355X: 3D Taylor-Green vortex, 262,144 particles.
+
+Workload (the 3D TGV configuration): periodic cube [0, 2 pi]^3, 64^3 particles on
+a lattice, float32, Quintic kernel h = dx (cutoff 3h), transport-velocity SPH
+(tvf = 1), density summation, Tait EOS c0 = 10, Re = 1600, dt from the CFL
+condition (2.2312e-3).  A step is one fused si_euler step: kick + drift +
+periodic wrap, cell binning, neighbour search, density + EOS, forces.
+
+Prints one JSON line.  ``value`` = particle-steps/s over the timed steps (inputs
+resident in HBM).  ``roofline`` = the dominant kernel's algorithmic bytes per
+launch / its mean device duration (HIP events on the library's stream).
+``cpu_baseline`` = the numpy oracle (a CPU restatement of jax-sph) on a bounded
+sample of the same workload, on this host.
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "jax-sph_amd"))
+
+HBM_PEAK_GBS = 8000.0
+FP32_VALU_TFLOPS = 157.3
+
+# algorithmic (compulsory) bytes per particle and launch, 3D float32:
+# force: reads r, u, v, rho, p, mass, eta, tag (14 x 4 B), writes dudt, dvdt (6 x 4 B)
+# density: reads r, mass, tag (5 x 4 B), writes rho, p (2 x 4 B)
+# nl_build: reads r (12 B), writes one int32 per pair (4 B x pairs / N)
+ALG_BYTES_3D = {"force": 80.0, "density": 28.0}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n-side", type=int, default=64)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample-side", type=int, default=20)
+    return ap.parse_args()
+
+
+def cpu_baseline(n_side, seconds=15.0):
+    """Oracle si_euler on a 3D TGV lattice of n_side^3 particles (same physics)."""
+    sys.path.insert(0, ROOT)
+    from oracle import nl_oracle as nlo
+    from oracle import sph_oracle as so
+    dx = 2 * np.pi / n_side
+    box = np.full(3, 2 * np.pi)
+    i = np.arange(n_side)
+    g = np.meshgrid(i, i, i, indexing="xy")
+    idx = np.vstack([x.ravel() for x in g]).T
+    r = ((idx + np.float32(0.5)).astype(np.float32) * np.float32(dx)).astype(np.float32)
+    N = len(r)
+    x, y, z = r[:, 0], r[:, 1], r[:, 2]
+    u = np.zeros_like(r)
+    u[:, 0] = np.sin(x) * np.cos(y) * np.cos(z)
+    u[:, 1] = -np.cos(x) * np.sin(y) * np.cos(z)
+    f = np.float32
+    st = {"r": r, "tag": np.zeros(N, np.int64), "u": u, "v": u.copy(),
+          "dudt": np.zeros_like(r), "dvdt": np.zeros_like(r), "drhodt": np.zeros(N, f),
+          "rho": np.ones(N, f), "p": np.zeros(N, f), "mass": np.full(N, f(dx ** 3), f),
+          "eta": np.full(N, f(1 / 1600), f), "dTdt": np.zeros(N, f), "T": np.ones(N, f),
+          "kappa": np.zeros(N, f), "Cp": np.zeros(N, f), "nw": np.zeros_like(r)}
+    eos = so.TaitEoS(100.0, 1.0, 0.0, 1.0)
+    dt = 0.25 * dx / 11.0
+    solver = so.WCSPH(box, eos, lambda q: np.zeros_like(q), dx, 3, dt, 10.0)
+    ns = nlo.NeighborSearch(box, solver.kernel.cutoff, mask_self=False)
+    nb = ns.allocate(st["r"])
+    adv = so.si_euler(1.0, solver.forward, box, None, ns.update)
+    st, nb = adv(dt, st, nb)  # warm-up
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        st, nb = adv(dt, st, nb)
+        steps += 1
+        if time.perf_counter() - t0 > seconds:
+            break
+    el = time.perf_counter() - t0
+    return {"value": N * steps / el, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": f"numpy oracle (CPU restatement of jax-sph), 3D TGV {n_side}^3 = {N} "
+                      f"particles, {steps} si_euler steps in {el:.1f} s, single thread, "
+                      f"{platform.processor() or platform.machine()}, "
+                      f"nproc={os.cpu_count()}"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not "
+                                   "part of it (--gpus must be 1)"}))
+        sys.exit(2)
+    import torch
+    import jax_sph_amd as js
+    from jax_sph_amd import case_setup as cs
+    from jax_sph_amd import integrator, partition
+
+    n_side = a.n_side
+    cfg = cs.load_config("tgv", {"case": {"dim": 3, "dx": 2 * np.pi / n_side,
+                                          "viscosity": 1 / 1600},
+                                 "solver": {"tvf": 1.0, "t_end": 1.0}})
+    (cfg, box, state, g_ext_fn, bc_fn, nw_fn, eos, key, disp, shift) = \
+        cs.load_case("tgv")(cfg).initialize()
+    solver = cs.make_solver(cfg, disp, eos, g_ext_fn)
+    fn = partition.neighbor_list(disp, box, r_cutoff=solver._kernel_fn.cutoff,
+                                 capacity_multiplier=1.25, mask_self=False,
+                                 format=partition.Sparse)
+    nb = fn.allocate(state["r"])
+    adv = integrator.si_euler(cfg.solver.tvf, solver.forward_wrapper(), shift, bc_fn, nw_fn)
+    assert adv._sph_fusable(nb) is not None, "bench must run the fused step"
+    N = state["r"].shape[0]
+    dt = cfg.solver.dt
+    ws = nb.workspace
+    st = state
+    for _ in range(a.warmup):
+        st, nb = adv(dt, st, nb)
+    torch.cuda.synchronize()
+    ws.prof_reset()
+    ws.prof_enable(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        st, nb = adv(dt, st, nb)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    ws.prof_enable(False)
+    prof = ws.prof_read()
+    overflow = int(nb.error.code)
+    value = N * a.steps / el
+    # dominant kernel and its roofline
+    name, (tot_ms, cnt) = max(prof.items(), key=lambda kv: kv[1][0])
+    avg_s = tot_ms / cnt / 1e3
+    pairs = nb.occupancy
+    alg = dict(ALG_BYTES_3D)
+    alg["nl_build"] = 12.0 + 4.0 * pairs / N
+    per_launch = alg.get(name, float("nan")) * N
+    achieved = per_launch / avg_s / 1e9
+    kernels = {k: {"ms_total": round(v[0], 4), "launches": v[1],
+                   "us_mean": round(v[0] / v[1] * 1e3, 2)} for k, v in
+               sorted(prof.items(), key=lambda kv: -kv[1][0])}
+    out = {
+        "metric": "particle-steps/s fwd & fwd+adjoint, 3D TGV 262k particles; % of HBM peak",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": el / a.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: 3D TGV lattice initial condition (sin/cos velocity field)",
+        "config": {"workload": f"3D Taylor-Green vortex, {n_side}^3 = {N} particles, tvf=1, "
+                               f"QSK h=dx, Re=1600, c0=10, dt={dt:.6g}, fused si_euler step",
+                   "particles": N, "pairs_per_step": pairs, "dt": dt},
+        "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None,
+                     "alg_bytes_per_launch": per_launch, "us_mean": avg_s * 1e6},
+        "kernels": kernels,
+        "partition_error_code": overflow,
+        "fwd_adjoint": None,
+    }
+    if not a.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(a.cpu_sample_side)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

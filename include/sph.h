@@ -1,0 +1,185 @@
+/*
+ * C-ABI of the MI355X SPH step library (libsph.so, gfx950).
+ *
+ * The reference (tumaer/jax-sph) has no native layer: its hot path is a set of
+ * Python closures compiled by XLA.  Each entry point below replaces one of
+ * those closures and is what a Python/ctypes (or any FFI) binding calls with
+ * raw device pointers from contiguous tensors.
+ *
+ * Conventions
+ *   - every function returns SPH_OK (0) or a negative SPH_ERR_* code;
+ *     sph_last_error() returns a message for the last failure on this thread;
+ *   - arrays are device pointers, float32, particle-major (N, dim) row-major
+ *     like the reference's state dict; tags are int32;
+ *   - the caller owns every state/output buffer and the int64 `info` block;
+ *     the workspace owns its scratch, sized once at sph_ws_create: update,
+ *     forward and step never allocate;
+ *   - all work is enqueued on `stream`; a workspace is not thread-safe
+ *     (one workspace per stream).
+ *
+ * info block (int64[8], device):
+ *   [0] PartitionErrorCode bits      [1] occupancy (real pairs)
+ *   [2] max particles in one cell    [3] max pairs of one particle
+ *   [4] rebuild flag (dr_threshold)  [5] internal: pair-buffer truncations
+ *   [6] chunk overflow count (scan)  [7] reserved
+ */
+#ifndef SPH_H
+#define SPH_H
+
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPH_OK 0
+#define SPH_ERR_ARG (-1)
+#define SPH_ERR_HIP (-2)
+#define SPH_ERR_SIZE (-3)
+
+/* PartitionErrorCode, jax_sph/jax_md/partition.py:434-457 */
+#define SPH_NEIGHBOR_LIST_OVERFLOW 1
+#define SPH_CELL_LIST_OVERFLOW 2
+#define SPH_CELL_SIZE_TOO_SMALL 4
+#define SPH_MALFORMED_BOX 8
+
+/* kernel.py:26-201 */
+enum sph_kernel_kind {
+  SPH_KERNEL_CSK = 0, SPH_KERNEL_QSK = 1, SPH_KERNEL_WC2K = 2, SPH_KERNEL_WC4K = 3,
+  SPH_KERNEL_WC6K = 4, SPH_KERNEL_GK = 5, SPH_KERNEL_SGK = 6
+};
+
+typedef struct sph_ws sph_ws;
+
+/* Geometry of one neighbour search (jax_md/partition.py:114-156, 723-822).
+ * The host derives cps/cell_size with the reference's float32 arithmetic. */
+typedef struct {
+  int32_t dim;            /* 2 or 3 */
+  int32_t n;              /* particles */
+  float box[3];           /* periodic side lengths, float32 */
+  float cutoff;           /* fl32(r_cutoff) + fl32(dr_threshold) */
+  float cutoff_sq;        /* fl32(cutoff * cutoff) */
+  int32_t use_cells;      /* 0: every particle is a candidate (no cell list) */
+  int32_t cps[3];         /* cells per side */
+  float cell_size[3];     /* fl32(box / cps) */
+  int32_t cell_capacity;  /* emulated fixed cell capacity: sets SPH_CELL_LIST_OVERFLOW */
+  int32_t mask_self;      /* drop (i, i) pairs */
+  int32_t ordered;        /* OrderedSparse: keep receiver < sender only */
+  int32_t ell_width;      /* pairs stored per particle; >= 3^dim * cell_capacity */
+} sph_nl_config;
+
+/* WCSPH configuration (solver.py:616-700 plus case constants). */
+typedef struct {
+  int32_t dim;
+  int32_t kernel;            /* enum sph_kernel_kind */
+  int32_t solver;            /* 0 = SPH, 1 = RIE */
+  int32_t density_evolution; /* 0 = summation, 1 = continuity */
+  int32_t bc_trick;
+  int32_t free_slip;
+  int32_t heat_conduction;
+  int32_t eos_kind;          /* 0 = Tait, 1 = RIEMANN */
+  float one_over_h, sigma;   /* fl32(1/h), fl32(kernel normalisation) */
+  float dt, c_ref, eta_limiter, artificial_alpha, av_h;  /* av_h = dx (solver.py:406) */
+  float p_ref, rho_ref, p_bg, gamma, inv_gamma, riemann_k; /* riemann_k = 100 u_ref^2 */
+  float box[3];
+  float g_const[3];          /* body force used when g_ext == NULL */
+  int32_t tvf_stress;        /* 1: evaluate the A = rho u (v-u) term; 0: caller proves v == u */
+  int32_t pad_;
+} sph_solver_params;
+
+/* Input state (orig particle order). nw/T/... may be NULL when unused. */
+typedef struct {
+  const float *r, *u, *v, *dudt, *dvdt, *rho, *p, *mass, *eta;
+  const float *T, *dTdt, *kappa, *Cp, *nw, *drhodt;
+  const float *g_ext;        /* (N, dim) body force, or NULL for g_const */
+  const int32_t *tag;
+} sph_state_in;
+
+/* Outputs of WCSPH.forward (solver.py:930-947); all (orig order). */
+typedef struct {
+  float *u, *v, *dudt, *dvdt, *rho, *p, *T, *dTdt, *drhodt;
+} sph_state_out;
+
+/* Case boundary hook applied after the RHS (integrator.py:54):
+ * 0 none (tgv.py:56-57), 1 lid-driven cavity (ldc.py:130-144),
+ * 2 dam break (db.py:299-307), 3 relaxation walls (case_setup.py:393-402). */
+typedef struct {
+  int32_t kind;
+  float u_lid[3];
+} sph_bc_spec;
+
+/* ---- library ---------------------------------------------------------- */
+const char *sph_last_error(void);
+int sph_version(void);
+
+/* ---- workspace: replaces the buffers XLA materialises per call ---------- */
+/* Allocates the cell arrays, two banks of sorted order + pair buffer, and the
+ * sorted particle records.  Replaces the shape decisions of
+ * NeighborListFns.allocate (jax_md/partition.py:956-971). */
+int sph_ws_create(const sph_nl_config *cfg, sph_ws **out);
+int sph_ws_destroy(sph_ws *ws);
+/* bytes of device memory held by the workspace */
+int64_t sph_ws_bytes(const sph_ws *ws);
+
+/* ---- neighbour search: jax_sph/partition.py:492-571 + jax_md/partition.py:723-1026 */
+/* Max particles in one cell on a grid (count_cell_filling, partition.py:159-174);
+ * writes info[2].  Used by allocate, before a workspace exists, to size the
+ * emulated cell capacity; `counts` is caller scratch of prod(cps) int32. */
+int sph_nl_fill_max(int32_t dim, int32_t n, const float *r, const float *cell_size,
+                    const int32_t *cps, int32_t *counts, int64_t *info, hipStream_t stream);
+/* Rebuild test of update (partition.py:1010-1018): info[4] = any(|r - r_ref|^2 > thr_sq). */
+int sph_nl_needs_rebuild(sph_ws *ws, const float *r, const float *r_ref, float threshold_sq,
+                         int64_t *info, hipStream_t stream);
+/* Cell list + candidate prune (partition.py:294-413, 832-909) into `bank`:
+ * bins r, sorts by cell, stores every pair with fp32 min-image d^2 < cutoff^2,
+ * writes info[0..3,5].  prev_info (nullable) supplies the sticky error bits.
+ * max_occupancy < 0 leaves SPH_NEIGHBOR_LIST_OVERFLOW for sph_nl_finalize. */
+int sph_nl_build(sph_ws *ws, int bank, const float *r, const int64_t *prev_info,
+                 int64_t max_occupancy, int64_t *info, hipStream_t stream);
+/* Sets SPH_NEIGHBOR_LIST_OVERFLOW iff info[1] > max_occupancy (partition.py:976). */
+int sph_nl_finalize(sph_ws *ws, int64_t max_occupancy, int64_t *info, hipStream_t stream);
+/* jaxmd_scan chunk budgets (jax_sph/partition.py:233-291): adds
+ * SPH_CELL_LIST_OVERFLOW when a chunk of n_sub senders holds > budget pairs. */
+int sph_nl_scan_budget(sph_ws *ws, int bank, int64_t n_sub, int64_t budget, int64_t *info,
+                       hipStream_t stream);
+/* Materialises NeighborList.idx: int32[2, max_occupancy], row 0 receivers,
+ * row 1 senders (sender-sorted), padded with N (partition.py:885-909, 971). */
+int sph_nl_write_idx(sph_ws *ws, int bank, int32_t *idx, int64_t max_occupancy,
+                     const int64_t *info, hipStream_t stream);
+
+/* ---- physics ------------------------------------------------------------ */
+/* W(r) and dW/dr of one kernel, elementwise (kernel.py:9-201). */
+int sph_kernel_eval(int32_t kind, int32_t dim, float one_over_h, float sigma, const float *r,
+                    int64_t n, float *w, float *dw, hipStream_t stream);
+/* WCSPH.forward (solver.py:705-949) over the pairs of `bank`. */
+int sph_forward(sph_ws *ws, int bank, const sph_solver_params *prm, const sph_state_in *in,
+                sph_state_out *out, hipStream_t stream);
+/* One si_euler step (integrator.py:22-56) fused with the neighbour update,
+ * forward and the case boundary hook: reads `in`, writes r/u/v to r_out/out
+ * and the rest of the state to `out`; the new pair set goes to `bank`. */
+int sph_step(sph_ws *ws, int bank, const sph_solver_params *prm, float dt, float tvf_coef,
+             const sph_bc_spec *bc, const sph_state_in *in, float *r_out, sph_state_out *out,
+             const int64_t *prev_info, int64_t max_occupancy, int64_t *info,
+             hipStream_t stream);
+
+/* si_euler kick/drift alone (integrator.py:26-30), for callers that compose
+ * the step from separate update/forward calls: u' = u + dt dudt,
+ * v' = u' + tvf_coef dvdt, r' = mod(r + dt v', box). */
+int sph_integrate(int32_t dim, int64_t n, const float *r, const float *u, const float *dudt,
+                  const float *dvdt, float *r_out, float *u_out, float *v_out, float dt,
+                  float tvf_coef, const float *box, hipStream_t stream);
+
+/* ---- profiling hooks ---------------------------------------------------- */
+/* When on, the library records a HIP event pair around every kernel it
+ * launches and accumulates per-kernel device time (ms) and launch counts. */
+int sph_prof_enable(sph_ws *ws, int on);
+/* Copies up to `cap` entries: names (NUL-separated into `names`, size names_cap),
+ * total ms and counts.  Returns the number of kernels. */
+int sph_prof_read(sph_ws *ws, char *names, int names_cap, double *ms, int64_t *count, int cap);
+int sph_prof_reset(sph_ws *ws);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPH_H */

@@ -1,0 +1,587 @@
+// Neighbour search on gfx950: counting-sort cell binning, a wavefront-scan
+// prefix sum, per-cell id sort, and the exact float32 pair predicate.
+//
+// Replaces (all XLA-generated in the reference):
+//   jax_md/partition.py:294-413   cell_list_fn (hash, argsort, slot scatter, occupancy)
+//   jax_md/partition.py:832-909   3^d candidate copies + prune + cumsum compaction
+//   jax_md/partition.py:956-976   capacity and NEIGHBOR_LIST_OVERFLOW
+//   jax_md/partition.py:1010-1018 rebuild test of update
+//   jax_sph/partition.py:208-291  jaxmd_scan chunk budgets
+#include <hip/hip_runtime.h>
+
+#include "sph_device.h"
+#include "sph_internal.h"
+
+namespace sph {
+
+constexpr int kBlock = 256;
+
+__host__ __device__ inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// ------------------------------------------------------------------ helpers
+template <typename T>
+__global__ void k_fill(T* __restrict__ a, int64_t n, T v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+__global__ void k_info_init(int64_t* __restrict__ info, const int64_t* __restrict__ prev) {
+  if (threadIdx.x == 0) {
+    info[0] = prev ? prev[0] : 0;
+    for (int k = 1; k < 8; ++k) info[k] = 0;
+  }
+}
+
+struct Grid {
+  float cs[3];   // cell size (fp32)
+  int cps[3];    // cells per side
+  int use_cells;
+};
+
+// int32(r / cell_size) per dimension (partition.py:367), clamped into the grid:
+// the reference aliases r == side into a wrong cell; clamping keeps it in the last one.
+template <int DIM>
+__device__ __forceinline__ void cell_coords(const float4 p, const Grid& g, int* c) {
+  float x[3] = {p.x, p.y, p.z};
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    int q = (int)(x[d] / g.cs[d]);  // correctly rounded division (build flag)
+    q = q < 0 ? 0 : q;
+    q = q >= g.cps[d] ? g.cps[d] - 1 : q;
+    c[d] = q;
+  }
+}
+
+template <int DIM>
+__device__ __forceinline__ int cell_hash(const int* c, const Grid& g) {
+  int h = c[0] + c[1] * g.cps[0];
+  if (DIM == 3) h += c[2] * g.cps[0] * g.cps[1];
+  return h;
+}
+
+template <int DIM>
+__device__ __forceinline__ float4 load_pos(const float* __restrict__ r, int64_t i) {
+  float4 p;
+  p.x = r[i * DIM + 0];
+  p.y = r[i * DIM + 1];
+  p.z = DIM == 3 ? r[i * DIM + 2] : 0.0f;
+  p.w = 0.0f;
+  return p;
+}
+
+// ----------------------------------------------------------- binning kernels
+template <int DIM>
+__global__ void k_hash_count(const float* __restrict__ r, int n, Grid g,
+                             int32_t* __restrict__ cell_count, int32_t* __restrict__ pcell,
+                             int32_t* __restrict__ pslot) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int h = 0;
+  if (g.use_cells) {
+    int c[3];
+    cell_coords<DIM>(load_pos<DIM>(r, i), g, c);
+    h = cell_hash<DIM>(c, g);
+  }
+  const int slot = atomicAdd(&cell_count[h], 1);
+  if (pcell) {
+    pcell[i] = h;
+    pslot[i] = slot;
+  }
+}
+
+__global__ void k_max_reduce(const int32_t* __restrict__ a, int64_t n, int64_t* __restrict__ out) {
+  int m = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    m = a[i] > m ? a[i] : m;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    int y = __shfl_down(m, o, 64);
+    m = y > m ? y : m;
+  }
+  if ((threadIdx.x & 63) == 0) atomicMax((unsigned long long*)out, (unsigned long long)m);
+}
+
+// Single-workgroup exclusive scan with wavefront (64-lane) inclusive scans.
+// out[k] = sum(in[0:k]) for k in [0, n]; also the max of in (for the cell capacity).
+constexpr int kScanThreads = 1024;
+constexpr int kScanItems = 8;
+
+template <typename TIn, typename TOut>
+__global__ __launch_bounds__(kScanThreads) void k_scan_single(const TIn* __restrict__ in, int64_t n,
+                                                              TOut* __restrict__ out,
+                                                              int64_t* __restrict__ max_out,
+                                                              int64_t* __restrict__ info,
+                                                              int64_t capacity) {
+  __shared__ TOut wave_tot[kScanThreads / 64];
+  __shared__ TOut carry_s;
+  __shared__ int64_t wmax[kScanThreads / 64];
+  const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+  TOut carry = 0;
+  int64_t mx = 0;
+  const int64_t chunk = (int64_t)kScanThreads * kScanItems;
+  for (int64_t base = 0; base < n; base += chunk) {
+    TOut v[kScanItems];
+    TOut s = 0;
+    int64_t i0 = base + (int64_t)t * kScanItems;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+      int64_t i = i0 + k;
+      TOut x = i < n ? (TOut)in[i] : (TOut)0;
+      v[k] = x;
+      s += x;
+      mx = (int64_t)x > mx ? (int64_t)x : mx;
+    }
+    // inclusive wave scan of the per-thread sums
+    TOut inc = s;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      TOut y = __shfl_up(inc, o, 64);
+      if (lane >= o) inc += y;
+    }
+    if (lane == 63) wave_tot[wid] = inc;
+    __syncthreads();
+    if (wid == 0) {
+      TOut w = lane < kScanThreads / 64 ? wave_tot[lane] : (TOut)0;
+      TOut winc = w;
+#pragma unroll
+      for (int o = 1; o < 16; o <<= 1) {
+        TOut y = __shfl_up(winc, o, 64);
+        if (lane >= o) winc += y;
+      }
+      if (lane < kScanThreads / 64) wave_tot[lane] = winc - w;  // exclusive
+      if (lane == kScanThreads / 64 - 1) carry_s = winc;
+    }
+    __syncthreads();
+    TOut run = carry + wave_tot[wid] + (inc - s);
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+      int64_t i = i0 + k;
+      if (i < n) out[i] = run;
+      run += v[k];
+    }
+    carry += carry_s;
+    __syncthreads();
+  }
+  // max reduction
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    int64_t y = __shfl_down(mx, o, 64);
+    mx = y > mx ? y : mx;
+  }
+  if (lane == 0) wmax[wid] = mx;
+  __syncthreads();
+  if (t == 0) {
+    out[n] = carry;
+    int64_t m = 0;
+    for (int w = 0; w < kScanThreads / 64; ++w) m = wmax[w] > m ? wmax[w] : m;
+    if (max_out) *max_out = m;
+    if (info && capacity >= 0 && m > capacity) info[0] |= SPH_CELL_LIST_OVERFLOW;
+  }
+}
+
+__global__ void k_scatter(const int32_t* __restrict__ pcell, const int32_t* __restrict__ pslot,
+                          const int32_t* __restrict__ cell_start, int32_t* __restrict__ unsorted,
+                          int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  unsorted[cell_start[pcell[i]] + pslot[i]] = i;
+}
+
+// One wavefront per cell: rank-sort the ids that arrived in the cell so that a
+// cell lists its particles in ascending id, like the reference's stable argsort.
+__global__ __launch_bounds__(256) void k_cell_sort(const int32_t* __restrict__ cell_start,
+                                                   const int32_t* __restrict__ unsorted,
+                                                   int32_t* __restrict__ perm,
+                                                   int32_t* __restrict__ iperm, int64_t ncell) {
+  const int lane = threadIdx.x & 63;
+  const int64_t cell = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (cell >= ncell) return;
+  const int b = cell_start[cell], e = cell_start[cell + 1], n = e - b;
+  if (n <= 64) {
+    int id = lane < n ? unsorted[b + lane] : 0x7fffffff;
+    int rank = 0;
+    for (int k = 0; k < n; ++k) {
+      int o = __shfl(id, k, 64);
+      rank += (o < id) ? 1 : 0;
+    }
+    if (lane < n) {
+      perm[b + rank] = id;
+      iperm[id] = b + rank;
+    }
+  } else {
+    for (int k0 = lane; k0 < n; k0 += 64) {
+      int id = unsorted[b + k0];
+      int rank = 0;
+      for (int k = 0; k < n; ++k) rank += (unsorted[b + k] < id) ? 1 : 0;
+      perm[b + rank] = id;
+      iperm[id] = b + rank;
+    }
+  }
+}
+
+template <int DIM>
+__global__ void k_gather_pos(const float* __restrict__ r, const int32_t* __restrict__ perm,
+                             float4* __restrict__ pos, int n) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  pos[s] = load_pos<DIM>(r, perm[s]);
+}
+
+// ----------------------------------------------------------- pair build
+struct BuildArgs {
+  const float4* pos;
+  const int32_t* cell_start;
+  const int32_t* perm;
+  int32_t* ell;
+  int32_t* cnt;
+  int64_t* info;
+  Grid g;
+  float box[3], half[3];
+  float cutoff_sq;
+  int n, npad, width, mask_self;
+};
+
+template <int DIM, bool ORDERED>
+__device__ __forceinline__ void test_range(const BuildArgs& a, int s, const float4 ps, int sid,
+                                           int lo, int hi, int& count) {
+  for (int b = lo; b < hi; ++b) {
+    if (a.mask_self && b == s) continue;
+    if (ORDERED && !(a.perm[b] < sid)) continue;
+    float d2 = metric_sq_exact<DIM>(ps, a.pos[b], a.box, a.half);
+    if (d2 < a.cutoff_sq) {
+      if (count < a.width) a.ell[(int64_t)count * a.npad + s] = b;
+      ++count;
+    }
+  }
+}
+
+// One thread per sorted particle (the sender): visit the 3^d cells around its
+// own cell as 3^(d-1) contiguous x-strips and keep every receiver with
+// d^2 < cutoff^2.  Pairs land in a column-major per-particle buffer.
+template <int DIM, bool ORDERED>
+__global__ __launch_bounds__(kBlock) void k_nl_build(BuildArgs a) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  int count = 0;
+  if (s < a.n) {
+    const float4 ps = a.pos[s];
+    const int sid = ORDERED ? a.perm[s] : 0;
+    if (!a.g.use_cells) {
+      test_range<DIM, ORDERED>(a, s, ps, sid, 0, a.n, count);
+    } else {
+      int c[3];
+      cell_coords<DIM>(ps, a.g, c);
+      const int nx = a.g.cps[0], ny = a.g.cps[1];
+      const int nz = DIM == 3 ? a.g.cps[2] : 1;
+      const int z0 = DIM == 3 ? -1 : 0, z1 = DIM == 3 ? 1 : 0;
+      for (int dz = z0; dz <= z1; ++dz) {
+        int cz = DIM == 3 ? c[2] + dz : 0;
+        cz = cz < 0 ? cz + nz : (cz >= nz ? cz - nz : cz);
+        for (int dy = -1; dy <= 1; ++dy) {
+          int cy = c[1] + dy;
+          cy = cy < 0 ? cy + ny : (cy >= ny ? cy - ny : cy);
+          const int row = (cz * ny + cy) * nx;
+          if (c[0] >= 1 && c[0] + 1 < nx) {
+            test_range<DIM, ORDERED>(a, s, ps, sid, a.cell_start[row + c[0] - 1],
+                                     a.cell_start[row + c[0] + 2], count);
+          } else {
+            for (int dx = -1; dx <= 1; ++dx) {
+              int cx = c[0] + dx;
+              cx = cx < 0 ? cx + nx : (cx >= nx ? cx - nx : cx);
+              test_range<DIM, ORDERED>(a, s, ps, sid, a.cell_start[row + cx],
+                                       a.cell_start[row + cx + 1], count);
+            }
+          }
+        }
+      }
+    }
+    a.cnt[s] = count < a.width ? count : a.width;
+    if (count > a.width) atomicAdd((unsigned long long*)&a.info[5], 1ull);
+  }
+  // block reductions: total pairs and the largest per-particle count
+  __shared__ unsigned long long tot[kBlock / 64];
+  __shared__ int mx[kBlock / 64];
+  unsigned long long v = (unsigned long long)count;
+  int m = count;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    v += __shfl_down(v, o, 64);
+    int y = __shfl_down(m, o, 64);
+    m = y > m ? y : m;
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) { tot[wid] = v; mx[wid] = m; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    int mm = 0;
+    for (int w = 0; w < kBlock / 64; ++w) { t += tot[w]; mm = mx[w] > mm ? mx[w] : mm; }
+    if (t) atomicAdd((unsigned long long*)&a.info[1], t);
+    atomicMax((unsigned long long*)&a.info[3], (unsigned long long)mm);
+  }
+}
+
+__global__ void k_finalize(int64_t* __restrict__ info, int64_t max_occ) {
+  if (threadIdx.x == 0 && info[1] > max_occ) info[0] |= SPH_NEIGHBOR_LIST_OVERFLOW;
+}
+
+template <int DIM>
+__global__ void k_needs_rebuild(const float* __restrict__ r, const float* __restrict__ ref, int n,
+                                Grid g, float bx, float by, float bz, float thr_sq,
+                                int64_t* __restrict__ info) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float box[3] = {bx, by, bz};
+  float half[3] = {bx * 0.5f, by * 0.5f, bz * 0.5f};
+  float d2 = metric_sq_exact<DIM>(load_pos<DIM>(r, i), load_pos<DIM>(ref, i), box, half);
+  if (d2 > thr_sq) atomicOr((unsigned long long*)&info[4], 1ull);
+}
+
+// ----------------------------------------------------------- idx writer
+__global__ void k_counts_orig(const int32_t* __restrict__ perm, const int32_t* __restrict__ cnt,
+                              int64_t* __restrict__ cnt64, int n) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < n) cnt64[perm[s]] = cnt[s];
+}
+
+__global__ void k_write_idx(const int32_t* __restrict__ perm, const int32_t* __restrict__ cnt,
+                            const int32_t* __restrict__ ell, const int64_t* __restrict__ off,
+                            int32_t* __restrict__ idx, int64_t max_occ, int n, int npad) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const int a = perm[s];
+  const int64_t base = off[a];
+  const int c = cnt[s];
+  for (int k = 0; k < c; ++k) {
+    int64_t p = base + k;
+    if (p >= max_occ) break;
+    idx[p] = perm[ell[(int64_t)k * npad + s]];
+    idx[max_occ + p] = a;
+  }
+}
+
+__global__ void k_pad_idx(int32_t* __restrict__ idx, int64_t max_occ,
+                          const int64_t* __restrict__ total, int n) {
+  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < max_occ && p >= *total) {
+    idx[p] = n;
+    idx[max_occ + p] = n;
+  }
+}
+
+__global__ void k_chunk_budget(const int64_t* __restrict__ off, int n, int64_t n_sub,
+                               int64_t budget, int64_t* __restrict__ info) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t lo = k * n_sub;
+  if (lo >= n) return;
+  int64_t hi = lo + n_sub < n ? lo + n_sub : n;
+  if (off[hi] - off[lo] > budget) {
+    atomicOr((unsigned long long*)&info[0], (unsigned long long)SPH_CELL_LIST_OVERFLOW);
+    atomicAdd((unsigned long long*)&info[6], 1ull);
+  }
+}
+
+Grid make_grid(const sph_nl_config& c) {
+  Grid g;
+  for (int d = 0; d < 3; ++d) {
+    g.cs[d] = c.cell_size[d];
+    g.cps[d] = c.cps[d] > 0 ? c.cps[d] : 1;
+  }
+  g.use_cells = c.use_cells;
+  return g;
+}
+
+// Bins r into `bank`: hash + histogram, scan, scatter, per-cell sort, positions.
+int bin_particles(sph_ws* ws, int bank, const float* r, const int64_t* prev_info, int64_t* info,
+                  hipStream_t st) {
+  const auto& c = ws->cfg;
+  const int n = c.n;
+  Bank& B = ws->bank[bank];
+  Grid g = make_grid(c);
+  SPH_LAUNCH(ws, "info_init", st, k_info_init<<<1, 64, 0, st>>>(info, prev_info));
+  SPH_LAUNCH(ws, "fill", st,
+             k_fill<int32_t><<<cdiv(ws->ncell, kBlock), kBlock, 0, st>>>(ws->cell_count,
+                                                                        ws->ncell, 0));
+  if (c.dim == 2)
+    SPH_LAUNCH(ws, "hash_count", st,
+               k_hash_count<2><<<cdiv(n, kBlock), kBlock, 0, st>>>(r, n, g, ws->cell_count,
+                                                                 ws->pcell, ws->pslot));
+  else
+    SPH_LAUNCH(ws, "hash_count", st,
+               k_hash_count<3><<<cdiv(n, kBlock), kBlock, 0, st>>>(r, n, g, ws->cell_count,
+                                                                 ws->pcell, ws->pslot));
+  SPH_LAUNCH(ws, "scan_cells", st,
+             (k_scan_single<int32_t, int32_t><<<1, kScanThreads, 0, st>>>(
+                 ws->cell_count, ws->ncell, B.cell_start, &info[2], info,
+                 c.use_cells ? (int64_t)c.cell_capacity : (int64_t)-1)));
+  SPH_LAUNCH(ws, "scatter", st,
+             k_scatter<<<cdiv(n, kBlock), kBlock, 0, st>>>(ws->pcell, ws->pslot, B.cell_start,
+                                                         ws->unsorted, n));
+  SPH_LAUNCH(ws, "cell_sort", st,
+             k_cell_sort<<<cdiv(ws->ncell * 64, 256), 256, 0, st>>>(B.cell_start, ws->unsorted,
+                                                                    B.perm, B.iperm, ws->ncell));
+  if (c.dim == 2)
+    SPH_LAUNCH(ws, "gather_pos", st,
+               k_gather_pos<2><<<cdiv(n, kBlock), kBlock, 0, st>>>(r, B.perm, B.pos, n));
+  else
+    SPH_LAUNCH(ws, "gather_pos", st,
+               k_gather_pos<3><<<cdiv(n, kBlock), kBlock, 0, st>>>(r, B.perm, B.pos, n));
+  return SPH_OK;
+}
+
+int build_pairs(sph_ws* ws, int bank, int64_t max_occupancy, int64_t* info, hipStream_t st) {
+  const auto& c = ws->cfg;
+  Bank& B = ws->bank[bank];
+  BuildArgs a;
+  a.pos = B.pos;
+  a.cell_start = B.cell_start;
+  a.perm = B.perm;
+  a.ell = B.ell;
+  a.cnt = B.cnt;
+  a.info = info;
+  a.g = make_grid(c);
+  for (int d = 0; d < 3; ++d) {
+    a.box[d] = c.box[d];
+    a.half[d] = c.box[d] * 0.5f;
+  }
+  a.cutoff_sq = c.cutoff_sq;
+  a.n = c.n;
+  a.npad = ws->npad;
+  a.width = c.ell_width;
+  a.mask_self = c.mask_self;
+  dim3 grid(cdiv(c.n, kBlock));
+  if (c.dim == 2) {
+    if (c.ordered) SPH_LAUNCH(ws, "nl_build", st, (k_nl_build<2, true><<<grid, kBlock, 0, st>>>(a)));
+    else SPH_LAUNCH(ws, "nl_build", st, (k_nl_build<2, false><<<grid, kBlock, 0, st>>>(a)));
+  } else {
+    if (c.ordered) SPH_LAUNCH(ws, "nl_build", st, (k_nl_build<3, true><<<grid, kBlock, 0, st>>>(a)));
+    else SPH_LAUNCH(ws, "nl_build", st, (k_nl_build<3, false><<<grid, kBlock, 0, st>>>(a)));
+  }
+  if (max_occupancy >= 0)
+    SPH_LAUNCH(ws, "finalize", st, k_finalize<<<1, 64, 0, st>>>(info, max_occupancy));
+  return SPH_OK;
+}
+
+// pair counts per original id and their exclusive scan (off64[0..N])
+int pair_offsets(sph_ws* ws, int bank, hipStream_t st) {
+  const int n = ws->cfg.n;
+  Bank& B = ws->bank[bank];
+  SPH_LAUNCH(ws, "counts_orig", st,
+             k_counts_orig<<<cdiv(n, kBlock), kBlock, 0, st>>>(B.perm, B.cnt, ws->cnt64, n));
+  SPH_LAUNCH(ws, "scan_pairs", st,
+             (k_scan_single<int64_t, int64_t><<<1, kScanThreads, 0, st>>>(
+                 ws->cnt64, n, ws->off64, nullptr, nullptr, -1)));
+  return SPH_OK;
+}
+
+}  // namespace sph
+
+using namespace sph;
+
+static int check_ws(sph_ws* ws, int bank) {
+  if (!ws) { set_error("null workspace"); return SPH_ERR_ARG; }
+  if (bank < 0 || bank > 1) { set_error("bank must be 0 or 1"); return SPH_ERR_ARG; }
+  return SPH_OK;
+}
+
+extern "C" int sph_nl_fill_max(int32_t dim, int32_t n, const float* r, const float* cell_size,
+                               const int32_t* cps, int32_t* counts, int64_t* info,
+                               hipStream_t st) {
+  if (!r || !cell_size || !cps || !counts || !info || n < 0 || (dim != 2 && dim != 3)) {
+    set_error("sph_nl_fill_max: bad argument");
+    return SPH_ERR_ARG;
+  }
+  Grid g;
+  int64_t nc = 1;
+  for (int d = 0; d < 3; ++d) {
+    g.cs[d] = d < dim ? cell_size[d] : 1.0f;
+    g.cps[d] = d < dim ? cps[d] : 1;
+    if (g.cps[d] < 1) { set_error("cps must be >= 1"); return SPH_ERR_ARG; }
+    nc *= g.cps[d];
+  }
+  g.use_cells = 1;
+  sph_ws* ws = nullptr;  // no profiling without a workspace
+  SPH_LAUNCH(ws, "fill", st, k_fill<int32_t><<<cdiv(nc, kBlock), kBlock, 0, st>>>(counts, nc, 0));
+  SPH_LAUNCH(ws, "fill", st, k_fill<int64_t><<<1, 64, 0, st>>>(info + 2, 1, 0));
+  if (n > 0) {
+    if (dim == 2)
+      SPH_LAUNCH(ws, "hash_count", st,
+                 k_hash_count<2><<<cdiv(n, kBlock), kBlock, 0, st>>>(r, n, g, counts, nullptr,
+                                                                   nullptr));
+    else
+      SPH_LAUNCH(ws, "hash_count", st,
+                 k_hash_count<3><<<cdiv(n, kBlock), kBlock, 0, st>>>(r, n, g, counts, nullptr,
+                                                                   nullptr));
+  }
+  int64_t blocks = cdiv(nc, kBlock);
+  blocks = blocks > 1024 ? 1024 : blocks;
+  SPH_LAUNCH(ws, "max_reduce", st, k_max_reduce<<<blocks, kBlock, 0, st>>>(counts, nc, info + 2));
+  return SPH_OK;
+}
+
+extern "C" int sph_nl_needs_rebuild(sph_ws* ws, const float* r, const float* r_ref,
+                                    float threshold_sq, int64_t* info, hipStream_t st) {
+  if (!ws || !r || !r_ref || !info) { set_error("sph_nl_needs_rebuild: null"); return SPH_ERR_ARG; }
+  const auto& c = ws->cfg;
+  Grid g = make_grid(c);
+  SPH_LAUNCH(ws, "fill", st, k_fill<int64_t><<<1, 64, 0, st>>>(info + 4, 1, 0));
+  if (c.dim == 2)
+    SPH_LAUNCH(ws, "needs_rebuild", st,
+               k_needs_rebuild<2><<<cdiv(c.n, kBlock), kBlock, 0, st>>>(
+                   r, r_ref, c.n, g, c.box[0], c.box[1], c.box[2], threshold_sq, info));
+  else
+    SPH_LAUNCH(ws, "needs_rebuild", st,
+               k_needs_rebuild<3><<<cdiv(c.n, kBlock), kBlock, 0, st>>>(
+                   r, r_ref, c.n, g, c.box[0], c.box[1], c.box[2], threshold_sq, info));
+  return SPH_OK;
+}
+
+extern "C" int sph_nl_build(sph_ws* ws, int bank, const float* r, const int64_t* prev_info,
+                            int64_t max_occupancy, int64_t* info, hipStream_t st) {
+  int rc = check_ws(ws, bank);
+  if (rc) return rc;
+  if (!r || !info) { set_error("sph_nl_build: null"); return SPH_ERR_ARG; }
+  rc = bin_particles(ws, bank, r, prev_info, info, st);
+  if (rc) return rc;
+  return build_pairs(ws, bank, max_occupancy, info, st);
+}
+
+extern "C" int sph_nl_finalize(sph_ws* ws, int64_t max_occupancy, int64_t* info, hipStream_t st) {
+  if (!ws || !info) { set_error("sph_nl_finalize: null"); return SPH_ERR_ARG; }
+  SPH_LAUNCH(ws, "finalize", st, k_finalize<<<1, 64, 0, st>>>(info, max_occupancy));
+  return SPH_OK;
+}
+
+extern "C" int sph_nl_scan_budget(sph_ws* ws, int bank, int64_t n_sub, int64_t budget,
+                                  int64_t* info, hipStream_t st) {
+  int rc = check_ws(ws, bank);
+  if (rc) return rc;
+  if (n_sub <= 0 || !info) { set_error("sph_nl_scan_budget: bad argument"); return SPH_ERR_ARG; }
+  rc = pair_offsets(ws, bank, st);
+  if (rc) return rc;
+  int64_t chunks = cdiv(ws->cfg.n, n_sub);
+  SPH_LAUNCH(ws, "chunk_budget", st,
+             k_chunk_budget<<<cdiv(chunks, kBlock), kBlock, 0, st>>>(ws->off64, ws->cfg.n, n_sub,
+                                                                     budget, info));
+  return SPH_OK;
+}
+
+extern "C" int sph_nl_write_idx(sph_ws* ws, int bank, int32_t* idx, int64_t max_occupancy,
+                                const int64_t* info, hipStream_t st) {
+  int rc = check_ws(ws, bank);
+  if (rc) return rc;
+  if (!idx || max_occupancy < 0) { set_error("sph_nl_write_idx: bad argument"); return SPH_ERR_ARG; }
+  if (max_occupancy == 0) return SPH_OK;
+  rc = pair_offsets(ws, bank, st);
+  if (rc) return rc;
+  const int n = ws->cfg.n;
+  Bank& B = ws->bank[bank];
+  SPH_LAUNCH(ws, "write_idx", st,
+             k_write_idx<<<cdiv(n, kBlock), kBlock, 0, st>>>(B.perm, B.cnt, B.ell, ws->off64, idx,
+                                                           max_occupancy, n, ws->npad));
+  SPH_LAUNCH(ws, "pad_idx", st,
+             k_pad_idx<<<cdiv(max_occupancy, kBlock), kBlock, 0, st>>>(idx, max_occupancy,
+                                                                      ws->off64 + n, n));
+  (void)info;
+  return SPH_OK;
+}

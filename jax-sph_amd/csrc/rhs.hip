@@ -1,0 +1,714 @@
+// WCSPH right-hand side and the fused si_euler step on gfx950.
+//
+// Particle data are gathered once per step into cell-sorted float4 records;
+// every sweep is a gather over the particle's own pair list (no atomics, a
+// fixed neighbour order, so results are bitwise reproducible run to run).
+//
+// Replaces solver.py:705-949 (WCSPH.forward), integrator.py:22-56 (si_euler)
+// and the per-step case hooks tgv.py:56-57, ldc.py:130-144, db.py:299-307.
+#include <hip/hip_runtime.h>
+
+#include "sph_device.h"
+#include "sph_internal.h"
+
+namespace sph {
+
+constexpr int kB = 256;
+static inline int64_t cdivl(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Bank helpers from nl.hip
+int bin_particles(sph_ws* ws, int bank, const float* r, const int64_t* prev_info, int64_t* info,
+                  hipStream_t st);
+int build_pairs(sph_ws* ws, int bank, int64_t max_occupancy, int64_t* info, hipStream_t st);
+
+// ---------------------------------------------------------- kernel dispatch
+// KIND >= 0: compile-time kernel; KIND == -1: runtime switch over the rest.
+template <int KIND, int DIM>
+struct KernelEval {
+  int kind;
+  float s, ih;
+  __device__ __forceinline__ float w(float q) const { return Kern<KIND, DIM>::w(q, s); }
+  __device__ __forceinline__ float dw(float q) const { return Kern<KIND, DIM>::dw(q, s, ih); }
+};
+
+template <int DIM>
+struct KernelEval<-1, DIM> {
+  int kind;
+  float s, ih;
+  __device__ __forceinline__ float w(float q) const {
+    switch (kind) {
+      case SPH_KERNEL_CSK: return Kern<SPH_KERNEL_CSK, DIM>::w(q, s);
+      case SPH_KERNEL_QSK: return Kern<SPH_KERNEL_QSK, DIM>::w(q, s);
+      case SPH_KERNEL_WC2K: return Kern<SPH_KERNEL_WC2K, DIM>::w(q, s);
+      case SPH_KERNEL_WC4K: return Kern<SPH_KERNEL_WC4K, DIM>::w(q, s);
+      case SPH_KERNEL_WC6K: return Kern<SPH_KERNEL_WC6K, DIM>::w(q, s);
+      case SPH_KERNEL_GK: return Kern<SPH_KERNEL_GK, DIM>::w(q, s);
+      default: return Kern<SPH_KERNEL_SGK, DIM>::w(q, s);
+    }
+  }
+  __device__ __forceinline__ float dw(float q) const {
+    switch (kind) {
+      case SPH_KERNEL_CSK: return Kern<SPH_KERNEL_CSK, DIM>::dw(q, s, ih);
+      case SPH_KERNEL_QSK: return Kern<SPH_KERNEL_QSK, DIM>::dw(q, s, ih);
+      case SPH_KERNEL_WC2K: return Kern<SPH_KERNEL_WC2K, DIM>::dw(q, s, ih);
+      case SPH_KERNEL_WC4K: return Kern<SPH_KERNEL_WC4K, DIM>::dw(q, s, ih);
+      case SPH_KERNEL_WC6K: return Kern<SPH_KERNEL_WC6K, DIM>::dw(q, s, ih);
+      case SPH_KERNEL_GK: return Kern<SPH_KERNEL_GK, DIM>::dw(q, s, ih);
+      default: return Kern<SPH_KERNEL_SGK, DIM>::dw(q, s, ih);
+    }
+  }
+};
+
+// --------------------------------------------------------------- vectors
+struct V3 { float x, y, z; };
+__device__ __forceinline__ V3 v3(float4 a) { return V3{a.x, a.y, a.z}; }
+__device__ __forceinline__ V3 operator-(V3 a, V3 b) { return V3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ V3 operator+(V3 a, V3 b) { return V3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ V3 operator*(float s, V3 a) { return V3{s * a.x, s * a.y, s * a.z}; }
+template <int DIM>
+__device__ __forceinline__ float dot(V3 a, V3 b) {
+  float d = a.x * b.x + a.y * b.y;
+  if (DIM == 3) d += a.z * b.z;
+  return d;
+}
+
+struct Geo {
+  float box[3], inv[3];
+};
+
+template <int DIM>
+__device__ __forceinline__ V3 min_image(float4 a, float4 b, const Geo& g) {
+  V3 d;
+  d.x = disp_fast(a.x, b.x, g.box[0], g.inv[0]);
+  d.y = disp_fast(a.y, b.y, g.box[1], g.inv[1]);
+  d.z = DIM == 3 ? disp_fast(a.z, b.z, g.box[2], g.inv[2]) : 0.0f;
+  return d;
+}
+
+template <int DIM>
+__device__ __forceinline__ V3 load_vec(const float* __restrict__ a, int64_t i) {
+  V3 v;
+  v.x = a[i * DIM];
+  v.y = a[i * DIM + 1];
+  v.z = DIM == 3 ? a[i * DIM + 2] : 0.0f;
+  return v;
+}
+
+template <int DIM>
+__device__ __forceinline__ void store_vec(float* __restrict__ a, int64_t i, V3 v) {
+  a[i * DIM] = v.x;
+  a[i * DIM + 1] = v.y;
+  if (DIM == 3) a[i * DIM + 2] = v.z;
+}
+
+__device__ __forceinline__ float4 f4(V3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
+
+// ------------------------------------------------------------- arguments
+struct Recs {
+  float4 *P, *U0, *V0, *Q0, *NW, *U1, *Q1, *U2, *V2, *Q2, *UT;
+  float* drho;
+};
+
+struct SweepArgs {
+  sph_solver_params prm;
+  sph_state_in in;
+  sph_state_out out;
+  Recs R;
+  const int32_t* perm;
+  const int32_t* ell;
+  const int32_t* cnt;
+  Geo geo;
+  int n, npad;
+  // which record set the force sweep reads
+  const float4 *Uf, *Vf, *Qf;
+  int heat_in_density;  // apply T += dt*dTdt in the density epilogue
+  sph_bc_spec bc;
+};
+
+// ------------------------------------------------------------- gather
+// Original-order state -> cell-sorted records (P, U0, V0, Q0, NW).
+template <int DIM>
+__global__ void k_gather_state(SweepArgs a, const float* __restrict__ r, const float* __restrict__ u,
+                               const float* __restrict__ v, float4* __restrict__ nl_pos) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.n) return;
+  const int i = a.perm[s];
+  const sph_state_in& in = a.in;
+  V3 rr = load_vec<DIM>(r, i);
+  a.R.P[s] = f4(rr, in.mass[i]);
+  if (nl_pos) nl_pos[s] = f4(rr, 0.0f);
+  a.R.U0[s] = f4(load_vec<DIM>(u, i), in.rho[i]);
+  a.R.V0[s] = f4(load_vec<DIM>(v, i), in.kappa ? in.kappa[i] : 0.0f);
+  a.R.Q0[s] = make_float4(in.p[i], in.eta[i], (float)in.tag[i], in.T ? in.T[i] : 0.0f);
+  if (in.nw || in.Cp) {
+    V3 n = in.nw ? load_vec<DIM>(in.nw, i) : V3{0.f, 0.f, 0.f};
+    a.R.NW[s] = f4(n, in.Cp ? in.Cp[i] : 0.0f);
+  }
+}
+
+template <int DIM>
+__device__ __forceinline__ V3 g_ext_of(const SweepArgs& a, int i) {
+  if (a.in.g_ext) return load_vec<DIM>(a.in.g_ext, i);
+  return V3{a.prm.g_const[0], a.prm.g_const[1], a.prm.g_const[2]};
+}
+
+// ------------------------------------------------------------- density
+// MODE 0: summation (solver.py:176-178, 794-796)
+// MODE 1: SPH continuity (solver.py:24-30, 750-756)
+// MODE 2: Riemann continuity (solver.py:108-164, 772-793)
+// plus the Riemann wall velocity u_tilde (solver.py:547-552) for wall targets,
+// then the EOS (solver.py:801).
+template <int DIM, int KIND, int MODE>
+__global__ __launch_bounds__(kB) void k_density(SweepArgs a) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.n) return;
+  const sph_solver_params& p = a.prm;
+  const KernelEval<KIND, DIM> K{p.kernel, p.sigma, p.one_over_h};
+  const Eos eos = make_eos(p);
+  const float4 Pi = a.R.P[s], Ui = a.R.U0[s], Qi = a.R.Q0[s];
+  const int tag_i = (int)Qi.z;
+  const bool fluid_i = tag_i == kFluid;
+  const bool want_ut = p.solver == 1 && p.bc_trick && !p.free_slip && is_wall(tag_i);
+  const int c = a.cnt[s];
+  const V3 ui = v3(Ui);
+  float wsum = 0.0f, acc = 0.0f, utw = 0.0f;
+  V3 ut = {0.f, 0.f, 0.f};
+  V3 gi = {0.f, 0.f, 0.f};
+  float4 NWi = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (MODE == 2) gi = g_ext_of<DIM>(a, a.perm[s]);
+  for (int k = 0; k < c; ++k) {
+    const int j = a.ell[(int64_t)k * a.npad + s];
+    const float4 Pj = a.R.P[j];
+    const V3 dr = min_image<DIM>(Pi, Pj, a.geo);
+    const float d2 = dot<DIM>(dr, dr);
+    const float d = sqrtf(d2);
+    const float q = d * p.one_over_h;
+    if (MODE == 0) {
+      wsum += K.w(q);
+    } else {
+      const float4 Uj = a.R.U0[j];
+      const float gw = K.dw(q);
+      const V3 e = (1.0f / (d + kEps)) * dr;
+      const V3 kg = gw * e;
+      if (MODE == 1) {
+        const float vj = Pj.w / Uj.w;
+        acc += vj * dot<DIM>(ui - v3(Uj), kg);
+      } else {
+        const float4 Qj = a.R.Q0[j];
+        const bool wall_j = is_wall((int)Qj.z);
+        const V3 uj = v3(Uj);
+        float uL, uR, pR, rhoR;
+        const float pL = Qi.x, rhoL = Ui.w;
+        const V3 me = (-1.0f) * e;
+        if (wall_j) {
+          const float4 Nj = a.R.NW[j];
+          const V3 nj = v3(Nj);
+          uL = -dot<DIM>(ui, nj);
+          uR = -uL + 2.0f * dot<DIM>(uj, nj);
+          pR = pL + rhoL * dot<DIM>(gi, (-1.0f) * dr);
+          rhoR = eos.rho_fn(pR);
+        } else {
+          uL = dot<DIM>(ui, me);
+          uR = dot<DIM>(uj, me);
+          pR = Qj.x;
+          rhoR = Uj.w;
+        }
+        const float Uavg = (uL + uR) * 0.5f;
+        const V3 vavg = 0.5f * (ui + uj);
+        const float rhoavg = (rhoL + rhoR) * 0.5f;
+        const float Ustar = Uavg + 0.5f * (pL - pR) / (rhoavg * p.c_ref);
+        const V3 vstar = Ustar * me + (vavg - Uavg * me);
+        acc += 2.0f * Ui.w * Pj.w / Uj.w * dot<DIM>(ui - vstar, kg);
+      }
+    }
+    if (want_ut) {
+      const float4 Qj = a.R.Q0[j];
+      if ((int)Qj.z == kFluid) {
+        const float w = K.w(q);
+        utw += w;
+        ut = ut + w * v3(a.R.U0[j]);
+      }
+    }
+  }
+  float rho = Ui.w, drhodt;
+  const int i = a.perm[s];
+  if (MODE == 0) {
+    if (fluid_i) rho = Pi.w * wsum;
+    drhodt = a.in.drhodt ? a.in.drhodt[i] : 0.0f;
+  } else if (MODE == 1) {
+    drhodt = Ui.w * acc;
+    rho = Ui.w + p.dt * drhodt;
+  } else {
+    drhodt = acc * (fluid_i ? 1.0f : 0.0f);
+    rho = Ui.w + p.dt * drhodt;
+  }
+  const float pr = eos.p_fn(rho);
+  float T = Qi.w;
+  if (a.heat_in_density) T = T + p.dt * a.in.dTdt[i];
+  a.R.U1[s] = make_float4(Ui.x, Ui.y, Ui.z, rho);
+  a.R.Q1[s] = make_float4(pr, Qi.y, Qi.z, T);
+  a.R.drho[s] = drhodt;
+  if (p.solver == 1 && p.bc_trick) {
+    // u_tilde of every particle: Shepard average for no-slip walls, u otherwise
+    // (only wall neighbours read it, solver.py:378)
+    V3 o = want_ut ? (1.0f / (utw + kEps)) * ut : ui;
+    a.R.UT[s] = f4(o, 0.0f);
+  }
+}
+
+// ------------------------------------------------------------- walls
+// MODE 0: generalized wall BC of the SPH solver (solver.py:431-530)
+// MODE 1: heat BC of the Riemann solver (solver.py:554-565)
+// followed by T += dt * dTdt when heat conduction is on (solver.py:834).
+template <int DIM, int KIND, int MODE>
+__global__ __launch_bounds__(kB) void k_walls(SweepArgs a) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.n) return;
+  const sph_solver_params& p = a.prm;
+  const KernelEval<KIND, DIM> K{p.kernel, p.sigma, p.one_over_h};
+  const Eos eos = make_eos(p);
+  const float4 Pi = a.R.P[s], Ui = a.R.U1[s], Vi = a.R.V0[s], Qi = a.R.Q1[s];
+  const int tag_i = (int)Qi.z;
+  const int i = a.perm[s];
+  V3 u = v3(Ui), v = v3(Vi);
+  float pr = Qi.x, T = Qi.w;
+  if (is_wall(tag_i)) {
+    float ws = 0.f, sp = 0.f, sT = 0.f;
+    V3 su = {0.f, 0.f, 0.f}, sv = {0.f, 0.f, 0.f}, srr = {0.f, 0.f, 0.f};
+    const int c = a.cnt[s];
+    for (int k = 0; k < c; ++k) {
+      const int j = a.ell[(int64_t)k * a.npad + s];
+      const float4 Qj = a.R.Q1[j];
+      if ((int)Qj.z != kFluid) continue;
+      const float4 Pj = a.R.P[j];
+      const V3 dr = min_image<DIM>(Pi, Pj, a.geo);
+      const float d = sqrtf(dot<DIM>(dr, dr));
+      const float w = K.w(d * p.one_over_h);
+      ws += w;
+      if (MODE == 0) {
+        const float4 Uj = a.R.U1[j], Vj = a.R.V0[j];
+        su = su + w * v3(Uj);
+        sv = sv + w * v3(Vj);
+        sp += w * Qj.x;
+        srr = srr + (Uj.w * w) * dr;
+      }
+      if (p.heat_conduction) sT += w * Qj.w;
+    }
+    const float inv = 1.0f / (ws + kEps);
+    if (MODE == 0) {
+      V3 uw = inv * su, vw = inv * sv;
+      if (p.free_slip) {
+        const V3 n = (-1.0f) * v3(a.R.NW[s]);
+        uw = dot<DIM>(uw, n) * n;
+        vw = dot<DIM>(vw, n) * n;
+      }
+      u = 2.0f * u - uw;
+      v = 2.0f * v - vw;
+      const V3 g = g_ext_of<DIM>(a, i);
+      pr = (sp + dot<DIM>(g, srr)) * inv;
+    }
+    if (p.heat_conduction && (tag_i == kSolidWall || tag_i == kMovingWall)) T = sT * inv;
+  }
+  const float rho = MODE == 0 ? eos.rho_fn(pr) : Ui.w;
+  if (p.heat_conduction) T = T + p.dt * a.in.dTdt[i];
+  a.R.U2[s] = f4(u, rho);
+  a.R.V2[s] = f4(v, Vi.w);
+  a.R.Q2[s] = make_float4(pr, Qi.y, Qi.z, T);
+}
+
+// ------------------------------------------------------------- forces
+// SOLVER 0: acceleration_standard_fn (solver.py:221-256) + transport velocity
+//           (solver.py:199-213); TVF: include the A = rho u (v - u) stress.
+// SOLVER 1: acceleration_fn_riemann (solver.py:316-401; its A term is zero).
+// Plus artificial viscosity (solver.py:404-428), heat (solver.py:591-610),
+// dudt + g_ext (solver.py:936) and the case boundary hook (integrator.py:54).
+template <int DIM, int KIND, int SOLVER, bool TVF>
+__global__ __launch_bounds__(kB) void k_force(SweepArgs a) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.n) return;
+  const sph_solver_params& p = a.prm;
+  const KernelEval<KIND, DIM> K{p.kernel, p.sigma, p.one_over_h};
+  const Eos eos = make_eos(p);
+  const float4 Pi = a.R.P[s], Ui = a.Uf[s], Vi = a.Vf[s], Qi = a.Qf[s];
+  const int i = a.perm[s];
+  const int tag_i = (int)Qi.z;
+  const float mi = Pi.w, rhoi = Ui.w, pi_ = Qi.x, etai = Qi.y, Ti = Qi.w;
+  const float inv_mi = 1.0f / mi;
+  const float Vi2 = (mi / rhoi) * (mi / rhoi);
+  const V3 ui = v3(Ui), vi = v3(Vi);
+  const V3 Ai = rhoi * ui;          // rho_i u_i
+  const V3 wi = vi - ui;            // v_i - u_i
+  const float pbg = eos.p_fn(0.0f);  // background pressure (solver.py:802)
+  const bool av = p.artificial_alpha != 0.0f;
+  const bool fluid_i = tag_i == kFluid;
+  const bool heat = p.heat_conduction != 0;
+  const float kappa_i = Vi.w;
+  const float Cp_i = heat ? a.R.NW[s].w : 1.0f;
+  const float av_pref = p.artificial_alpha * p.av_h * 10.0f;  // alpha * h_ab * c_ab
+  const float av_h2 = 0.01f * p.av_h * p.av_h;
+  V3 gi = g_ext_of<DIM>(a, i);
+  V3 acc = {0.f, 0.f, 0.f}, accv = {0.f, 0.f, 0.f};
+  float dT = 0.0f;
+  const int c = a.cnt[s];
+  for (int k = 0; k < c; ++k) {
+    const int j = a.ell[(int64_t)k * a.npad + s];
+    const float4 Pj = a.R.P[j];
+    const V3 dr = min_image<DIM>(Pi, Pj, a.geo);
+    const float d2 = dot<DIM>(dr, dr);
+    if (d2 == 0.0f) continue;  // self pair / coincident: every term is exactly 0
+    const float d = sqrtf(d2);
+    const float q = d * p.one_over_h;
+    const float gw = K.dw(q);
+    const float4 Uj = a.Uf[j], Qj = a.Qf[j];
+    const float mj = Pj.w, rhoj = Uj.w, pj = Qj.x, etaj = Qj.y;
+    const V3 uj = v3(Uj);
+    const float inv_d = 1.0f / (d + kEps);
+    const float Vj = mj / rhoj;
+    const float cpre = (Vi2 + Vj * Vj) * inv_mi * gw * inv_d;
+    const float eta_ij = 2.0f * etai * etaj / (etai + etaj + kEps);
+    if (SOLVER == 0) {
+      const float pij = (rhoj * pi_ + rhoi * pj) / (rhoi + rhoj);
+      V3 t = (-pij) * dr + eta_ij * (ui - uj);
+      if (TVF) {
+        const V3 vj = v3(a.Vf[j]);
+        const V3 Ar = 0.5f * (dot<DIM>(wi, dr) * Ai + dot<DIM>(vj - uj, dr) * (rhoj * uj));
+        t = t + Ar;
+      }
+      acc = acc + cpre * t;
+    } else {
+      const int tag_j = (int)Qj.z;
+      const bool wall_j = is_wall(tag_j);
+      const V3 e = inv_d * dr;
+      const V3 kg = gw * e;
+      const V3 me = (-1.0f) * e;
+      float uL, uR, pR, rhoR;
+      V3 vij;
+      if (wall_j) {
+        const V3 nj = v3(a.R.NW[j]);
+        uL = -dot<DIM>(ui, nj);
+        uR = -uL + 2.0f * dot<DIM>(uj, nj);
+        pR = pi_ + rhoi * dot<DIM>(gi, (-1.0f) * dr);
+        rhoR = eos.rho_fn(pR);
+        // u_tilde_j: Shepard wall velocity when the Riemann wall BC is on, else u_j
+        const V3 ud = p.bc_trick ? 2.0f * uj - v3(a.R.UT[j]) : uj;
+        vij = ui - ud;
+      } else {
+        uL = dot<DIM>(ui, me);
+        uR = dot<DIM>(uj, me);
+        pR = pj;
+        rhoR = rhoj;
+        vij = ui - uj;
+      }
+      const float Pavg = (pi_ + pR) * 0.5f;
+      const float rhoavg = (rhoi + rhoR) * 0.5f;
+      float beta;
+      if (p.eta_limiter == -1.0f) {
+        beta = p.c_ref;
+      } else {
+        beta = fminf(p.eta_limiter * fmaxf(uL - uR, 0.0f), p.c_ref);
+      }
+      const float Pstar = Pavg + 0.5f * rhoavg * (uL - uR) * beta;
+      const float irr = 1.0f / (rhoi * rhoj);
+      const V3 eq9 = (-2.0f * mj * (Pstar * irr)) * kg;
+      float mask = 1.0f;
+      if (p.bc_trick && p.free_slip) mask = fluid_i ? 1.0f : 0.0f;
+      const V3 eq6 = (2.0f * mj * eta_ij * irr * inv_d * gw * mask) * vij;
+      acc = acc + eq9 + eq6;
+    }
+    accv = accv + (cpre * pbg) * dr;
+    if (av) {
+      const bool fluid_j = (int)Qj.z == kFluid;
+      if (fluid_i && fluid_j) {
+        const float rho_ab = (rhoi + rhoj) * 0.5f;
+        const float num = mj * av_pref * dot<DIM>(ui - uj, dr);
+        const float den = rho_ab * (d2 + av_h2);
+        acc = acc + (num * gw * inv_d / den) * dr;
+      }
+    }
+    if (heat) {
+      const float kj = a.Vf[j].w;
+      const float keff = (kappa_i * kj) / (kappa_i + kj);
+      const float F = dot<DIM>(dr, (gw * inv_d) * dr) / (d2 + kEps);
+      dT += 4.0f * mj * keff * (Ti - Qj.w) * F / (Cp_i * rhoi * rhoj);
+    }
+  }
+  V3 dudt = acc + gi;
+  V3 dvdt = accv;
+  V3 u_out = ui, v_out = vi;
+  float p_out = pi_;
+  // case boundary hook (applied to the output state, integrator.py:54)
+  if (a.bc.kind == 1) {
+    if (tag_i == kSolidWall) {
+      u_out = v_out = V3{0.f, 0.f, 0.f};
+      dudt = dvdt = V3{0.f, 0.f, 0.f};
+    } else if (tag_i == kMovingWall) {
+      u_out = v_out = V3{a.bc.u_lid[0], a.bc.u_lid[1], a.bc.u_lid[2]};
+      dudt = dvdt = V3{0.f, 0.f, 0.f};
+    }
+  } else if (a.bc.kind == 2) {
+    if (tag_i == kSolidWall) {
+      u_out = v_out = V3{0.f, 0.f, 0.f};
+      dudt = dvdt = V3{0.f, 0.f, 0.f};
+      p_out = 0.0f;
+    }
+  } else if (a.bc.kind == 3) {
+    if (is_wall(tag_i)) {
+      u_out = v_out = V3{0.f, 0.f, 0.f};
+      dudt = dvdt = V3{0.f, 0.f, 0.f};
+    }
+  }
+  const sph_state_out& o = a.out;
+  store_vec<DIM>(o.dudt, i, dudt);
+  store_vec<DIM>(o.dvdt, i, dvdt);
+  store_vec<DIM>(o.u, i, u_out);
+  store_vec<DIM>(o.v, i, v_out);
+  o.rho[i] = rhoi;
+  o.p[i] = p_out;
+  o.drhodt[i] = a.R.drho[s];
+  if (o.T) o.T[i] = Ti;
+  if (o.dTdt) o.dTdt[i] = heat ? dT : (a.in.dTdt ? a.in.dTdt[i] : 0.0f);
+}
+
+// ------------------------------------------------------------- integrate
+// si_euler kick + transport velocity + drift + periodic wrap (integrator.py:26-30)
+template <int DIM>
+__global__ void k_integrate(const float* __restrict__ r, const float* __restrict__ u,
+                            const float* __restrict__ dudt, const float* __restrict__ dvdt,
+                            float* __restrict__ r_out, float* __restrict__ u_out,
+                            float* __restrict__ v_out, int n, float dt, float ctvf, float bx,
+                            float by, float bz) {
+#pragma clang fp contract(off)
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (int64_t)n * DIM) return;
+  const int d = (int)(k % DIM);
+  const float side = d == 0 ? bx : (d == 1 ? by : bz);
+  float uu = u[k] + dt * dudt[k];
+  float vv = uu + ctvf * dvdt[k];
+  float x = r[k] + dt * vv;
+  if (x >= side || x < 0.0f) x = jmod(x, side);
+  u_out[k] = uu;
+  v_out[k] = vv;
+  r_out[k] = x;
+}
+
+// kernel.py: W(r), dW/dr elementwise
+template <int DIM>
+__global__ void k_kernel_eval(int kind, float ih, float sg, const float* __restrict__ r, int64_t n,
+                              float* __restrict__ w, float* __restrict__ dw) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  KernelEval<-1, DIM> K{kind, sg, ih};
+  float q = r[k] * ih;
+  if (w) w[k] = K.w(q);
+  if (dw) dw[k] = K.dw(q);
+}
+
+}  // namespace sph
+
+using namespace sph;
+
+namespace {
+
+Geo make_geo(const float* box) {
+  Geo g;
+  for (int d = 0; d < 3; ++d) {
+    g.box[d] = box[d];
+    g.inv[d] = box[d] != 0.0f ? 1.0f / box[d] : 0.0f;
+  }
+  return g;
+}
+
+template <int DIM, int KIND>
+int launch_sweeps(sph_ws* ws, SweepArgs& a, hipStream_t st) {
+  const int n = a.n;
+  dim3 grid((unsigned)cdivl(n, kB));
+  const auto& p = a.prm;
+  // density
+  if (!p.density_evolution)
+    SPH_LAUNCH(ws, "density", st, (k_density<DIM, KIND, 0><<<grid, kB, 0, st>>>(a)));
+  else if (p.solver == 0)
+    SPH_LAUNCH(ws, "density", st, (k_density<DIM, KIND, 1><<<grid, kB, 0, st>>>(a)));
+  else
+    SPH_LAUNCH(ws, "density", st, (k_density<DIM, KIND, 2><<<grid, kB, 0, st>>>(a)));
+  // walls
+  const bool gwbc = p.bc_trick && p.solver == 0;
+  const bool heat_bc = p.bc_trick && p.solver == 1 && p.heat_conduction;
+  if (gwbc) {
+    SPH_LAUNCH(ws, "walls", st, (k_walls<DIM, KIND, 0><<<grid, kB, 0, st>>>(a)));
+  } else if (heat_bc) {
+    SPH_LAUNCH(ws, "walls", st, (k_walls<DIM, KIND, 1><<<grid, kB, 0, st>>>(a)));
+  }
+  if (gwbc || heat_bc) {
+    a.Uf = a.R.U2; a.Vf = a.R.V2; a.Qf = a.R.Q2;
+  } else {
+    a.Uf = a.R.U1; a.Vf = a.R.V0; a.Qf = a.R.Q1;
+  }
+  if (p.solver == 0) {
+    if (p.tvf_stress)
+      SPH_LAUNCH(ws, "force", st, (k_force<DIM, KIND, 0, true><<<grid, kB, 0, st>>>(a)));
+    else
+      SPH_LAUNCH(ws, "force", st, (k_force<DIM, KIND, 0, false><<<grid, kB, 0, st>>>(a)));
+  } else {
+    SPH_LAUNCH(ws, "force", st, (k_force<DIM, KIND, 1, false><<<grid, kB, 0, st>>>(a)));
+  }
+  return SPH_OK;
+}
+
+template <int DIM>
+int launch_kind(sph_ws* ws, SweepArgs& a, hipStream_t st) {
+  switch (a.prm.kernel) {
+    case SPH_KERNEL_QSK: return launch_sweeps<DIM, SPH_KERNEL_QSK>(ws, a, st);
+    case SPH_KERNEL_WC2K: return launch_sweeps<DIM, SPH_KERNEL_WC2K>(ws, a, st);
+    default: return launch_sweeps<DIM, -1>(ws, a, st);
+  }
+}
+
+int check_state(const sph_solver_params* prm, const sph_state_in* in, const sph_state_out* out) {
+  if (!prm || !in || !out) { set_error("null argument"); return SPH_ERR_ARG; }
+  if (!in->r || !in->u || !in->v || !in->rho || !in->p || !in->mass || !in->eta || !in->tag) {
+    set_error("state is missing r/u/v/rho/p/mass/eta/tag");
+    return SPH_ERR_ARG;
+  }
+  if (!out->u || !out->v || !out->dudt || !out->dvdt || !out->rho || !out->p || !out->drhodt) {
+    set_error("output is missing u/v/dudt/dvdt/rho/p/drhodt");
+    return SPH_ERR_ARG;
+  }
+  if (prm->heat_conduction && (!in->T || !in->dTdt || !in->kappa || !in->Cp || !out->T ||
+                               !out->dTdt)) {
+    set_error("heat conduction needs T, dTdt, kappa, Cp in and T, dTdt out");
+    return SPH_ERR_ARG;
+  }
+  if ((prm->solver == 1 || (prm->free_slip && prm->bc_trick)) && !in->nw) {
+    set_error("wall normals nw are required for this boundary treatment");
+    return SPH_ERR_ARG;
+  }
+  if (prm->kernel < 0 || prm->kernel > 6) { set_error("unknown kernel"); return SPH_ERR_ARG; }
+  return SPH_OK;
+}
+
+SweepArgs make_args(sph_ws* ws, int bank, const sph_solver_params* prm, const sph_state_in* in,
+                    sph_state_out* out) {
+  SweepArgs a;
+  a.prm = *prm;
+  a.in = *in;
+  a.out = *out;
+  a.R = Recs{ws->P, ws->U0, ws->V0, ws->Q0, ws->NW, ws->U1, ws->Q1, ws->U2, ws->V2, ws->Q2,
+             ws->UT, ws->drho};
+  a.perm = ws->bank[bank].perm;
+  a.ell = ws->bank[bank].ell;
+  a.cnt = ws->bank[bank].cnt;
+  a.geo = make_geo(prm->box);
+  a.n = ws->cfg.n;
+  a.npad = ws->npad;
+  a.Uf = a.R.U1; a.Vf = a.R.V0; a.Qf = a.R.Q1;
+  a.heat_in_density = prm->heat_conduction && !prm->bc_trick;
+  a.bc.kind = 0;
+  a.bc.u_lid[0] = a.bc.u_lid[1] = a.bc.u_lid[2] = 0.0f;
+  return a;
+}
+
+}  // namespace
+
+extern "C" int sph_forward(sph_ws* ws, int bank, const sph_solver_params* prm,
+                           const sph_state_in* in, sph_state_out* out, hipStream_t st) {
+  if (!ws || bank < 0 || bank > 1) { set_error("sph_forward: bad workspace/bank"); return SPH_ERR_ARG; }
+  int rc = check_state(prm, in, out);
+  if (rc) return rc;
+  if (prm->dim != ws->cfg.dim) { set_error("sph_forward: dim mismatch"); return SPH_ERR_ARG; }
+  SweepArgs a = make_args(ws, bank, prm, in, out);
+  dim3 grid((unsigned)cdivl(a.n, kB));
+  if (prm->dim == 2) {
+    SPH_LAUNCH(ws, "gather_state", st,
+               k_gather_state<2><<<grid, kB, 0, st>>>(a, in->r, in->u, in->v, nullptr));
+    return launch_kind<2>(ws, a, st);
+  }
+  SPH_LAUNCH(ws, "gather_state", st,
+             k_gather_state<3><<<grid, kB, 0, st>>>(a, in->r, in->u, in->v, nullptr));
+  return launch_kind<3>(ws, a, st);
+}
+
+extern "C" int sph_step(sph_ws* ws, int bank, const sph_solver_params* prm, float dt, float tvf_coef,
+                        const sph_bc_spec* bc, const sph_state_in* in, float* r_out,
+                        sph_state_out* out, const int64_t* prev_info, int64_t max_occupancy,
+                        int64_t* info, hipStream_t st) {
+  if (!ws || bank < 0 || bank > 1 || !r_out || !info) {
+    set_error("sph_step: bad argument");
+    return SPH_ERR_ARG;
+  }
+  int rc = check_state(prm, in, out);
+  if (rc) return rc;
+  if (!in->dudt || !in->dvdt) { set_error("sph_step: dudt/dvdt required"); return SPH_ERR_ARG; }
+  if (prm->dim != ws->cfg.dim) { set_error("sph_step: dim mismatch"); return SPH_ERR_ARG; }
+  if (in->g_ext) { set_error("sph_step: position-dependent g_ext is not fused"); return SPH_ERR_ARG; }
+  const int n = ws->cfg.n, dim = prm->dim;
+  // 1. kick/drift into the output buffers
+  const int64_t nk = (int64_t)n * dim;
+  if (dim == 2)
+    SPH_LAUNCH(ws, "integrate", st,
+               k_integrate<2><<<(unsigned)cdivl(nk, kB), kB, 0, st>>>(
+                   in->r, in->u, in->dudt, in->dvdt, r_out, out->u, out->v, n, dt, tvf_coef,
+                   prm->box[0], prm->box[1], prm->box[2]));
+  else
+    SPH_LAUNCH(ws, "integrate", st,
+               k_integrate<3><<<(unsigned)cdivl(nk, kB), kB, 0, st>>>(
+                   in->r, in->u, in->dudt, in->dvdt, r_out, out->u, out->v, n, dt, tvf_coef,
+                   prm->box[0], prm->box[1], prm->box[2]));
+  // 2. neighbour update at the new positions
+  rc = bin_particles(ws, bank, r_out, prev_info, info, st);
+  if (rc) return rc;
+  SweepArgs a = make_args(ws, bank, prm, in, out);
+  if (bc) a.bc = *bc;
+  dim3 grid((unsigned)cdivl(n, kB));
+  if (dim == 2)
+    SPH_LAUNCH(ws, "gather_state", st,
+               k_gather_state<2><<<grid, kB, 0, st>>>(a, r_out, out->u, out->v, nullptr));
+  else
+    SPH_LAUNCH(ws, "gather_state", st,
+               k_gather_state<3><<<grid, kB, 0, st>>>(a, r_out, out->u, out->v, nullptr));
+  rc = build_pairs(ws, bank, max_occupancy, info, st);
+  if (rc) return rc;
+  // 3. forward + case hook
+  a.in.r = r_out;
+  a.in.u = out->u;
+  a.in.v = out->v;
+  return dim == 2 ? launch_kind<2>(ws, a, st) : launch_kind<3>(ws, a, st);
+}
+
+extern "C" int sph_kernel_eval(int32_t kind, int32_t dim, float one_over_h, float sigma,
+                               const float* r, int64_t n, float* w, float* dw, hipStream_t st) {
+  if (!r || n < 0 || kind < 0 || kind > 6 || dim < 1 || dim > 3) {
+    set_error("sph_kernel_eval: bad argument");
+    return SPH_ERR_ARG;
+  }
+  if (n == 0) return SPH_OK;
+  unsigned g = (unsigned)cdivl(n, kB);
+  if (dim == 1) k_kernel_eval<1><<<g, kB, 0, st>>>(kind, one_over_h, sigma, r, n, w, dw);
+  else if (dim == 2) k_kernel_eval<2><<<g, kB, 0, st>>>(kind, one_over_h, sigma, r, n, w, dw);
+  else k_kernel_eval<3><<<g, kB, 0, st>>>(kind, one_over_h, sigma, r, n, w, dw);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "kernel_eval");
+  return SPH_OK;
+}
+
+extern "C" int sph_integrate(int32_t dim, int64_t n, const float* r, const float* u,
+                             const float* dudt, const float* dvdt, float* r_out, float* u_out,
+                             float* v_out, float dt, float tvf_coef, const float* box,
+                             hipStream_t st) {
+  if (!r || !u || !dudt || !dvdt || !r_out || !u_out || !v_out || !box || n < 0 ||
+      (dim != 2 && dim != 3)) {
+    set_error("sph_integrate: bad argument");
+    return SPH_ERR_ARG;
+  }
+  if (n == 0) return SPH_OK;
+  const int64_t nk = n * dim;
+  unsigned g = (unsigned)cdivl(nk, kB);
+  if (dim == 2)
+    k_integrate<2><<<g, kB, 0, st>>>(r, u, dudt, dvdt, r_out, u_out, v_out, (int)n, dt, tvf_coef,
+                                     box[0], box[1], 0.0f);
+  else
+    k_integrate<3><<<g, kB, 0, st>>>(r, u, dudt, dvdt, r_out, u_out, v_out, (int)n, dt, tvf_coef,
+                                     box[0], box[1], box[2]);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "integrate");
+  return SPH_OK;
+}

@@ -1,0 +1,86 @@
+// Host-side workspace layout shared by the neighbour-search and physics units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/sph.h"
+
+namespace sph {
+
+// One bank = everything a NeighborList snapshot needs (two banks let the
+// previous list stay readable while the next one is built).
+struct Bank {
+  int32_t* perm = nullptr;        // sorted slot -> original particle id
+  int32_t* iperm = nullptr;       // original id -> sorted slot
+  int32_t* cell_start = nullptr;  // ncell + 1
+  float4* pos = nullptr;          // sorted positions the pairs were built from
+  int32_t* ell = nullptr;         // pair buffer, column-major [ell_width][npad]
+  int32_t* cnt = nullptr;         // pairs per sorted particle
+};
+
+struct ProfEntry {
+  const char* name;
+  hipEvent_t a, b;
+};
+
+struct ProfTotal {
+  std::string name;
+  double ms = 0.0;
+  int64_t count = 0;
+};
+
+}  // namespace sph
+
+struct sph_ws {
+  sph_nl_config cfg;
+  int64_t ncell = 0;
+  int32_t npad = 0;        // N rounded up to 64 (pair-buffer column stride)
+  int64_t bytes = 0;
+  // binning scratch (shared by both banks)
+  int32_t* cell_count = nullptr;  // ncell
+  int32_t* pcell = nullptr;       // N: cell of each original particle
+  int32_t* pslot = nullptr;       // N: arrival slot inside the cell
+  int32_t* unsorted = nullptr;    // N: ids scattered by arrival order
+  int64_t* off64 = nullptr;       // N + 1: pair offsets of the idx writer
+  int64_t* scan_tmp = nullptr;    // block carries of the scans
+  int64_t* cnt64 = nullptr;       // N: pair count per original id
+  sph::Bank bank[2];
+  // sorted particle records used by the physics sweeps
+  float4 *P = nullptr;   // x, y, z, mass
+  float4 *U0 = nullptr, *U1 = nullptr, *U2 = nullptr;  // u, rho
+  float4 *V0 = nullptr, *V2 = nullptr;                // v, kappa
+  float4 *Q0 = nullptr, *Q1 = nullptr, *Q2 = nullptr;  // p, eta, tag, T
+  float4* NW = nullptr;  // wall normal
+  float4* UT = nullptr;  // Riemann wall velocity u_tilde
+  float* drho = nullptr; // sorted drhodt
+  // profiling
+  bool prof_on = false;
+  std::vector<sph::ProfEntry> prof_pending;
+  std::vector<hipEvent_t> prof_pool;
+  std::vector<sph::ProfTotal> prof_totals;
+};
+
+namespace sph {
+void set_error(const std::string& msg);
+int hip_fail(hipError_t e, const char* where);
+// profiling wrappers around a launch
+void prof_begin(sph_ws* ws, const char* name, hipStream_t s);
+void prof_end(sph_ws* ws, hipStream_t s);
+}  // namespace sph
+
+#define SPH_CHECK_LAUNCH(where)                                  \
+  do {                                                           \
+    hipError_t _e = hipGetLastError();                           \
+    if (_e != hipSuccess) return sph::hip_fail(_e, where);       \
+  } while (0)
+
+#define SPH_LAUNCH(ws, name, stream, ...)                        \
+  do {                                                           \
+    sph::prof_begin(ws, name, stream);                           \
+    __VA_ARGS__;                                                 \
+    sph::prof_end(ws, stream);                                   \
+    SPH_CHECK_LAUNCH(name);                                      \
+  } while (0)

@@ -1,0 +1,184 @@
+// Workspace lifetime, error reporting and the profiling hooks of libsph.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <string>
+
+#include "sph_internal.h"
+
+namespace sph {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int hip_fail(hipError_t e, const char* where) {
+  set_error(std::string(where) + ": " + hipGetErrorString(e));
+  return SPH_ERR_HIP;
+}
+
+void prof_begin(sph_ws* ws, const char* name, hipStream_t s) {
+  if (!ws || !ws->prof_on) return;
+  ProfEntry e;
+  e.name = name;
+  for (int k = 0; k < 2; ++k) {
+    hipEvent_t ev;
+    if (!ws->prof_pool.empty()) {
+      ev = ws->prof_pool.back();
+      ws->prof_pool.pop_back();
+    } else if (hipEventCreate(&ev) != hipSuccess) {
+      return;
+    }
+    (k == 0 ? e.a : e.b) = ev;
+  }
+  hipEventRecord(e.a, s);
+  ws->prof_pending.push_back(e);
+}
+
+void prof_end(sph_ws* ws, hipStream_t s) {
+  if (!ws || !ws->prof_on || ws->prof_pending.empty()) return;
+  hipEventRecord(ws->prof_pending.back().b, s);
+}
+
+static void prof_collect(sph_ws* ws) {
+  for (auto& e : ws->prof_pending) {
+    float ms = 0.0f;
+    hipEventSynchronize(e.b);
+    hipEventElapsedTime(&ms, e.a, e.b);
+    ProfTotal* t = nullptr;
+    for (auto& x : ws->prof_totals)
+      if (x.name == e.name) t = &x;
+    if (!t) {
+      ws->prof_totals.push_back(ProfTotal{e.name, 0.0, 0});
+      t = &ws->prof_totals.back();
+    }
+    t->ms += ms;
+    t->count += 1;
+    ws->prof_pool.push_back(e.a);
+    ws->prof_pool.push_back(e.b);
+  }
+  ws->prof_pending.clear();
+}
+
+template <typename T>
+static int alloc(sph_ws* ws, T** p, size_t count) {
+  size_t bytes = sizeof(T) * (count ? count : 1);
+  hipError_t e = hipMalloc((void**)p, bytes);
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+  ws->bytes += (int64_t)bytes;
+  return SPH_OK;
+}
+
+}  // namespace sph
+
+using namespace sph;
+
+extern "C" const char* sph_last_error(void) { return g_last_error.c_str(); }
+
+extern "C" int sph_version(void) { return 1; }
+
+extern "C" int sph_ws_create(const sph_nl_config* cfg, sph_ws** out) {
+  if (!cfg || !out) { set_error("sph_ws_create: null argument"); return SPH_ERR_ARG; }
+  if (cfg->dim != 2 && cfg->dim != 3) { set_error("dim must be 2 or 3"); return SPH_ERR_ARG; }
+  if (cfg->n <= 0) { set_error("n must be positive"); return SPH_ERR_ARG; }
+  if (cfg->ell_width <= 0) { set_error("ell_width must be positive"); return SPH_ERR_ARG; }
+  int64_t ncell = 1;
+  if (cfg->use_cells) {
+    for (int d = 0; d < cfg->dim; ++d) {
+      if (cfg->cps[d] < 3) { set_error("cell list needs >= 3 cells per side"); return SPH_ERR_ARG; }
+      ncell *= cfg->cps[d];
+    }
+  }
+  const int64_t N = cfg->n;
+  const int64_t npad = (N + 63) / 64 * 64;
+  if ((double)cfg->ell_width * (double)npad > 4.0e10) {
+    set_error("pair buffer would exceed 160 GB; reduce cell capacity");
+    return SPH_ERR_SIZE;
+  }
+  sph_ws* ws = new sph_ws();
+  ws->cfg = *cfg;
+  ws->ncell = ncell;
+  ws->npad = (int32_t)npad;
+  int rc = SPH_OK;
+#define A(ptr, n) if (rc == SPH_OK) rc = alloc(ws, &(ptr), (size_t)(n))
+  A(ws->cell_count, ncell);
+  A(ws->pcell, N);
+  A(ws->pslot, N);
+  A(ws->unsorted, N);
+  A(ws->off64, N + 1);
+  A(ws->scan_tmp, 1024);
+  A(ws->cnt64, N);
+  for (int b = 0; b < 2; ++b) {
+    A(ws->bank[b].perm, N);
+    A(ws->bank[b].iperm, N);
+    A(ws->bank[b].cell_start, ncell + 1);
+    A(ws->bank[b].pos, N);
+    A(ws->bank[b].ell, (int64_t)cfg->ell_width * npad);
+    A(ws->bank[b].cnt, N);
+  }
+  A(ws->P, N);
+  A(ws->U0, N); A(ws->U1, N); A(ws->U2, N);
+  A(ws->V0, N); A(ws->V2, N);
+  A(ws->Q0, N); A(ws->Q1, N); A(ws->Q2, N);
+  A(ws->NW, N); A(ws->UT, N);
+  A(ws->drho, N);
+#undef A
+  if (rc != SPH_OK) { sph_ws_destroy(ws); return rc; }
+  *out = ws;
+  return SPH_OK;
+}
+
+extern "C" int sph_ws_destroy(sph_ws* ws) {
+  if (!ws) return SPH_OK;
+  hipDeviceSynchronize();
+  void* ptrs[] = {ws->cell_count, ws->pcell, ws->pslot, ws->unsorted, ws->off64, ws->scan_tmp,
+                  ws->cnt64, ws->P, ws->U0, ws->U1, ws->U2, ws->V0, ws->V2, ws->Q0, ws->Q1,
+                  ws->Q2, ws->NW, ws->UT, ws->drho};
+  for (void* p : ptrs)
+    if (p) hipFree(p);
+  for (auto& b : ws->bank) {
+    void* bp[] = {b.perm, b.iperm, b.cell_start, b.pos, b.ell, b.cnt};
+    for (void* p : bp)
+      if (p) hipFree(p);
+  }
+  for (auto& e : ws->prof_pending) { ws->prof_pool.push_back(e.a); ws->prof_pool.push_back(e.b); }
+  for (auto ev : ws->prof_pool) hipEventDestroy(ev);
+  delete ws;
+  return SPH_OK;
+}
+
+extern "C" int64_t sph_ws_bytes(const sph_ws* ws) { return ws ? ws->bytes : 0; }
+
+extern "C" int sph_prof_enable(sph_ws* ws, int on) {
+  if (!ws) return SPH_ERR_ARG;
+  ws->prof_on = on != 0;
+  return SPH_OK;
+}
+
+extern "C" int sph_prof_reset(sph_ws* ws) {
+  if (!ws) return SPH_ERR_ARG;
+  prof_collect(ws);
+  ws->prof_totals.clear();
+  return SPH_OK;
+}
+
+extern "C" int sph_prof_read(sph_ws* ws, char* names, int names_cap, double* ms, int64_t* count,
+                             int cap) {
+  if (!ws) return SPH_ERR_ARG;
+  prof_collect(ws);
+  int n = (int)ws->prof_totals.size();
+  int pos = 0;
+  for (int i = 0; i < n && i < cap; ++i) {
+    const auto& t = ws->prof_totals[i];
+    if (ms) ms[i] = t.ms;
+    if (count) count[i] = t.count;
+    if (names) {
+      int len = (int)t.name.size();
+      if (pos + len + 1 <= names_cap) {
+        std::memcpy(names + pos, t.name.c_str(), len + 1);
+        pos += len + 1;
+      }
+    }
+  }
+  return n;
+}

@@ -1,0 +1,384 @@
+"""Case initialisation (host side, once per run) for the cases on the SPH step.
+
+Mirrors ``jax_sph/case_setup.py:44-231``, ``jax_sph/defaults.py:6-148`` and the
+case files ``cases/{tgv,ldc,db,pf,cf}.{py,yaml}``: the same defaults, CFL time
+step, equation of state, lattice and wall positions (float32 rounded like the
+reference), tags, initial velocities, wall normals (nearest point of a 5x
+finer wall layer, ``utils.py:169-194``) and the 16-key state dict, returned as
+device tensors.  ``initialize()`` returns the reference's 10-tuple.
+
+Per-step hooks are objects, not closures, so the fused step can run them on
+the device: ``CaseBC`` for ``bc_fn`` and ``solver.GravityField`` for constant
+body forces.  Random draws (position noise) use a seeded ``torch.Generator``
+instead of a JAX PRNG key, so noisy inputs are not bitwise the reference's.
+"""
+from __future__ import annotations
+
+import copy
+import math
+from typing import Any, Dict
+
+import numpy as np
+import torch
+
+from . import _lib
+from .eos import RIEMANNEoS, TaitEoS
+from .jax_md import space
+from .solver import GravityField
+from .utils import Tag, get_noise_masked, pos_box_2d, pos_box_3d, pos_init_cartesian_2d
+from .utils import pos_init_cartesian_3d, wall_tags
+
+EPS = float(np.finfo(float).eps)
+
+
+class Cfg(dict):
+    """Attribute-access dict standing in for the reference's OmegaConf config."""
+
+    def __getattr__(self, k):
+        try:
+            return self[k]
+        except KeyError as e:
+            raise AttributeError(k) from e
+
+    def __setattr__(self, k, v):
+        self[k] = v
+
+    @staticmethod
+    def wrap(d):
+        if isinstance(d, dict):
+            return Cfg({k: Cfg.wrap(v) for k, v in d.items()})
+        return d
+
+
+def defaults() -> Cfg:
+    """defaults.py:6-148."""
+    return Cfg.wrap({
+        "config": None, "seed": 123, "no_jit": False, "gpu": 0, "dtype": "float32",
+        "case": {"source": None, "mode": "sim", "dim": 3, "dx": 0.05, "state0_path": None,
+                 "state0_keys": ["r"], "r0_type": "cartesian", "r0_noise_factor": 0.0,
+                 "g_ext_magnitude": 0.0, "viscosity": 0.01, "u_ref": 1.0, "c_ref_factor": 10.0,
+                 "rho_ref": 1.0, "T_ref": 1.0, "kappa_ref": 0.0, "Cp_ref": 0.0, "special": {}},
+        "solver": {"name": "SPH", "tvf": 0.0, "cfl": 0.25, "density_evolution": False,
+                   "density_renormalize": False, "dt": None, "t_end": 0.2,
+                   "artificial_alpha": 0.0, "free_slip": False, "eta_limiter": 3, "kappa": 0,
+                   "n_walls": 3, "heat_conduction": False, "is_bc_trick": False,
+                   "diff_delta": 0.1, "diff_alpha": 0.01},
+        "kernel": {"name": "QSK", "h_factor": 1.0},
+        "eos": {"name": "Tait", "gamma": 1.0, "p_bg_factor": 0.0},
+        "nl": {"backend": "jaxmd_vmap", "num_partitions": 1},
+        "io": {"write_type": [], "write_every": 1, "data_path": "./",
+               "print_props": ["Ekin", "u_max"]},
+    })
+
+
+# cases/*.yaml (the values each case file overrides)
+CASE_YAML = {
+    "tgv": {"case": {"source": "tgv.py", "dim": 2, "dx": 0.02}, "solver": {"t_end": 5.0}},
+    "ldc": {"case": {"source": "ldc.py", "dim": 2, "dx": 0.02, "special": {"u_x_lid": 1.0}},
+            "solver": {"dt": 0.0004, "t_end": 5.0, "is_bc_trick": True},
+            "eos": {"p_bg_factor": 0.01}},
+    "db": {"case": {"source": "db.py", "dim": 2, "dx": 0.02, "g_ext_magnitude": 1.0,
+                    "u_ref": 1.4142135623730951, "viscosity": 0.00005,
+                    "special": {"L_wall": 5.366, "H_wall": 2.0, "L": 2.0, "H": 1.0, "W": 0.2,
+                                "box_offset": 0.1}},
+           "solver": {"is_bc_trick": True, "density_evolution": True, "artificial_alpha": 0.1,
+                      "t_end": 12.0, "dt": 0.0003}},
+    "pf": {"seed": 123, "case": {"source": "pf.py", "dim": 2, "dx": 0.0166666,
+                                 "viscosity": 100.0, "u_ref": 1.25, "g_ext_magnitude": 1000.0,
+                                 "special": {"L": 0.4, "H": 1.0}},
+           "solver": {"dt": 0.0000005, "t_end": 0.01, "is_bc_trick": True}},
+    "cf": {"seed": 123, "case": {"source": "cf.py", "dim": 2, "dx": 0.0166666,
+                                 "viscosity": 100.0, "u_ref": 1.25,
+                                 "special": {"L": 0.4, "H": 1.0, "u_x_wall": 1.25}},
+           "solver": {"dt": 0.0000005, "t_end": 0.01, "is_bc_trick": True}},
+}
+
+
+def _merge(a: Dict, b: Dict):
+    for k, v in b.items():
+        if isinstance(v, dict) and isinstance(a.get(k), dict):
+            _merge(a[k], v)
+        else:
+            a[k] = copy.deepcopy(v)
+    return a
+
+
+def load_config(case: str, overrides: Dict[str, Any] | None = None) -> Cfg:
+    """defaults <- cases/<case>.yaml <- overrides (main.py:20-47 without files)."""
+    cfg = defaults()
+    _merge(cfg, CASE_YAML[case])
+    if overrides:
+        _merge(cfg, overrides)
+    cfg = Cfg.wrap(cfg)
+    cfg.config = f"cases/{case}.yaml"
+    return cfg
+
+
+class CaseBC:
+    """A case's ``bc_fn`` as data (integrator.py:54 applies it after the RHS).
+
+    kind 0: none (tgv.py:56-57); 1: solid walls at rest and moving walls at
+    ``u_lid`` (ldc.py:130-144, cf.py); 2: dam-break walls at rest with p = 0
+    (db.py:299-307); 3: all walls at rest (relaxation, case_setup.py:393-402)."""
+
+    def __init__(self, kind: int, u_lid=(0.0, 0.0, 0.0)):
+        self.kind = int(kind)
+        self.u_lid = np.asarray(u_lid, dtype=np.float32).reshape(-1)
+
+    def __call__(self, state):
+        if self.kind == 0:
+            return state
+        state = dict(state)
+        tag = torch.as_tensor(state["tag"])
+        u, v = state["u"], state["v"]
+        dim = u.shape[1]
+        zero = torch.zeros((), dtype=u.dtype, device=u.device)
+        if self.kind == 3:
+            m1 = torch.isin(tag, torch.as_tensor(wall_tags, device=tag.device))[:, None]
+            m2 = torch.zeros_like(m1)
+        else:
+            m1 = (tag == Tag.SOLID_WALL)[:, None]
+            m2 = (tag == Tag.MOVING_WALL)[:, None] if self.kind == 1 else torch.zeros_like(m1)
+        lid = torch.as_tensor(self.u_lid[:dim], device=u.device)
+        for k in ("u", "v"):
+            x = torch.where(m1, zero, state[k])
+            state[k] = torch.where(m2, lid, x)
+        for k in ("dudt", "dvdt"):
+            state[k] = torch.where(m1 | m2, zero, state[k])
+        if self.kind == 2:
+            state["p"] = torch.where(m1[:, 0], zero, state["p"])
+        return state
+
+
+def _compute_nws(r, tag, dx, n_walls, offset_vec, wall_part_fn):
+    """utils.py:169-194: normal = unit vector to the nearest point of a fine wall layer."""
+    from scipy.spatial import KDTree
+    dx_fac = 5
+    walls = np.isin(tag, wall_tags)
+    r_aligned = r[walls] - offset_vec
+    layer = wall_part_fn(dx / dx_fac, 1) - offset_vec / n_walls / dx_fac
+    tree = KDTree(layer)
+    dist, idx = tree.query(r_aligned, k=1)
+    dr = layer[idx] - r_aligned
+    nw_walls = (dr / (dist[:, None] + EPS)).astype(r.dtype)
+    nw = np.zeros_like(r)
+    nw[walls] = nw_walls
+    return nw
+
+
+class SimulationSetup:
+    """case_setup.py:29-231 for one of tgv / ldc / db / pf / cf."""
+
+    def __init__(self, cfg: Cfg):
+        self.cfg = cfg
+        self.case = cfg.case
+        self.special = cfg.case.special
+        self.name = cfg.case.source.split(".")[0]
+        if cfg.case.mode != "sim" or cfg.case.r0_type != "cartesian":
+            raise NotImplementedError("relaxation runs and relaxed inputs are not provided")
+
+    # ---------------------------------------------------------------- geometry
+    def _box_size(self, dim, n_walls):
+        dx, sp, name = self.case.dx, self.special, self.name
+        if name == "tgv":
+            return np.ones(dim) if dim == 2 else 2 * np.pi * np.ones(3)
+        if name == "ldc":
+            dx2n = 2 * n_walls * dx
+            return np.ones(2) + dx2n if dim == 2 else np.array([1 + dx2n, 1 + dx2n, 0.5])
+        if name == "db":
+            bo = sp.box_offset
+            b = [sp.L_wall + 2 * n_walls * dx + bo, sp.H_wall + 2 * n_walls * dx + bo]
+            return np.array(b if dim == 2 else b + [sp.W])
+        if name in ("pf", "cf"):
+            dx2n = dx * n_walls * 2
+            return np.array([sp.L, sp.H + dx2n] if dim == 2 else [sp.L, sp.H + dx2n, 0.4])
+        raise ValueError(name)
+
+    def _walls(self, dx, n_walls, dim):
+        sp, name = self.special, self.name
+        if name == "ldc":
+            return (pos_box_2d(np.ones(2), dx, n_walls) if dim == 2
+                    else pos_box_3d(np.array([1.0, 1.0, 0.5]), dx, n_walls))
+        if name == "db":
+            return (pos_box_2d(np.array([sp.L_wall, sp.H_wall]), dx, n_walls) if dim == 2
+                    else pos_box_3d(np.array([sp.L_wall, sp.H_wall, 1.0]), dx, n_walls))
+        if name in ("pf", "cf"):
+            dxn = dx * n_walls
+            if dim == 2:
+                horiz = pos_init_cartesian_2d(np.array([sp.L, dxn]), dx)
+                top =(horiz + np.array([0.0, sp.H + dxn]).astype(np.float32)).astype(np.float32)
+            else:
+                horiz = pos_init_cartesian_3d(np.array([sp.L, dxn, 0.4]), dx)
+                top = (horiz + np.array([0.0, sp.H + dxn, 0.0]).astype(np.float32)).astype(
+                    np.float32)
+            return np.concatenate([horiz, top])
+        return np.zeros((0, dim), np.float32)
+
+    def offset_vec(self, dim):
+        nw, dx = self.cfg.solver.n_walls, self.case.dx
+        if self.name in ("ldc", "db"):
+            return np.ones(dim) * nw * dx if dim == 2 else np.array([1.0, 1.0, 0.0]) * nw * dx
+        if self.name in ("pf", "cf"):
+            return np.array([0.0, 1.0] if dim == 2 else [0.0, 1.0, 0.0]) * nw * dx
+        return np.zeros(dim)
+
+    def _positions(self, dim, dx, n_walls):
+        name, sp = self.name, self.special
+        f32 = np.float32
+        if name == "tgv":
+            box = self._box_size(dim, n_walls)
+            r = pos_init_cartesian_2d(box, dx) if dim == 2 else pos_init_cartesian_3d(box, dx)
+            return r, np.full(len(r), Tag.FLUID, np.int32)
+        if name == "ldc":
+            fluid_box = np.ones(2) if dim == 2 else np.array([1.0, 1.0, 0.5])
+        elif name == "db":
+            fluid_box = np.array([sp.L, sp.H] if dim == 2 else [sp.L, sp.H, 1.0])
+        else:
+            fluid_box = np.array([sp.L, sp.H] if dim == 2 else [sp.L, sp.H, 0.4])
+        lat = pos_init_cartesian_2d(fluid_box, dx) if dim == 2 else pos_init_cartesian_3d(
+            fluid_box, dx)
+        if name == "ldc" or (name == "db" and dim == 2):
+            off = np.full(dim, n_walls * dx)               # ldc.py:65,85, db.py:241
+        else:
+            off = self.offset_vec(dim)
+        r_f = (lat + off.astype(f32)).astype(f32)
+        r_w = self._walls(dx, n_walls, dim).astype(f32)
+        r = np.concatenate([r_w, r_f]).astype(f32)
+        tag = np.concatenate([np.full(len(r_w), Tag.SOLID_WALL, np.int32),
+                              np.full(len(r_f), Tag.FLUID, np.int32)])
+        if name in ("ldc", "cf"):
+            box = self._box_size(dim, n_walls)
+            lid = r[:, 1] > (box[1] - n_walls * dx)                    # ldc.py:79
+            tag = np.where(lid, np.int32(Tag.MOVING_WALL), tag).astype(np.int32)
+        return r, tag
+
+    def _velocity(self, r):
+        dim = r.shape[1]
+        v = np.zeros_like(r)
+        if self.name == "tgv":
+            f = np.float32
+            if dim == 2:                                                # tgv.py:37-43
+                x, y = r[:, 0], r[:, 1]
+                v[:, 0] = -f(1.0) * np.cos(f(2.0 * np.pi) * x) * np.sin(f(2.0 * np.pi) * y)
+                v[:, 1] = f(1.0) * np.sin(f(2.0 * np.pi) * x) * np.cos(f(2.0 * np.pi) * y)
+            else:                                                       # tgv.py:45-51
+                x, y, z = r[:, 0], r[:, 1], r[:, 2]
+                zt = np.cos(z)
+                v[:, 0] = np.sin(x) * np.cos(y) * zt
+                v[:, 1] = -np.cos(x) * np.sin(y) * zt
+        return v.astype(np.float32)
+
+    def g_ext_fn(self, dim):
+        name, g = self.name, self.case.g_ext_magnitude
+        if name == "db":
+            return GravityField([0.0, -g] + ([0.0] if dim == 3 else []))
+        if name == "pf":
+            nw, dx = self.cfg.solver.n_walls, self.case.dx
+            box = self._box_size(dim, nw)
+
+            def pf_force(r, box=box, dxn=nw * dx, g=g):             # pf.py:134-143
+                res = torch.zeros_like(r)
+                fm = (r[:, 1] < float(box[1] - dxn)) & (r[:, 1] > float(dxn))
+                res[:, 0] = torch.where(fm, torch.ones_like(r[:, 0]), torch.zeros_like(r[:, 0]))
+                return res * float(np.float32(g))
+            return pf_force
+        return GravityField(np.zeros(dim, np.float32))
+
+    def bc_fn(self, dim):
+        if self.name == "ldc":
+            return CaseBC(1, [self.special.u_x_lid, 0.0, 0.0])
+        if self.name == "cf":
+            return CaseBC(1, [self.special.u_x_wall, 0.0, 0.0])
+        if self.name == "db":
+            return CaseBC(2)
+        if self.name == "pf":
+            return CaseBC(1)
+        return CaseBC(0)
+
+    # ----------------------------------------------------------------- main
+    def initialize(self):
+        cfg = self.cfg
+        dx, dim = cfg.case.dx, cfg.case.dim
+        rho_ref, viscosity, u_ref = cfg.case.rho_ref, cfg.case.viscosity, cfg.case.u_ref
+        cfl = cfg.solver.cfl
+        c_ref = cfg.case.c_ref_factor * u_ref                           # case_setup.py:83-86
+        p_ref = rho_ref * c_ref ** 2 / cfg.eos.gamma
+        p_bg = cfg.eos.p_bg_factor * p_ref
+        h = dx
+        mass_ref = h ** dim * rho_ref
+        dt_conv = cfl * h / (c_ref + u_ref)                             # case_setup.py:94-110
+        dt_visc = cfl * h ** 2 * rho_ref / (viscosity + EPS)
+        dt_body = cfl * (h / (cfg.case.g_ext_magnitude + EPS)) ** 0.5
+        dt = float(np.amin([dt_conv, dt_visc, dt_body]))
+        if cfg.solver.dt is not None:
+            dt = cfg.solver.dt
+        sequence_length = int(cfg.solver.t_end / dt)
+        eos = (RIEMANNEoS(rho_ref, p_bg, u_ref) if cfg.solver.name == "RIE"
+               else TaitEoS(p_ref, rho_ref, p_bg, cfg.eos.gamma))
+        n_walls = cfg.solver.n_walls
+        box = self._box_size(dim, n_walls)
+        r, tag = self._positions(dim, dx, n_walls)
+        displacement_fn, shift_fn = space.periodic(side=box)
+        gen = torch.Generator().manual_seed(int(cfg.seed))
+        if cfg.case.r0_noise_factor != 0.0:
+            noise = get_noise_masked(r.shape, tag == Tag.FLUID, gen,
+                                     cfg.case.r0_noise_factor * dx).numpy()
+            side = np.asarray(box, np.float32)
+            r = np.mod(r + noise, side).astype(np.float32)
+        v = self._velocity(r)
+        N = len(r)
+        f32 = np.float32
+        rho = np.full(N, f32(rho_ref), f32)
+        mass = np.full(N, f32(mass_ref), f32)
+        eta = np.full(N, f32(viscosity), f32)
+        T = np.full(N, f32(cfg.case.T_ref), f32)
+        kappa = np.full(N, f32(cfg.case.kappa_ref), f32)
+        Cp = np.full(N, f32(cfg.case.Cp_ref), f32)
+        is_nw = (cfg.solver.free_slip or cfg.solver.name == "RIE") and cfg.solver.is_bc_trick
+        if is_nw:
+            nw = _compute_nws(r, tag, dx, n_walls, self.offset_vec(dim),
+                              lambda d, n: self._walls(d, n, dim))
+        else:
+            nw = np.zeros_like(r)
+        dev = _lib.device()
+
+        def T_(x, dtype=torch.float32):
+            return torch.as_tensor(np.ascontiguousarray(x), dtype=dtype, device=dev)
+
+        vt = T_(v)
+        rho_t = T_(rho)
+        state = {
+            "r": T_(r), "tag": T_(tag, torch.int32), "u": vt, "v": vt.clone(),
+            "dudt": torch.zeros_like(vt), "dvdt": torch.zeros_like(vt),
+            "drhodt": torch.zeros_like(rho_t), "rho": rho_t, "p": eos.p_fn(rho_t),
+            "mass": T_(mass), "eta": T_(eta), "dTdt": torch.zeros_like(rho_t), "T": T_(T),
+            "kappa": T_(kappa), "Cp": T_(Cp), "nw": T_(nw),
+        }
+        cfg.case.c_ref, cfg.case.p_ref, cfg.case.p_bg = c_ref, p_ref, p_bg
+        cfg.solver.dt, cfg.solver.sequence_length = dt, sequence_length
+        cfg.case.num_particles_max = N
+        cfg.case.pbc = [True, True, True]
+        bc_fn = self.bc_fn(dim)
+        state = bc_fn(state)
+        return (cfg, box, state, self.g_ext_fn(dim), bc_fn, None, eos, gen, displacement_fn,
+                shift_fn)
+
+
+def load_case(name: str) -> type:
+    """case_setup.py:407-425 stand-in: the setup class for a case name."""
+    if name.endswith(".py") or name.endswith(".yaml"):
+        name = name.split("/")[-1].split(".")[0]
+    if name not in CASE_YAML:
+        raise ValueError(f"unknown case {name!r}; available: {sorted(CASE_YAML)}")
+    return SimulationSetup
+
+
+def make_solver(cfg, displacement_fn, eos, g_ext_fn):
+    """simulate.py:49-69: WCSPH from a config."""
+    from .solver import WCSPH
+    return WCSPH(displacement_fn, eos, g_ext_fn, cfg.case.dx, cfg.case.dim, cfg.solver.dt,
+                 cfg.case.c_ref, cfg.solver.eta_limiter, cfg.solver.diff_delta,
+                 cfg.solver.diff_alpha, cfg.solver.name, cfg.kernel.name, cfg.kernel.h_factor,
+                 cfg.solver.is_bc_trick, cfg.solver.density_evolution,
+                 cfg.solver.artificial_alpha, cfg.solver.free_slip,
+                 cfg.solver.density_renormalize, cfg.solver.heat_conduction)

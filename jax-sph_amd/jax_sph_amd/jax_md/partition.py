@@ -1,0 +1,402 @@
+"""Neighbour lists on the HIP library (drop-in for ``jax_sph/jax_md/partition.py``).
+
+Same call signatures and semantics as the reference's ``neighbor_list``
+(``jax_md/partition.py:723-1026``) for the Sparse and OrderedSparse formats:
+
+* ``allocate`` sizes everything on the host (eager, like the reference):
+  the emulated cell capacity ``int(max_fill * capacity_multiplier) + extra``
+  (partition.py:195-199, 316-319) and ``max_occupancy = int(occ *
+  capacity_multiplier + N * extra)`` capped like partition.py:956-970;
+* ``update`` never resizes; it sets ``NEIGHBOR_LIST_OVERFLOW`` when the pair
+  count exceeds ``max_occupancy`` and ``CELL_LIST_OVERFLOW`` when a cell holds
+  more particles than the emulated capacity; error bits are sticky;
+* ``NeighborList.idx`` is int32 ``[2, max_occupancy]`` (receivers, senders),
+  sender-sorted and padded with N.  It is written by a HIP kernel the first
+  time it is read; the step itself works on the per-particle pair buffers.
+
+Differences kept on purpose: inside one sender the receivers are listed in
+cell-strip order instead of the reference's rolled-copy order (pair *sets* are
+bit-exact); a particle exactly on the upper box face is binned into the last
+cell instead of being aliased into a wrong cell.  Each ``update`` writes into
+the other of two buffer banks, so the list an update started from stays
+readable; a list two updates old raises ``StaleNeighborListError``.
+"""
+from __future__ import annotations
+
+import ctypes
+import enum
+from typing import Callable, Optional
+
+import numpy as np
+import torch
+
+from .. import _lib
+from .space import PeriodicDisplacement
+
+PEC_NONE = 0
+
+
+class PartitionErrorCode(enum.IntEnum):
+    """partition.py:434-457."""
+
+    NONE = 0
+    NEIGHBOR_LIST_OVERFLOW = 1 << 0
+    CELL_LIST_OVERFLOW = 1 << 1
+    CELL_SIZE_TOO_SMALL = 1 << 2
+    MALFORMED_BOX = 1 << 3
+
+
+PEC = PartitionErrorCode
+
+
+class NeighborListFormat(enum.Enum):
+    """partition.py:579-599."""
+
+    Dense = 0
+    Sparse = 1
+    OrderedSparse = 2
+
+
+Dense = NeighborListFormat.Dense
+Sparse = NeighborListFormat.Sparse
+OrderedSparse = NeighborListFormat.OrderedSparse
+
+
+def is_sparse(fmt) -> bool:
+    return fmt is Sparse or fmt is OrderedSparse
+
+
+class StaleNeighborListError(RuntimeError):
+    """The buffers behind this NeighborList were reused by a later update."""
+
+
+class PartitionError:
+    """partition.py:463-499; ``code`` is a 0-d uint8 device tensor."""
+
+    def __init__(self, code: torch.Tensor):
+        self.code = code
+
+    def update(self, bit, pred) -> "PartitionError":
+        pred = torch.as_tensor(pred, device=self.code.device)
+        return PartitionError(self.code | torch.where(pred, torch.tensor(int(bit), dtype=torch.uint8,
+                                                                          device=self.code.device),
+                                                      torch.zeros((), dtype=torch.uint8,
+                                                                  device=self.code.device)))
+
+    def __str__(self) -> str:
+        c = int(self.code)
+        if not c:
+            return ""
+        if c & PEC.NEIGHBOR_LIST_OVERFLOW:
+            return "Partition Error: Neighbor list buffer overflow."
+        if c & PEC.CELL_LIST_OVERFLOW:
+            return "Partition Error: Cell list buffer overflow"
+        if c & PEC.CELL_SIZE_TOO_SMALL:
+            return "Partition Error: Cell size too small"
+        if c & PEC.MALFORMED_BOX:
+            return "Partition Error: Incorrect box format. Expecting upper triangular."
+        raise ValueError(f"Unexpected Error Code {c}.")
+
+    __repr__ = __str__
+
+
+class NeighborList:
+    """partition.py:620-671.  Dynamic state lives in device buffers of a workspace bank."""
+
+    def __init__(self, search: "_NeighborSearch", ws: "_lib.Workspace", bank: int,
+                 info: torch.Tensor, reference_position: torch.Tensor,
+                 cell_list_capacity: Optional[int], max_occupancy: int, update_fn: Callable):
+        self._search = search
+        self._ws = ws
+        self._bank = bank
+        self._gen = ws.generation[bank]
+        self._info = info
+        self._idx = None
+        self.reference_position = reference_position
+        self.cell_list_capacity = cell_list_capacity
+        self.max_occupancy = int(max_occupancy)
+        self.format = search.format
+        self.cell_size = search.cutoff if search.use_cells else None
+        self.cell_list_fn = search if search.use_cells else None
+        self.update_fn = update_fn
+
+    # --- dynamic fields
+    def _check_live(self):
+        if self._ws.generation[self._bank] != self._gen:
+            raise StaleNeighborListError(
+                "this NeighborList's buffers were reused by a later update; read .idx "
+                "before updating twice")
+
+    @property
+    def idx(self) -> torch.Tensor:
+        if self._idx is None:
+            self._check_live()
+            self._idx = self._search.write_idx(self)
+        return self._idx
+
+    @property
+    def error(self) -> PartitionError:
+        return PartitionError(self._info[0].to(torch.uint8))
+
+    @property
+    def did_buffer_overflow(self) -> torch.Tensor:
+        return self.error.code & (PEC.NEIGHBOR_LIST_OVERFLOW | PEC.CELL_LIST_OVERFLOW)
+
+    @property
+    def cell_size_too_small(self) -> torch.Tensor:
+        return self.error.code & PEC.CELL_SIZE_TOO_SMALL
+
+    @property
+    def malformed_box(self) -> torch.Tensor:
+        return self.error.code & PEC.MALFORMED_BOX
+
+    # --- extras of this implementation
+    @property
+    def occupancy(self) -> int:
+        """Number of real pairs found by the last build (host sync)."""
+        return int(self._info[1])
+
+    @property
+    def info(self) -> torch.Tensor:
+        return self._info
+
+    @property
+    def bank(self) -> int:
+        return self._bank
+
+    @property
+    def workspace(self) -> "_lib.Workspace":
+        return self._ws
+
+    def update(self, position, **kwargs) -> "NeighborList":
+        return self.update_fn(position, self, **kwargs)
+
+
+class NeighborListFns:
+    """partition.py:674-717."""
+
+    def __init__(self, allocate: Callable, update: Callable):
+        self.allocate = allocate
+        self.update = update
+
+    def __call__(self, position, neighbors=None, extra_capacity: int = 0, **kwargs):
+        if neighbors is None:
+            return self.allocate(position, extra_capacity, **kwargs)
+        return self.update(position, neighbors, **kwargs)
+
+    def __iter__(self):
+        return iter((self.allocate, self.update))
+
+
+def _f32(x):
+    return np.float32(x)
+
+
+class _NeighborSearch:
+    """Host side of one neighbour-list configuration; owns the HIP workspace."""
+
+    def __init__(self, displacement_or_metric, box, r_cutoff, dr_threshold=0.0,
+                 capacity_multiplier=1.25, disable_cell_list=False, mask_self=True,
+                 custom_mask_function=None, fractional_coordinates=False, format=Sparse,
+                 backend="jaxmd_vmap", num_partitions=1):
+        if not isinstance(format, NeighborListFormat):
+            raise ValueError(f"Neighbor list format must be a member of NeighborListFormat "
+                             f"found {format}.")
+        if format is Dense:
+            raise NotImplementedError("Dense neighbour lists are not on the SPH step; use Sparse")
+        if custom_mask_function is not None:
+            raise NotImplementedError("custom_mask_function is not supported")
+        if fractional_coordinates:
+            raise NotImplementedError("fractional coordinates are not supported")
+        if displacement_or_metric is not None and not isinstance(displacement_or_metric,
+                                                                 PeriodicDisplacement):
+            raise NotImplementedError("only space.periodic displacements are supported "
+                                      "(case_setup.py:133 always uses them)")
+        self.box_arg = box
+        self.r_cutoff = float(r_cutoff)
+        self.dr_threshold = float(dr_threshold)
+        self.cutoff = _f32(_f32(r_cutoff) + _f32(dr_threshold))          # partition.py:815-821
+        self.cutoff_sq = _f32(self.cutoff * self.cutoff)
+        self.threshold_sq = _f32(_f32(_f32(dr_threshold) / _f32(2)) ** 2)
+        self.capacity_multiplier = float(capacity_multiplier)
+        self.disable_cell_list = disable_cell_list
+        self.mask_self = bool(mask_self)
+        self.format = format
+        self.backend = backend
+        self.num_partitions = int(num_partitions)
+        self.use_cells = None
+
+    # ------------------------------------------------------------ geometry
+    def _geometry(self, dim):
+        box = np.asarray(self.box_arg, dtype=np.float32).reshape(-1)
+        if box.size == 1:
+            box = np.full(dim, box[0], np.float32)
+        if box.size != dim:
+            raise ValueError(f"box has {box.size} entries for {dim}-d positions")
+        self.box = box
+        self.use_cells = (not self.disable_cell_list) and bool(
+            np.all(self.cutoff < box / np.float32(3.0)))              # partition.py:929
+        if self.backend == "jaxmd_scan" and not self.use_cells:
+            raise AssertionError("Don't use scan with very few cells")
+        if self.use_cells:
+            cps = np.floor(box / self.cutoff)                          # partition.py:128
+            self.cell_size = (box / cps).astype(np.float32)
+            self.cps = cps.astype(np.int32)
+            if np.any(self.cps < 3):
+                raise ValueError("Box must be at least 3x the size of the grid spacing in "
+                                 "each dimension.")
+            cps2 = np.floor(box / self.cell_size)                      # partition.py:159-174
+            self.cell_size2 = (box / cps2).astype(np.float32)
+            self.cps2 = cps2.astype(np.int32)
+        else:
+            self.cell_size = np.ones(dim, np.float32)
+            self.cps = np.ones(dim, np.int32)
+
+    def _config(self, n, dim, cap, width):
+        c = _lib.NLConfig()
+        c.dim, c.n = dim, n
+        for d in range(3):
+            c.box[d] = float(self.box[d]) if d < dim else 0.0
+            c.cps[d] = int(self.cps[d]) if d < dim else 1
+            c.cell_size[d] = float(self.cell_size[d]) if d < dim else 1.0
+        c.cutoff, c.cutoff_sq = float(self.cutoff), float(self.cutoff_sq)
+        c.use_cells = 1 if self.use_cells else 0
+        c.cell_capacity = int(cap) if cap is not None else -1
+        c.mask_self = 1 if self.mask_self else 0
+        c.ordered = 1 if self.format is OrderedSparse else 0
+        c.ell_width = int(width)
+        return c
+
+    # ------------------------------------------------------------ builds
+    def allocate(self, position, extra_capacity: int = 0, **kwargs) -> NeighborList:
+        dev = _lib.device()
+        r = _lib.as_f32(position, dev)
+        n, dim = r.shape
+        if dim not in (2, 3):
+            raise ValueError(f"Cell list spatial dimension must be 2 or 3. Found {dim}.")
+        self._geometry(dim)
+        L = _lib.lib()
+        st = _lib.stream()
+        info = torch.zeros(8, dtype=torch.int64, device=dev)
+        cap = None
+        if self.use_cells:
+            # capacity estimate on the grid re-derived from the rounded cell size
+            counts = torch.empty(int(np.prod(self.cps2.astype(np.int64))), dtype=torch.int32,
+                                 device=dev)
+            cs2 = (ctypes.c_float * 3)(*([float(x) for x in self.cell_size2] + [1.0] * (3 - dim)))
+            cp2 = (ctypes.c_int32 * 3)(*([int(x) for x in self.cps2] + [1] * (3 - dim)))
+            _lib.check(L.sph_nl_fill_max(dim, n, _lib.ptr(r), cs2, cp2, _lib.ptr(counts),
+                                         _lib.ptr(info), st), "sph_nl_fill_max")
+            max_fill = int(info[2])
+            cap = int(max_fill * self.capacity_multiplier) + int(extra_capacity)
+            width = min(3 ** dim * max(cap, 1), n)
+        else:
+            width = n
+        ws = _lib.Workspace(self._config(n, dim, cap, width))
+        ws.n, ws.dim, ws.cell_capacity = n, dim, cap
+        ws.candidate_width = n * (3 ** dim) * cap if self.use_cells else n * n
+        bank = 0
+        ws.generation[bank] += 1
+        _lib.check(L.sph_nl_build(ws.handle, bank, _lib.ptr(r), None, -1, _lib.ptr(info), st),
+                   "sph_nl_build")
+        max_occ = self._max_occupancy(int(info[1]), ws, extra_capacity)
+        _lib.check(L.sph_nl_finalize(ws.handle, max_occ, _lib.ptr(info), st), "sph_nl_finalize")
+        self._scan_bits(ws, bank, info)
+        return NeighborList(self, ws, bank, info, r.clone(), cap, max_occ, self.update)
+
+    def _max_occupancy(self, occ, ws, extra_capacity):
+        n = ws.n
+        extra = n * int(extra_capacity) if is_sparse(self.format) else int(extra_capacity)
+        m = int(_f32(_f32(_f32(occ) * _f32(self.capacity_multiplier)) + _f32(extra)))
+        width = ws.candidate_width
+        if self.backend == "jaxmd_scan":
+            width = self.num_partitions * self._scan_budget(ws)[1]
+        m = min(m, width)
+        if self.format is OrderedSparse:
+            limit = n * (n - 1) // 2
+        else:
+            limit = n * (n - 1) if self.mask_self else n * n
+        return min(m, limit)
+
+    def _scan_budget(self, ws):
+        # jax_sph/partition.py:213-235
+        n_sub = int(np.ceil(ws.n / self.num_partitions))
+        vol = 0.34907 if ws.dim == 2 else 0.15514
+        considered = (3 ** ws.dim) * ws.cell_capacity
+        return n_sub, int(n_sub * considered * vol * self.capacity_multiplier)
+
+    def _scan_bits(self, ws, bank, info):
+        if self.backend != "jaxmd_scan":
+            return
+        n_sub, budget = self._scan_budget(ws)
+        _lib.check(_lib.lib().sph_nl_scan_budget(ws.handle, bank, n_sub, budget,
+                                                 _lib.ptr(info), _lib.stream()),
+                   "sph_nl_scan_budget")
+
+    def update(self, position, neighbors: NeighborList, **kwargs) -> NeighborList:
+        if neighbors._search is not self:
+            return neighbors._search.update(position, neighbors, **kwargs)
+        ws = neighbors.workspace
+        r = _lib.as_f32(position)
+        if tuple(r.shape) != (ws.n, ws.dim):
+            raise ValueError(f"update got positions of shape {tuple(r.shape)}, the list was "
+                             f"allocated for {(ws.n, ws.dim)}")
+        L = _lib.lib()
+        st = _lib.stream()
+        if self.dr_threshold > 0.0:                                  # partition.py:1010-1018
+            _lib.check(L.sph_nl_needs_rebuild(ws.handle, _lib.ptr(r),
+                                              _lib.ptr(neighbors.reference_position),
+                                              float(self.threshold_sq),
+                                              _lib.ptr(neighbors.info), st),
+                       "sph_nl_needs_rebuild")
+            if not int(neighbors.info[4]):
+                return neighbors
+        info = torch.empty(8, dtype=torch.int64, device=r.device)
+        bank = 1 - neighbors.bank
+        ws.generation[bank] += 1
+        _lib.check(L.sph_nl_build(ws.handle, bank, _lib.ptr(r), _lib.ptr(neighbors.info),
+                                  neighbors.max_occupancy, _lib.ptr(info), st), "sph_nl_build")
+        self._scan_bits(ws, bank, info)
+        return NeighborList(self, ws, bank, info, r.clone() if self.dr_threshold > 0 else r,
+                            neighbors.cell_list_capacity, neighbors.max_occupancy,
+                            neighbors.update_fn)
+
+    def adopt_step(self, neighbors: NeighborList, bank: int, info: torch.Tensor,
+                   r: torch.Tensor) -> NeighborList:
+        """NeighborList for a bank just written by the fused step."""
+        self._scan_bits(neighbors.workspace, bank, info)
+        return NeighborList(self, neighbors.workspace, bank, info, r,
+                            neighbors.cell_list_capacity, neighbors.max_occupancy,
+                            neighbors.update_fn)
+
+    def write_idx(self, nbrs: NeighborList) -> torch.Tensor:
+        m = nbrs.max_occupancy
+        idx = torch.empty((2, m), dtype=torch.int32, device=nbrs.info.device)
+        if m:
+            _lib.check(_lib.lib().sph_nl_write_idx(nbrs.workspace.handle, nbrs.bank, _lib.ptr(idx),
+                                                   m, _lib.ptr(nbrs.info), _lib.stream()),
+                       "sph_nl_write_idx")
+        return idx
+
+
+def neighbor_list(displacement_or_metric, box, r_cutoff: float, dr_threshold: float = 0.0,
+                  capacity_multiplier: float = 1.25, disable_cell_list: bool = False,
+                  mask_self: bool = True, custom_mask_function=None,
+                  fractional_coordinates: bool = False, format: NeighborListFormat = Dense,
+                  **static_kwargs) -> NeighborListFns:
+    """partition.py:723-1026 (vmap backend semantics)."""
+    s = _NeighborSearch(displacement_or_metric, box, r_cutoff, dr_threshold,
+                        capacity_multiplier, disable_cell_list, mask_self,
+                        custom_mask_function, fractional_coordinates, format,
+                        backend=static_kwargs.get("backend", "jaxmd_vmap"),
+                        num_partitions=static_kwargs.get("num_partitions", 1))
+
+    def allocate_fn(position, extra_capacity: int = 0, **kwargs):
+        return s.allocate(position, extra_capacity, **kwargs)
+
+    def update_fn(position, neighbors, **kwargs):
+        return s.update(position, neighbors, **kwargs)
+
+    fns = NeighborListFns(allocate_fn, update_fn)
+    fns.search = s
+    return fns

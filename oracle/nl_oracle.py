@@ -274,7 +274,8 @@ class NeighborSearch:
     def _capacity(self, occupancy, N, extra_capacity, width):
         t = self.dtype.type
         if self.dtype == np.float32:
-            m = int(t(t(occupancy) * t(self.capacity_multiplier)) + N * extra_capacity)
+            # int(occ * mult + extra): int32 * weak float -> float32, + weak int -> float32
+            m = int(t(t(t(occupancy) * t(self.capacity_multiplier)) + t(N * extra_capacity)))
         else:
             m = int(occupancy * self.capacity_multiplier + N * extra_capacity)
         m = min(m, width)

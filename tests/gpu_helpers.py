@@ -1,0 +1,89 @@
+"""Helpers shared by the GPU parity tests: state conversion and oracle twins."""
+import numpy as np
+import torch
+
+from oracle import nl_oracle as nlo
+from oracle import sph_oracle as so
+
+import jax_sph_amd as js
+from jax_sph_amd import case_setup as cs
+
+
+def to_np(state):
+    out = {}
+    for k, v in state.items():
+        a = v.detach().cpu().numpy() if isinstance(v, torch.Tensor) else np.asarray(v)
+        out[k] = a.astype(np.int64) if k == "tag" else a.astype(np.float32)
+    return out
+
+
+def oracle_eos(eos):
+    if isinstance(eos, js.eos.RIEMANNEoS):
+        return so.RIEMANNEoS(eos.rho_ref, eos.p_bg, eos.u_ref, np.float32)
+    return so.TaitEoS(eos.p_ref, eos.rho_ref, eos.p_bg, eos.gamma, np.float32)
+
+
+def oracle_g(g_ext_fn):
+    if isinstance(g_ext_fn, js.solver.GravityField):
+        g = g_ext_fn.g.astype(np.float32)
+        return lambda r: np.broadcast_to(g, r.shape).astype(np.float32)
+
+    def g_np(r):
+        return g_ext_fn(torch.as_tensor(r, device="cuda")).cpu().numpy().astype(np.float32)
+    return g_np
+
+
+def oracle_solver(s: "js.solver.WCSPH", box):
+    return so.WCSPH(box, oracle_eos(s.eos), oracle_g(s.g_ext_fn), s.dx, s.dim, s.dt, s.c_ref,
+                    s.eta_limiter, s.solver, s._kernel_fn.name, s._kernel_fn.h / s.dx,
+                    s.is_bc_trick, s.is_rho_evol, s.artificial_alpha, s.is_free_slip,
+                    s.is_heat_conduction, np.float32)
+
+
+def oracle_bc(bc: "cs.CaseBC", dim):
+    def f(state):
+        st = dict(state)
+        tag = st["tag"]
+        if bc.kind == 0:
+            return st
+        m1 = (tag == 1)[:, None] if bc.kind != 3 else np.isin(tag, (1, 2, 3))[:, None]
+        m2 = (tag == 2)[:, None] if bc.kind == 1 else np.zeros_like(m1)
+        lid = bc.u_lid[:dim].astype(np.float32)
+        for k in ("u", "v"):
+            st[k] = np.where(m2, lid, np.where(m1, np.float32(0), st[k])).astype(np.float32)
+        for k in ("dudt", "dvdt"):
+            st[k] = np.where(m1 | m2, np.float32(0), st[k]).astype(np.float32)
+        if bc.kind == 2:
+            st["p"] = np.where(m1[:, 0], np.float32(0), st["p"]).astype(np.float32)
+        return st
+    return f
+
+
+def setup(case, overrides=None):
+    cfg = cs.load_config(case, overrides)
+    Case = cs.load_case(case)
+    (cfg, box, state, g_ext_fn, bc_fn, nw_fn, eos, key, disp, shift) = Case(cfg).initialize()
+    solver = cs.make_solver(cfg, disp, eos, g_ext_fn)
+    return cfg, box, state, g_ext_fn, bc_fn, eos, disp, shift, solver
+
+
+def rel_err(a, b):
+    """max |a - b| / max |b| (field-scale relative error)."""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    scale = np.abs(b).max()
+    if scale == 0:
+        return float(np.abs(a).max())
+    return float(np.abs(a - b).max() / scale)
+
+
+FIELDS = ("u", "v", "dudt", "dvdt", "rho", "p", "drhodt", "T", "dTdt")
+
+
+def assert_state_close(got, ref, tol, fields=FIELDS, label=""):
+    bad = {}
+    for k in fields:
+        e = rel_err(to_np({k: got[k]})[k], ref[k])
+        if e > tol:
+            bad[k] = e
+    assert not bad, f"{label}: relative errors above {tol}: {bad}"

@@ -1,0 +1,161 @@
+"""GPU parity of the neighbour search: bit-exact sorted pair sets against the
+reference's golden targets and the oracle, overflow bits, capacities."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import nl_oracle as nlo
+from tests.test_oracle_nl import GOLDEN, _ref_sort
+
+pytestmark = pytest.mark.gpu
+
+js = pytest.importorskip("jax_sph_amd")
+from jax_sph_amd import partition  # noqa: E402
+from jax_sph_amd.jax_md import space  # noqa: E402
+
+
+def _nl(box, cutoff, mask_self, backend="jaxmd_vmap", parts=1, **kw):
+    disp, _ = space.periodic(side=np.asarray(box))
+    return partition.neighbor_list(disp, np.asarray(box), r_cutoff=cutoff, backend=backend,
+                                   mask_self=mask_self, format=partition.NeighborListFormat.Sparse,
+                                   num_partitions=parts, **kw)
+
+
+def _pairs(nb, N):
+    return nlo.sorted_pairs(nb.idx.cpu().numpy(), N)
+
+
+@pytest.mark.parametrize("backend,parts", [("jaxmd_vmap", 1), ("jaxmd_scan", 1),
+                                           ("jaxmd_scan", 2)])
+@pytest.mark.parametrize("case", range(len(GOLDEN)))
+def test_golden_targets(case, backend, parts):
+    """tests/test_neighbors.py:20-121 of the reference, on the HIP search."""
+    r, cutoff, box, mask_self, target = GOLDEN[case]
+    N = len(r)
+    fn = _nl(box, cutoff, mask_self, backend, parts)
+    nb = fn.allocate(torch.tensor(r, dtype=torch.float32, device="cuda"), num_particles=N)
+    nb2 = nb.update(torch.tensor(r, dtype=torch.float32, device="cuda"))
+    assert not bool(nb.did_buffer_overflow) and not bool(nb2.did_buffer_overflow)
+    idx, idx2 = nb.idx.cpu().numpy(), nb2.idx.cpu().numpy()
+    assert (idx == idx2).all(), "allocate differs from update"
+    assert ((idx[0] == N) == (idx[1] == N)).all(), "one sided edges"
+    real = idx[:, idx[0] < N]
+    selfe = real[0] == real[1]
+    if mask_self:
+        assert selfe.sum() == 0
+    else:
+        assert len(np.unique(real[0][selfe])) == N
+    np.testing.assert_array_equal(_ref_sort(idx, N), target)
+
+
+CONFIGS = [
+    (2, 3000, [1.0, 1.0], 0.06), (2, 4000, [1.3, 0.7], 0.05), (3, 3000, [1.0, 1.0, 1.0], 0.15),
+    (3, 2500, [0.9, 1.2, 1.0], 0.2), (2, 200, [1.0, 1.0], 0.4),  # no cell list: box < 3 rc
+]
+
+
+@pytest.mark.parametrize("dim,N,box,cutoff", CONFIGS)
+@pytest.mark.parametrize("mask_self", [True, False])
+@pytest.mark.parametrize("backend", ["jaxmd_vmap", "jaxmd_scan"])
+def test_random_matches_oracle(dim, N, box, cutoff, mask_self, backend):
+    if backend == "jaxmd_scan" and cutoff * 3 >= min(box):
+        pytest.skip("scan needs a cell list")
+    rng = np.random.default_rng(N + dim)
+    r = (rng.random((N, dim)) * np.asarray(box)).astype(np.float32)
+    ref = nlo.NeighborSearch(np.asarray(box), cutoff, mask_self=mask_self, backend=backend,
+                             num_partitions=3 if backend == "jaxmd_scan" else 1).allocate(r)
+    nb = _nl(box, cutoff, mask_self, backend, 3 if backend == "jaxmd_scan" else 1).allocate(
+        torch.tensor(r, device="cuda"))
+    np.testing.assert_array_equal(_pairs(nb, N), nlo.sorted_pairs(ref.idx, N))
+    assert nb.occupancy == ref.occupancy
+    assert nb.max_occupancy == ref.max_occupancy
+    assert nb.cell_list_capacity == ref.cell_list_capacity
+    assert int(nb.error.code) == ref.error
+
+
+@pytest.mark.parametrize("dim,n,dx", [(2, 50, 0.02), (2, 100, 0.01), (3, 24, 2 * np.pi / 24)])
+def test_lattice_ties_bit_exact(dim, n, dx):
+    """Quintic cutoff 3dx on a lattice: every on-cutoff tie decided like fp32 numpy."""
+    from tests.helpers import lattice
+    r = lattice(n, dx, dim)
+    box = np.full(dim, n * dx)
+    ref = nlo.NeighborSearch(box, 3 * dx, mask_self=False).allocate(r)
+    nb = _nl(box, 3 * dx, False).allocate(torch.tensor(r, device="cuda"))
+    np.testing.assert_array_equal(_pairs(nb, len(r)), nlo.sorted_pairs(ref.idx, len(r)))
+    assert nb.occupancy == ref.occupancy and nb.max_occupancy == ref.max_occupancy
+
+
+def test_overflow_bits_and_sticky_errors():
+    rng = np.random.default_rng(3)
+    box = np.array([1.0, 1.0])
+    r = (rng.random((2000, 2)) * box).astype(np.float32)
+    fn = _nl(box, 0.05, False)
+    ref_ns = nlo.NeighborSearch(box, 0.05, mask_self=False)
+    nb = fn.allocate(torch.tensor(r, device="cuda"))
+    ref = ref_ns.allocate(r)
+    assert not bool(nb.did_buffer_overflow)
+    rc = (r * np.float32(0.3)).astype(np.float32)           # cluster into a corner
+    nb2 = nb.update(torch.tensor(rc, device="cuda"))
+    ref2 = ref_ns.update(rc, ref)
+    assert int(nb2.error.code) == ref2.error == 3
+    nb3 = nb2.update(torch.tensor(r, device="cuda"))        # sticky
+    assert int(nb3.error.code) == 3
+    # capacity_multiplier < 1: the list overflows at allocate
+    small = _nl(box, 0.05, False, capacity_multiplier=0.5).allocate(torch.tensor(r, device="cuda"))
+    ref_small = nlo.NeighborSearch(box, 0.05, mask_self=False, capacity_multiplier=0.5).allocate(r)
+    assert int(small.error.code) == ref_small.error
+    assert small.did_buffer_overflow
+
+
+def test_update_with_threshold_skips_rebuild():
+    rng = np.random.default_rng(5)
+    box = np.array([1.0, 1.0])
+    r = torch.tensor((rng.random((500, 2)) * box).astype(np.float32), device="cuda")
+    fn = _nl(box, 0.1, False, dr_threshold=0.02)
+    nb = fn.allocate(r)
+    _, shift = space.periodic(side=box)
+    assert nb.update(shift(r, torch.full_like(r, 0.004))) is nb
+    nb2 = nb.update(shift(r, torch.full_like(r, 0.02)))
+    assert nb2 is not nb
+    # the skin list holds every pair within cutoff + threshold of the new positions
+    r2 = nb2.reference_position.cpu().numpy()
+    ref = nlo.NeighborSearch(box, 0.1, dr_threshold=0.02, mask_self=False).allocate(r2)
+    np.testing.assert_array_equal(_pairs(nb2, 500), nlo.sorted_pairs(ref.idx, 500))
+
+
+def test_stale_list_raises():
+    box = np.array([1.0, 1.0])
+    r = torch.rand(300, 2, device="cuda")
+    fn = _nl(box, 0.1, False)
+    nb = fn.allocate(r)
+    nb2 = nb.update(r)
+    _ = nb.idx  # previous bank still readable
+    nb3 = nb2.update(r)
+    nb4 = nb3.update(r)
+    with pytest.raises(partition.StaleNeighborListError):
+        _ = nb2.idx
+    assert (nb4.idx == nb3.idx).all()
+
+
+def test_full_size_3d_tgv_lattice_sampled_rows():
+    """64^3 = 262,144 particles (the 3D TGV lattice): 512 sampled senders are
+    checked row by row against an exact brute force over all particles."""
+    from tests.helpers import lattice
+    n, dx = 64, 2 * np.pi / 64
+    r = lattice(n, dx, 3)
+    box = np.full(3, 2 * np.pi)
+    nb = _nl(box, 3 * dx, False).allocate(torch.tensor(r, device="cuda"))
+    assert not bool(nb.did_buffer_overflow)
+    idx = nb.idx.cpu().numpy()
+    N = len(r)
+    rng = np.random.default_rng(0)
+    sample = rng.choice(N, 512, replace=False)
+    t = np.float32
+    c2 = t(t(3 * dx) * t(3 * dx))
+    side = box.astype(np.float32)
+    for s in sample:
+        d = nlo.square_distance(nlo.periodic_displacement(side, r[s][None, :] - r))
+        want = np.sort(np.nonzero(d < c2)[0])
+        got = np.sort(idx[0][idx[1] == s])
+        np.testing.assert_array_equal(got, want)
+    assert nb.occupancy == int((idx[0] < N).sum())
