@@ -22,8 +22,9 @@ FLAGS = [
     "-shared",
     # packed FP32 instructions are avoided on this hardware pool
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    # IEEE float32 division/sqrt: the cell hash must round like the reference
-    "-fhip-fp32-correctly-rounded-divide-sqrt",
+    # fast float32 division/sqrt in the sweeps; the cell hash rounds exactly
+    # through a float64 quotient
+    "-fno-hip-fp32-correctly-rounded-divide-sqrt",
     "-Wno-unused-command-line-argument",
 ]
 

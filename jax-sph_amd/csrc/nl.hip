@@ -9,6 +9,8 @@
 //   jax_sph/partition.py:208-291  jaxmd_scan chunk budgets
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+
 #include "sph_device.h"
 #include "sph_internal.h"
 
@@ -45,7 +47,9 @@ __device__ __forceinline__ void cell_coords(const float4 p, const Grid& g, int* 
   float x[3] = {p.x, p.y, p.z};
 #pragma unroll
   for (int d = 0; d < DIM; ++d) {
-    int q = (int)(x[d] / g.cs[d]);  // correctly rounded division (build flag)
+    // fl32(x / cs), correctly rounded: a float64 quotient rounded to float32 is
+    // the correctly rounded float32 quotient (53 >= 2*24 + 2 bits)
+    int q = (int)(float)((double)x[d] / (double)g.cs[d]);
     q = q < 0 ? 0 : q;
     q = q >= g.cps[d] ? g.cps[d] - 1 : q;
     c[d] = q;
@@ -239,18 +243,48 @@ struct BuildArgs {
   Grid g;
   float box[3], half[3];
   float cutoff_sq;
+  // fast filter: squared distances of the cell-adjacent image below `lo` are
+  // in, above `hi` out; the band [lo, hi) bounds the difference between that
+  // and the exact reference metric, and is decided by the exact predicate.
+  float lo, hi;
+  int fast;
   int n, npad, width, mask_self;
 };
 
+// Candidates [lo, hi) of one cell range whose image is shifted by `img`
+// (0 or +-box per axis).  Hits are collected 32 at a time in a bit mask so
+// the divergent stores happen once per hit, in candidate order.
 template <int DIM, bool ORDERED>
 __device__ __forceinline__ void test_range(const BuildArgs& a, int s, const float4 ps, int sid,
-                                           int lo, int hi, int& count) {
-  for (int b = lo; b < hi; ++b) {
-    if (a.mask_self && b == s) continue;
-    if (ORDERED && !(a.perm[b] < sid)) continue;
-    float d2 = metric_sq_exact<DIM>(ps, a.pos[b], a.box, a.half);
-    if (d2 < a.cutoff_sq) {
-      if (count < a.width) a.ell[(int64_t)count * a.npad + s] = b;
+                                           int lo, int hi, const float* img, int& count) {
+  const float sx = ps.x - img[0], sy = ps.y - img[1], sz = ps.z - img[2];
+  for (int b0 = lo; b0 < hi; b0 += 32) {
+    const int nb = hi - b0 < 32 ? hi - b0 : 32;
+    unsigned mask = 0u;
+    for (int t = 0; t < nb; ++t) {
+      const int b = b0 + t;
+      const float4 pb = a.pos[b];
+      bool hit;
+      if (a.fast) {
+        const float dx = sx - pb.x, dy = sy - pb.y;
+        float d2 = dx * dx + dy * dy;
+        if (DIM == 3) {
+          const float dz = sz - pb.z;
+          d2 += dz * dz;
+        }
+        hit = d2 < a.lo;
+        if (!hit && d2 < a.hi) hit = metric_sq_exact<DIM>(ps, pb, a.box, a.half) < a.cutoff_sq;
+      } else {
+        hit = metric_sq_exact<DIM>(ps, pb, a.box, a.half) < a.cutoff_sq;
+      }
+      if (a.mask_self && b == s) hit = false;
+      if (ORDERED && hit && !(a.perm[b] < sid)) hit = false;
+      mask |= hit ? (1u << t) : 0u;
+    }
+    while (mask) {
+      const int t = __ffs(mask) - 1;
+      mask &= mask - 1u;
+      if (count < a.width) a.ell[(int64_t)count * a.npad + s] = b0 + t;
       ++count;
     }
   }
@@ -266,8 +300,9 @@ __global__ __launch_bounds__(kBlock) void k_nl_build(BuildArgs a) {
   if (s < a.n) {
     const float4 ps = a.pos[s];
     const int sid = ORDERED ? a.perm[s] : 0;
+    float img[3] = {0.0f, 0.0f, 0.0f};
     if (!a.g.use_cells) {
-      test_range<DIM, ORDERED>(a, s, ps, sid, 0, a.n, count);
+      test_range<DIM, ORDERED>(a, s, ps, sid, 0, a.n, img, count);
     } else {
       int c[3];
       cell_coords<DIM>(ps, a.g, c);
@@ -276,20 +311,27 @@ __global__ __launch_bounds__(kBlock) void k_nl_build(BuildArgs a) {
       const int z0 = DIM == 3 ? -1 : 0, z1 = DIM == 3 ? 1 : 0;
       for (int dz = z0; dz <= z1; ++dz) {
         int cz = DIM == 3 ? c[2] + dz : 0;
-        cz = cz < 0 ? cz + nz : (cz >= nz ? cz - nz : cz);
+        img[2] = 0.0f;
+        if (cz < 0) { cz += nz; img[2] = -a.box[2]; }
+        else if (cz >= nz) { cz -= nz; img[2] = a.box[2]; }
         for (int dy = -1; dy <= 1; ++dy) {
           int cy = c[1] + dy;
-          cy = cy < 0 ? cy + ny : (cy >= ny ? cy - ny : cy);
+          img[1] = 0.0f;
+          if (cy < 0) { cy += ny; img[1] = -a.box[1]; }
+          else if (cy >= ny) { cy -= ny; img[1] = a.box[1]; }
           const int row = (cz * ny + cy) * nx;
           if (c[0] >= 1 && c[0] + 1 < nx) {
+            img[0] = 0.0f;
             test_range<DIM, ORDERED>(a, s, ps, sid, a.cell_start[row + c[0] - 1],
-                                     a.cell_start[row + c[0] + 2], count);
+                                     a.cell_start[row + c[0] + 2], img, count);
           } else {
             for (int dx = -1; dx <= 1; ++dx) {
               int cx = c[0] + dx;
-              cx = cx < 0 ? cx + nx : (cx >= nx ? cx - nx : cx);
+              img[0] = 0.0f;
+              if (cx < 0) { cx += nx; img[0] = -a.box[0]; }
+              else if (cx >= nx) { cx -= nx; img[0] = a.box[0]; }
               test_range<DIM, ORDERED>(a, s, ps, sid, a.cell_start[row + cx],
-                                       a.cell_start[row + cx + 1], count);
+                                       a.cell_start[row + cx + 1], img, count);
             }
           }
         }
@@ -449,6 +491,21 @@ int build_pairs(sph_ws* ws, int bank, int64_t max_occupancy, int64_t* info, hipS
   a.npad = ws->npad;
   a.width = c.ell_width;
   a.mask_self = c.mask_self;
+  // The cell-adjacent image is the minimum image of every pair within the
+  // cutoff when each side has >= 5 cells (2 cells < box/2).  Both the fast and
+  // the exact squared distance are then within a few ulp(box) per component of
+  // the true one; `band` bounds their difference with a wide margin.
+  float maxbox = 0.0f;
+  a.fast = c.use_cells ? 1 : 0;
+  for (int d = 0; d < c.dim; ++d) {
+    maxbox = c.box[d] > maxbox ? c.box[d] : maxbox;
+    if (c.cps[d] < 5) a.fast = 0;
+  }
+  const float ulp_box = std::nextafter(maxbox, INFINITY) - maxbox;
+  const float ulp_c2 = std::nextafter(c.cutoff_sq, INFINITY) - c.cutoff_sq;
+  const float band = 64.0f * c.cutoff * ulp_box + 16.0f * ulp_c2;
+  a.lo = c.cutoff_sq - band;
+  a.hi = c.cutoff_sq + band;
   dim3 grid(cdiv(c.n, kBlock));
   if (c.dim == 2) {
     if (c.ordered) SPH_LAUNCH(ws, "nl_build", st, (k_nl_build<2, true><<<grid, kBlock, 0, st>>>(a)));

@@ -87,3 +87,39 @@ def assert_state_close(got, ref, tol, fields=FIELDS, label=""):
         if e > tol:
             bad[k] = e
     assert not bad, f"{label}: relative errors above {tol}: {bad}"
+
+
+def assert_within_noise_floor(got, ref32, ref64, tol, fields=FIELDS, factor=4.0, label=""):
+    """Each field of the GPU result is as close to the float64 oracle as the
+    float32 oracle is: err(gpu, ref64) <= max(tol, factor * err(ref32, ref64)).
+
+    Fields such as p = c^2 (rho - rho0) or a transport-velocity sum on a near
+    lattice cancel to a few float32 ulps of their summands; there any float32
+    implementation differs from another by that noise, which this bound admits
+    while still failing a wrong term (whose error is O(1) of the field)."""
+    bad = {}
+    for k in fields:
+        g = to_np({k: got[k]})[k].astype(np.float64)
+        e_g = rel_err(g, ref64[k])
+        e_32 = rel_err(ref32[k], ref64[k])
+        bound = max(tol, factor * e_32)
+        if e_g > bound:
+            bad[k] = (e_g, e_32)
+    assert not bad, f"{label}: (err vs fp64, fp32-oracle err) above bound: {bad}"
+
+
+def to_f64(state_np):
+    return {k: (v if k == "tag" else v.astype(np.float64)) for k, v in state_np.items()}
+
+
+def oracle_solver64(s: "js.solver.WCSPH", box):
+    e = s.eos
+    if isinstance(e, js.eos.RIEMANNEoS):
+        eos = so.RIEMANNEoS(e.rho_ref, e.p_bg, e.u_ref, np.float64)
+    else:
+        eos = so.TaitEoS(e.p_ref, e.rho_ref, e.p_bg, e.gamma, np.float64)
+    g32 = oracle_g(s.g_ext_fn)
+    return so.WCSPH(box, eos, lambda r: g32(r.astype(np.float32)).astype(np.float64), s.dx,
+                    s.dim, s.dt, s.c_ref, s.eta_limiter, s.solver, s._kernel_fn.name,
+                    s._kernel_fn.h / s.dx, s.is_bc_trick, s.is_rho_evol, s.artificial_alpha,
+                    s.is_free_slip, s.is_heat_conduction, np.float64)

@@ -16,7 +16,7 @@ from tests import gpu_helpers as G  # noqa: E402
 TOL = 1e-5
 
 
-def _perturb(state, seed, dim, fluid_only=True, heat=False):
+def _perturb(state, seed, dim, fluid_only=True, heat=False, dx=None, shift=None):
     g = torch.Generator(device="cpu").manual_seed(seed)
     st = dict(state)
     n = st["r"].shape[0]
@@ -24,6 +24,10 @@ def _perturb(state, seed, dim, fluid_only=True, heat=False):
 
     def rnd(*shape):
         return torch.randn(*shape, generator=g).to("cuda")
+
+    # move fluid particles off the lattice so no sum cancels to rounding noise
+    fl_r = (st["tag"] == 0)[:, None].float()
+    st["r"] = shift(st["r"], 0.15 * dx * rnd(n, dim) * fl_r)
 
     st["u"] = st["u"] + 0.2 * rnd(n, dim) * fl
     st["v"] = st["u"] + 0.05 * rnd(n, dim) * fl
@@ -42,7 +46,7 @@ def _perturb(state, seed, dim, fluid_only=True, heat=False):
 def _run(case, overrides, seed=0, heat=False, fluid_only=True):
     cfg, box, state, g, bc, eos, disp, shift, solver = G.setup(case, overrides)
     dim = cfg.case.dim
-    st = _perturb(state, seed, dim, fluid_only, heat)
+    st = _perturb(state, seed, dim, fluid_only, heat, cfg.case.dx, shift)
     fn = partition.neighbor_list(disp, box, r_cutoff=solver._kernel_fn.cutoff,
                                  capacity_multiplier=1.25, mask_self=False,
                                  format=partition.Sparse)
@@ -53,7 +57,8 @@ def _run(case, overrides, seed=0, heat=False, fluid_only=True):
     ref_nl = nlo.NeighborSearch(box, solver._kernel_fn.cutoff, mask_self=False).allocate(
         np_st["r"])
     ref = G.oracle_solver(solver, box).forward(np_st, ref_nl.idx)
-    return out, ref, cfg
+    ref64 = G.oracle_solver64(solver, box).forward(G.to_f64(np_st), ref_nl.idx)
+    return out, ref, ref64, cfg
 
 
 CASES = [
@@ -73,8 +78,8 @@ CASES = [
 
 @pytest.mark.parametrize("case,ov", CASES, ids=[f"{c}{i}" for i, (c, _) in enumerate(CASES)])
 def test_forward_matches_oracle(case, ov):
-    out, ref, _ = _run(case, ov, seed=1)
-    G.assert_state_close(out, ref, TOL, label=case)
+    out, ref, ref64, _ = _run(case, ov, seed=1)
+    G.assert_within_noise_floor(out, ref, ref64, TOL, label=case)
 
 
 @pytest.mark.parametrize("case,ov", [
@@ -83,8 +88,8 @@ def test_forward_matches_oracle(case, ov):
     ("db", {"case": {"dx": 0.04}, "solver": {"name": "RIE", "heat_conduction": True}}),
 ])
 def test_heat_conduction_matches_oracle(case, ov):
-    out, ref, _ = _run(case, ov, seed=2, heat=True, fluid_only=False)
-    G.assert_state_close(out, ref, TOL, label=case)
+    out, ref, ref64, _ = _run(case, ov, seed=2, heat=True, fluid_only=False)
+    G.assert_within_noise_floor(out, ref, ref64, TOL, label=case)
 
 
 @pytest.mark.parametrize("kernel,hf", [("CSK", 1.0), ("QSK", 1.0), ("WC2K", 1.3), ("WC4K", 1.3),
@@ -92,8 +97,8 @@ def test_heat_conduction_matches_oracle(case, ov):
 def test_all_kernels_match_oracle(kernel, hf):
     ov = {"case": {"dim": 2, "dx": 0.025}, "solver": {"tvf": 1.0},
           "kernel": {"name": kernel, "h_factor": hf}}
-    out, ref, _ = _run("tgv", ov, seed=3)
-    G.assert_state_close(out, ref, 2 * TOL if kernel in ("GK", "SGK") else TOL, label=kernel)
+    out, ref, ref64, _ = _run("tgv", ov, seed=3)
+    G.assert_within_noise_floor(out, ref, ref64, TOL, label=kernel)
 
 
 @pytest.mark.parametrize("kernel", ["CSK", "QSK", "WC2K", "WC4K", "WC6K", "GK", "SGK"])
@@ -105,4 +110,5 @@ def test_kernel_eval_matches_oracle(kernel, dim):
     rn = r.cpu().numpy()
     w, dw = k.w(r).cpu().numpy(), k.grad_w(r).cpu().numpy()
     assert G.rel_err(w, ko.w(rn)) < 2e-6
-    assert G.rel_err(dw, ko.grad_w(rn)) < 2e-6
+    # dW/dr cancels near q = 0 (81 - 96 + 15 for the quintic): a few ulps of its terms
+    assert G.rel_err(dw, ko.grad_w(rn)) < 1e-5
