@@ -248,61 +248,77 @@ struct BuildArgs {
   // and the exact reference metric, and is decided by the exact predicate.
   float lo, hi;
   int fast;
-  int n, npad, width, mask_self;
+  int n, npad, lane_width, mask_self;
 };
 
-// Candidates [lo, hi) of one cell range whose image is shifted by `img`
-// (0 or +-box per axis).  Hits are collected 32 at a time in a bit mask so
-// the divergent stores happen once per hit, in candidate order.
-template <int DIM, bool ORDERED>
-__device__ __forceinline__ void test_range(const BuildArgs& a, int s, const float4 ps, int sid,
-                                           int lo, int hi, const float* img, int& count) {
+// Lane h (of TPP) tests candidates lo + h, lo + h + TPP, ... < hi of one cell
+// range whose image is shifted by `img` (0 or +-box per axis).  Hits are
+// collected 32 at a time in a bit mask so the divergent stores happen once per
+// hit, in candidate order, into the lane's own sub-list.
+template <int DIM, int TPP, bool ORDERED>
+__device__ __forceinline__ void test_range(const BuildArgs& a, int s, int h, const float4 ps,
+                                           int sid, int lo, int hi, const float* img,
+                                           int32_t* __restrict__ out, int& count) {
   const float sx = ps.x - img[0], sy = ps.y - img[1], sz = ps.z - img[2];
-  for (int b0 = lo; b0 < hi; b0 += 32) {
-    const int nb = hi - b0 < 32 ? hi - b0 : 32;
+  constexpr int G = 8;  // candidates whose loads are issued together
+  for (int c0 = lo + h; c0 < hi; c0 += 32 * TPP) {
+    const int nb = (hi - c0 + TPP - 1) / TPP < 32 ? (hi - c0 + TPP - 1) / TPP : 32;
     unsigned mask = 0u;
-    for (int t = 0; t < nb; ++t) {
-      const int b = b0 + t;
-      const float4 pb = a.pos[b];
-      bool hit;
-      if (a.fast) {
-        const float dx = sx - pb.x, dy = sy - pb.y;
-        float d2 = dx * dx + dy * dy;
-        if (DIM == 3) {
-          const float dz = sz - pb.z;
-          d2 += dz * dz;
-        }
-        hit = d2 < a.lo;
-        if (!hit && d2 < a.hi) hit = metric_sq_exact<DIM>(ps, pb, a.box, a.half) < a.cutoff_sq;
-      } else {
-        hit = metric_sq_exact<DIM>(ps, pb, a.box, a.half) < a.cutoff_sq;
+    for (int t0 = 0; t0 < nb; t0 += G) {
+      float4 pg[G];
+#pragma unroll
+      for (int u = 0; u < G; ++u) {
+        const int b = c0 + (t0 + u) * TPP;
+        pg[u] = a.pos[b < hi ? b : hi - 1];
       }
-      if (a.mask_self && b == s) hit = false;
-      if (ORDERED && hit && !(a.perm[b] < sid)) hit = false;
-      mask |= hit ? (1u << t) : 0u;
+#pragma unroll
+      for (int u = 0; u < G; ++u) {
+        const int t = t0 + u;
+        const int b = c0 + t * TPP;
+        const float4 pb = pg[u];
+        bool hit;
+        if (a.fast) {
+          const float dx = sx - pb.x, dy = sy - pb.y;
+          float d2 = dx * dx + dy * dy;
+          if (DIM == 3) {
+            const float dz = sz - pb.z;
+            d2 += dz * dz;
+          }
+          hit = d2 < a.lo;
+          if (!hit && d2 < a.hi) hit = metric_sq_exact<DIM>(ps, pb, a.box, a.half) < a.cutoff_sq;
+        } else {
+          hit = metric_sq_exact<DIM>(ps, pb, a.box, a.half) < a.cutoff_sq;
+        }
+        if (t >= nb) hit = false;
+        if (a.mask_self && b == s) hit = false;
+        if (ORDERED && hit && !(a.perm[b] < sid)) hit = false;
+        mask |= hit ? (1u << t) : 0u;
+      }
     }
     while (mask) {
       const int t = __ffs(mask) - 1;
       mask &= mask - 1u;
-      if (count < a.width) a.ell[(int64_t)count * a.npad + s] = b0 + t;
+      if (count < a.lane_width) out[(int64_t)count * a.npad] = c0 + t * TPP;
       ++count;
     }
   }
 }
 
-// One thread per sorted particle (the sender): visit the 3^d cells around its
-// own cell as 3^(d-1) contiguous x-strips and keep every receiver with
-// d^2 < cutoff^2.  Pairs land in a column-major per-particle buffer.
-template <int DIM, bool ORDERED>
+// TPP lanes per sorted particle (the sender): each visits the 3^d cells around
+// the particle's own cell as 3^(d-1) contiguous x-strips, keeps its share of
+// the receivers with d^2 < cutoff^2 and stores them in its own sub-list.
+template <int DIM, int TPP, bool ORDERED>
 __global__ __launch_bounds__(kBlock) void k_nl_build(BuildArgs a) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = tid / TPP, h = tid % TPP;
   int count = 0;
   if (s < a.n) {
     const float4 ps = a.pos[s];
     const int sid = ORDERED ? a.perm[s] : 0;
+    int32_t* out = a.ell + ((int64_t)h * a.lane_width) * a.npad + s;
     float img[3] = {0.0f, 0.0f, 0.0f};
     if (!a.g.use_cells) {
-      test_range<DIM, ORDERED>(a, s, ps, sid, 0, a.n, img, count);
+      test_range<DIM, TPP, ORDERED>(a, s, h, ps, sid, 0, a.n, img, out, count);
     } else {
       int c[3];
       cell_coords<DIM>(ps, a.g, c);
@@ -322,29 +338,33 @@ __global__ __launch_bounds__(kBlock) void k_nl_build(BuildArgs a) {
           const int row = (cz * ny + cy) * nx;
           if (c[0] >= 1 && c[0] + 1 < nx) {
             img[0] = 0.0f;
-            test_range<DIM, ORDERED>(a, s, ps, sid, a.cell_start[row + c[0] - 1],
-                                     a.cell_start[row + c[0] + 2], img, count);
+            test_range<DIM, TPP, ORDERED>(a, s, h, ps, sid, a.cell_start[row + c[0] - 1],
+                                          a.cell_start[row + c[0] + 2], img, out, count);
           } else {
             for (int dx = -1; dx <= 1; ++dx) {
               int cx = c[0] + dx;
               img[0] = 0.0f;
               if (cx < 0) { cx += nx; img[0] = -a.box[0]; }
               else if (cx >= nx) { cx -= nx; img[0] = a.box[0]; }
-              test_range<DIM, ORDERED>(a, s, ps, sid, a.cell_start[row + cx],
-                                       a.cell_start[row + cx + 1], img, count);
+              test_range<DIM, TPP, ORDERED>(a, s, h, ps, sid, a.cell_start[row + cx],
+                                            a.cell_start[row + cx + 1], img, out, count);
             }
           }
         }
       }
     }
-    a.cnt[s] = count < a.width ? count : a.width;
-    if (count > a.width) atomicAdd((unsigned long long*)&a.info[5], 1ull);
+    a.cnt[h * a.npad + s] = count < a.lane_width ? count : a.lane_width;
+    if (count > a.lane_width) atomicAdd((unsigned long long*)&a.info[5], 1ull);
   }
-  // block reductions: total pairs and the largest per-particle count
+  // pairs of the particle (sum over its TPP lanes), then block reductions:
+  // total pairs and the largest per-particle count
+  int tot_p = count;
+#pragma unroll
+  for (int o = 1; o < TPP; o <<= 1) tot_p += __shfl_xor(tot_p, o, 64);
   __shared__ unsigned long long tot[kBlock / 64];
   __shared__ int mx[kBlock / 64];
   unsigned long long v = (unsigned long long)count;
-  int m = count;
+  int m = tot_p;
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
     v += __shfl_down(v, o, 64);
@@ -381,24 +401,30 @@ __global__ void k_needs_rebuild(const float* __restrict__ r, const float* __rest
 
 // ----------------------------------------------------------- idx writer
 __global__ void k_counts_orig(const int32_t* __restrict__ perm, const int32_t* __restrict__ cnt,
-                              int64_t* __restrict__ cnt64, int n) {
+                              int64_t* __restrict__ cnt64, int n, int npad, int tpp) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s < n) cnt64[perm[s]] = cnt[s];
+  if (s >= n) return;
+  int64_t c = 0;
+  for (int h = 0; h < tpp; ++h) c += cnt[h * npad + s];
+  cnt64[perm[s]] = c;
 }
 
+// Receivers of sender a in sub-list order (lane 0's list, then lane 1's, ...).
 __global__ void k_write_idx(const int32_t* __restrict__ perm, const int32_t* __restrict__ cnt,
                             const int32_t* __restrict__ ell, const int64_t* __restrict__ off,
-                            int32_t* __restrict__ idx, int64_t max_occ, int n, int npad) {
+                            int32_t* __restrict__ idx, int64_t max_occ, int n, int npad, int tpp,
+                            int lane_width) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const int a = perm[s];
-  const int64_t base = off[a];
-  const int c = cnt[s];
-  for (int k = 0; k < c; ++k) {
-    int64_t p = base + k;
-    if (p >= max_occ) break;
-    idx[p] = perm[ell[(int64_t)k * npad + s]];
-    idx[max_occ + p] = a;
+  int64_t p = off[a];
+  for (int h = 0; h < tpp; ++h) {
+    const int c = cnt[h * npad + s];
+    const int32_t* lst = ell + ((int64_t)h * lane_width) * npad + s;
+    for (int k = 0; k < c && p < max_occ; ++k, ++p) {
+      idx[p] = perm[lst[(int64_t)k * npad]];
+      idx[max_occ + p] = a;
+    }
   }
 }
 
@@ -489,7 +515,7 @@ int build_pairs(sph_ws* ws, int bank, int64_t max_occupancy, int64_t* info, hipS
   a.cutoff_sq = c.cutoff_sq;
   a.n = c.n;
   a.npad = ws->npad;
-  a.width = c.ell_width;
+  a.lane_width = ws->lane_width;
   a.mask_self = c.mask_self;
   // The cell-adjacent image is the minimum image of every pair within the
   // cutoff when each side has >= 5 cells (2 cells < box/2).  Both the fast and
@@ -506,13 +532,19 @@ int build_pairs(sph_ws* ws, int bank, int64_t max_occupancy, int64_t* info, hipS
   const float band = 64.0f * c.cutoff * ulp_box + 16.0f * ulp_c2;
   a.lo = c.cutoff_sq - band;
   a.hi = c.cutoff_sq + band;
-  dim3 grid(cdiv(c.n, kBlock));
+  constexpr int T2 = tpp_of(2), T3 = tpp_of(3);
   if (c.dim == 2) {
-    if (c.ordered) SPH_LAUNCH(ws, "nl_build", st, (k_nl_build<2, true><<<grid, kBlock, 0, st>>>(a)));
-    else SPH_LAUNCH(ws, "nl_build", st, (k_nl_build<2, false><<<grid, kBlock, 0, st>>>(a)));
+    dim3 grid(cdiv((int64_t)c.n * T2, kBlock));
+    if (c.ordered)
+      SPH_LAUNCH(ws, "nl_build", st, (k_nl_build<2, T2, true><<<grid, kBlock, 0, st>>>(a)));
+    else
+      SPH_LAUNCH(ws, "nl_build", st, (k_nl_build<2, T2, false><<<grid, kBlock, 0, st>>>(a)));
   } else {
-    if (c.ordered) SPH_LAUNCH(ws, "nl_build", st, (k_nl_build<3, true><<<grid, kBlock, 0, st>>>(a)));
-    else SPH_LAUNCH(ws, "nl_build", st, (k_nl_build<3, false><<<grid, kBlock, 0, st>>>(a)));
+    dim3 grid(cdiv((int64_t)c.n * T3, kBlock));
+    if (c.ordered)
+      SPH_LAUNCH(ws, "nl_build", st, (k_nl_build<3, T3, true><<<grid, kBlock, 0, st>>>(a)));
+    else
+      SPH_LAUNCH(ws, "nl_build", st, (k_nl_build<3, T3, false><<<grid, kBlock, 0, st>>>(a)));
   }
   if (max_occupancy >= 0)
     SPH_LAUNCH(ws, "finalize", st, k_finalize<<<1, 64, 0, st>>>(info, max_occupancy));
@@ -524,7 +556,8 @@ int pair_offsets(sph_ws* ws, int bank, hipStream_t st) {
   const int n = ws->cfg.n;
   Bank& B = ws->bank[bank];
   SPH_LAUNCH(ws, "counts_orig", st,
-             k_counts_orig<<<cdiv(n, kBlock), kBlock, 0, st>>>(B.perm, B.cnt, ws->cnt64, n));
+             k_counts_orig<<<cdiv(n, kBlock), kBlock, 0, st>>>(B.perm, B.cnt, ws->cnt64, n,
+                                                             ws->npad, ws->tpp));
   SPH_LAUNCH(ws, "scan_pairs", st,
              (k_scan_single<int64_t, int64_t><<<1, kScanThreads, 0, st>>>(
                  ws->cnt64, n, ws->off64, nullptr, nullptr, -1)));
@@ -635,7 +668,8 @@ extern "C" int sph_nl_write_idx(sph_ws* ws, int bank, int32_t* idx, int64_t max_
   Bank& B = ws->bank[bank];
   SPH_LAUNCH(ws, "write_idx", st,
              k_write_idx<<<cdiv(n, kBlock), kBlock, 0, st>>>(B.perm, B.cnt, B.ell, ws->off64, idx,
-                                                           max_occupancy, n, ws->npad));
+                                                           max_occupancy, n, ws->npad, ws->tpp,
+                                                           ws->lane_width));
   SPH_LAUNCH(ws, "pad_idx", st,
              k_pad_idx<<<cdiv(max_occupancy, kBlock), kBlock, 0, st>>>(idx, max_occupancy,
                                                                       ws->off64 + n, n));

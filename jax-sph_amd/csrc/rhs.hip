@@ -118,12 +118,44 @@ struct SweepArgs {
   const int32_t* ell;
   const int32_t* cnt;
   Geo geo;
-  int n, npad;
+  int n, npad, lane_width;
   // which record set the force sweep reads
   const float4 *Uf, *Vf, *Qf;
   int heat_in_density;  // apply T += dt*dTdt in the density epilogue
   sph_bc_spec bc;
 };
+
+// Visits the pairs of sorted particle s.  Pair indices are fetched in batches
+// of 8 independent (lane-coalesced) loads so their latency is paid once per
+// batch; the body is unrolled by two so gathers of one pair overlap the
+// arithmetic of the other.
+template <typename F>
+__device__ __forceinline__ void for_each_pair(const SweepArgs& a, int s, int h, F&& f) {
+  constexpr int B = 8;
+  const int c = a.cnt[h * a.npad + s];
+  const int32_t* lst = a.ell + ((int64_t)h * a.lane_width) * a.npad + s;
+  for (int k0 = 0; k0 < c; k0 += B) {
+    int jb[B];
+#pragma unroll
+    for (int t = 0; t < B; ++t) jb[t] = (k0 + t < c) ? lst[(int64_t)(k0 + t) * a.npad] : -1;
+#pragma unroll 2
+    for (int t = 0; t < B; ++t)
+      if (jb[t] >= 0) f(jb[t]);
+  }
+}
+
+// Sum of x over the TPP consecutive lanes serving one particle.
+template <int TPP>
+__device__ __forceinline__ float lane_sum(float x) {
+#pragma unroll
+  for (int o = 1; o < TPP; o <<= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
+
+template <int TPP>
+__device__ __forceinline__ V3 lane_sum(V3 v) {
+  return V3{lane_sum<TPP>(v.x), lane_sum<TPP>(v.y), lane_sum<TPP>(v.z)};
+}
 
 // ------------------------------------------------------------- gather
 // Original-order state -> cell-sorted records (P, U0, V0, Q0, NW).
@@ -160,8 +192,10 @@ __device__ __forceinline__ V3 g_ext_of(const SweepArgs& a, int i) {
 // then the EOS (solver.py:801).
 template <int DIM, int KIND, int MODE>
 __global__ __launch_bounds__(kB) void k_density(SweepArgs a) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= a.n) return;
+  constexpr int TPP = tpp_of(DIM);
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = tid / TPP, h = tid % TPP;
+  if (s >= a.n) return;  // whole lane groups leave together
   const sph_solver_params& p = a.prm;
   const KernelEval<KIND, DIM> K{p.kernel, p.sigma, p.one_over_h};
   const Eos eos = make_eos(p);
@@ -169,15 +203,12 @@ __global__ __launch_bounds__(kB) void k_density(SweepArgs a) {
   const int tag_i = (int)Qi.z;
   const bool fluid_i = tag_i == kFluid;
   const bool want_ut = p.solver == 1 && p.bc_trick && !p.free_slip && is_wall(tag_i);
-  const int c = a.cnt[s];
   const V3 ui = v3(Ui);
   float wsum = 0.0f, acc = 0.0f, utw = 0.0f;
   V3 ut = {0.f, 0.f, 0.f};
   V3 gi = {0.f, 0.f, 0.f};
-  float4 NWi = make_float4(0.f, 0.f, 0.f, 0.f);
   if (MODE == 2) gi = g_ext_of<DIM>(a, a.perm[s]);
-  for (int k = 0; k < c; ++k) {
-    const int j = a.ell[(int64_t)k * a.npad + s];
+  for_each_pair(a, s, h, [&](const int j) {
     const float4 Pj = a.R.P[j];
     const V3 dr = min_image<DIM>(Pi, Pj, a.geo);
     const float d2 = dot<DIM>(dr, dr);
@@ -229,7 +260,17 @@ __global__ __launch_bounds__(kB) void k_density(SweepArgs a) {
         ut = ut + w * v3(a.R.U0[j]);
       }
     }
+  });
+  if (MODE == 0) {
+    wsum = lane_sum<TPP>(wsum);
+  } else {
+    acc = lane_sum<TPP>(acc);
   }
+  if (p.solver == 1 && p.bc_trick) {
+    utw = lane_sum<TPP>(utw);
+    ut = lane_sum<TPP>(ut);
+  }
+  if (h != 0) return;
   float rho = Ui.w, drhodt;
   const int i = a.perm[s];
   if (MODE == 0) {
@@ -262,8 +303,10 @@ __global__ __launch_bounds__(kB) void k_density(SweepArgs a) {
 // followed by T += dt * dTdt when heat conduction is on (solver.py:834).
 template <int DIM, int KIND, int MODE>
 __global__ __launch_bounds__(kB) void k_walls(SweepArgs a) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= a.n) return;
+  constexpr int TPP = tpp_of(DIM);
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = tid / TPP, h = tid % TPP;
+  if (s >= a.n) return;  // whole lane groups leave together
   const sph_solver_params& p = a.prm;
   const KernelEval<KIND, DIM> K{p.kernel, p.sigma, p.one_over_h};
   const Eos eos = make_eos(p);
@@ -275,11 +318,9 @@ __global__ __launch_bounds__(kB) void k_walls(SweepArgs a) {
   if (is_wall(tag_i)) {
     float ws = 0.f, sp = 0.f, sT = 0.f;
     V3 su = {0.f, 0.f, 0.f}, sv = {0.f, 0.f, 0.f}, srr = {0.f, 0.f, 0.f};
-    const int c = a.cnt[s];
-    for (int k = 0; k < c; ++k) {
-      const int j = a.ell[(int64_t)k * a.npad + s];
+    for_each_pair(a, s, h, [&](const int j) {
       const float4 Qj = a.R.Q1[j];
-      if ((int)Qj.z != kFluid) continue;
+      if ((int)Qj.z != kFluid) return;
       const float4 Pj = a.R.P[j];
       const V3 dr = min_image<DIM>(Pi, Pj, a.geo);
       const float d = sqrtf(dot<DIM>(dr, dr));
@@ -293,7 +334,15 @@ __global__ __launch_bounds__(kB) void k_walls(SweepArgs a) {
         srr = srr + (Uj.w * w) * dr;
       }
       if (p.heat_conduction) sT += w * Qj.w;
+    });
+    ws = lane_sum<TPP>(ws);
+    if (MODE == 0) {
+      su = lane_sum<TPP>(su);
+      sv = lane_sum<TPP>(sv);
+      sp = lane_sum<TPP>(sp);
+      srr = lane_sum<TPP>(srr);
     }
+    if (p.heat_conduction) sT = lane_sum<TPP>(sT);
     const float inv = 1.0f / (ws + kEps);
     if (MODE == 0) {
       V3 uw = inv * su, vw = inv * sv;
@@ -309,6 +358,7 @@ __global__ __launch_bounds__(kB) void k_walls(SweepArgs a) {
     }
     if (p.heat_conduction && (tag_i == kSolidWall || tag_i == kMovingWall)) T = sT * inv;
   }
+  if (h != 0) return;
   const float rho = MODE == 0 ? eos.rho_fn(pr) : Ui.w;
   if (p.heat_conduction) T = T + p.dt * a.in.dTdt[i];
   a.R.U2[s] = f4(u, rho);
@@ -324,8 +374,10 @@ __global__ __launch_bounds__(kB) void k_walls(SweepArgs a) {
 // dudt + g_ext (solver.py:936) and the case boundary hook (integrator.py:54).
 template <int DIM, int KIND, int SOLVER, bool TVF>
 __global__ __launch_bounds__(kB) void k_force(SweepArgs a) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= a.n) return;
+  constexpr int TPP = tpp_of(DIM);
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = tid / TPP, h = tid % TPP;
+  if (s >= a.n) return;  // whole lane groups leave together
   const sph_solver_params& p = a.prm;
   const KernelEval<KIND, DIM> K{p.kernel, p.sigma, p.one_over_h};
   const Eos eos = make_eos(p);
@@ -349,13 +401,11 @@ __global__ __launch_bounds__(kB) void k_force(SweepArgs a) {
   V3 gi = g_ext_of<DIM>(a, i);
   V3 acc = {0.f, 0.f, 0.f}, accv = {0.f, 0.f, 0.f};
   float dT = 0.0f;
-  const int c = a.cnt[s];
-  for (int k = 0; k < c; ++k) {
-    const int j = a.ell[(int64_t)k * a.npad + s];
+  for_each_pair(a, s, h, [&](const int j) {
     const float4 Pj = a.R.P[j];
     const V3 dr = min_image<DIM>(Pi, Pj, a.geo);
     const float d2 = dot<DIM>(dr, dr);
-    if (d2 == 0.0f) continue;  // self pair / coincident: every term is exactly 0
+    if (d2 == 0.0f) return;  // self pair / coincident: every term is exactly 0
     const float d = sqrtf(d2);
     const float q = d * p.one_over_h;
     const float gw = K.dw(q);
@@ -431,7 +481,11 @@ __global__ __launch_bounds__(kB) void k_force(SweepArgs a) {
       const float F = dot<DIM>(dr, (gw * inv_d) * dr) / (d2 + kEps);
       dT += 4.0f * mj * keff * (Ti - Qj.w) * F / (Cp_i * rhoi * rhoj);
     }
-  }
+  });
+  acc = lane_sum<TPP>(acc);
+  accv = lane_sum<TPP>(accv);
+  if (heat) dT = lane_sum<TPP>(dT);
+  if (h != 0) return;
   V3 dudt = acc + gi;
   V3 dvdt = accv;
   V3 u_out = ui, v_out = vi;
@@ -521,7 +575,7 @@ Geo make_geo(const float* box) {
 template <int DIM, int KIND>
 int launch_sweeps(sph_ws* ws, SweepArgs& a, hipStream_t st) {
   const int n = a.n;
-  dim3 grid((unsigned)cdivl(n, kB));
+  dim3 grid((unsigned)cdivl((int64_t)n * tpp_of(DIM), kB));
   const auto& p = a.prm;
   // density
   if (!p.density_evolution)
@@ -600,6 +654,7 @@ SweepArgs make_args(sph_ws* ws, int bank, const sph_solver_params* prm, const sp
   a.geo = make_geo(prm->box);
   a.n = ws->cfg.n;
   a.npad = ws->npad;
+  a.lane_width = ws->lane_width;
   a.Uf = a.R.U1; a.Vf = a.R.V0; a.Qf = a.R.Q1;
   a.heat_in_density = prm->heat_conduction && !prm->bc_trick;
   a.bc.kind = 0;

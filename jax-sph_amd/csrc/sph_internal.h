@@ -17,9 +17,17 @@ struct Bank {
   int32_t* iperm = nullptr;       // original id -> sorted slot
   int32_t* cell_start = nullptr;  // ncell + 1
   float4* pos = nullptr;          // sorted positions the pairs were built from
-  int32_t* ell = nullptr;         // pair buffer, column-major [ell_width][npad]
-  int32_t* cnt = nullptr;         // pairs per sorted particle
+  // Pair buffer: each sorted particle s owns kTpp sub-lists (one per lane that
+  // serves it); entry k of sub-list h lives at ((h * lane_width + k) * npad + s).
+  int32_t* ell = nullptr;
+  int32_t* cnt = nullptr;         // [kTpp][npad] entries per sub-list
 };
+
+// Lanes that serve one particle in the pair build and the sweeps.  At the
+// particle counts of interest one lane per particle leaves a few waves per
+// SIMD; several lanes per particle (combined with cross-lane shuffles) give the
+// scheduler enough independent gathers to hide memory latency.
+__host__ __device__ constexpr int tpp_of(int dim) { return dim == 3 ? 4 : 2; }
 
 struct ProfEntry {
   const char* name;
@@ -38,6 +46,8 @@ struct sph_ws {
   sph_nl_config cfg;
   int64_t ncell = 0;
   int32_t npad = 0;        // N rounded up to 64 (pair-buffer column stride)
+  int32_t tpp = 1;         // lanes per particle (tpp_of(dim))
+  int32_t lane_width = 0;  // capacity of one sub-list
   int64_t bytes = 0;
   // binning scratch (shared by both banks)
   int32_t* cell_count = nullptr;  // ncell

@@ -99,6 +99,15 @@ extern "C" int sph_ws_create(const sph_nl_config* cfg, sph_ws** out) {
   ws->cfg = *cfg;
   ws->ncell = ncell;
   ws->npad = (int32_t)npad;
+  // Lane h of a particle scans every tpp-th candidate of each cell range, so
+  // it keeps at most ceil(len / tpp) of each of the 3^dim ranges.
+  ws->tpp = tpp_of(cfg->dim);
+  {
+    const int64_t nr = cfg->use_cells ? (cfg->dim == 3 ? 27 : 9) : 1;
+    int64_t lw = (cfg->ell_width + ws->tpp - 1) / ws->tpp + nr;
+    const int64_t lw_n = (N + ws->tpp - 1) / ws->tpp + 1;
+    ws->lane_width = (int32_t)(lw < lw_n ? lw : lw_n);
+  }
   int rc = SPH_OK;
 #define A(ptr, n) if (rc == SPH_OK) rc = alloc(ws, &(ptr), (size_t)(n))
   A(ws->cell_count, ncell);
@@ -113,8 +122,8 @@ extern "C" int sph_ws_create(const sph_nl_config* cfg, sph_ws** out) {
     A(ws->bank[b].iperm, N);
     A(ws->bank[b].cell_start, ncell + 1);
     A(ws->bank[b].pos, N);
-    A(ws->bank[b].ell, (int64_t)cfg->ell_width * npad);
-    A(ws->bank[b].cnt, N);
+    A(ws->bank[b].ell, (int64_t)ws->tpp * ws->lane_width * npad);
+    A(ws->bank[b].cnt, (int64_t)ws->tpp * npad);
   }
   A(ws->P, N);
   A(ws->U0, N); A(ws->U1, N); A(ws->U2, N);
