@@ -42,7 +42,21 @@ def parse():
     ap.add_argument("--n-side", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-side", type=int, default=20)
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="only run a device-to-device copy of this many MiB (5 times), to "
+                         "scale hardware byte counters; prints nothing else")
     return ap.parse_args()
+
+
+def copy_calibration(mb):
+    import torch
+    n = mb * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_mib": mb, "launches": 5}))
 
 
 def cpu_baseline(n_side, seconds=15.0):
@@ -90,6 +104,9 @@ def cpu_baseline(n_side, seconds=15.0):
 
 def main():
     a = parse()
+    if a.copy_calibration_mb:
+        copy_calibration(a.copy_calibration_mb)
+        return
     if a.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not "
                                    "part of it (--gpus must be 1)"}))

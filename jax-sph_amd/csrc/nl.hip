@@ -27,13 +27,6 @@ __global__ void k_fill(T* __restrict__ a, int64_t n, T v) {
   if (i < n) a[i] = v;
 }
 
-__global__ void k_info_init(int64_t* __restrict__ info, const int64_t* __restrict__ prev) {
-  if (threadIdx.x == 0) {
-    info[0] = prev ? prev[0] : 0;
-    for (int k = 1; k < 8; ++k) info[k] = 0;
-  }
-}
-
 struct Grid {
   float cs[3];   // cell size (fp32)
   int cps[3];    // cells per side
@@ -74,6 +67,27 @@ __device__ __forceinline__ float4 load_pos(const float* __restrict__ r, int64_t 
 }
 
 // ----------------------------------------------------------- binning kernels
+// atomicAdd(&counter[key], 1) for every active lane, with one atomic per
+// distinct key in the wavefront (neighbouring particles share cells, so a
+// wave holds only a few keys).  Returns the lane's slot.
+__device__ __forceinline__ int wave_aggregated_inc(int32_t* __restrict__ counter, int key) {
+  const int lane = threadIdx.x & 63;
+  const unsigned long long lt = (1ull << lane) - 1ull;
+  unsigned long long active = __ballot(1);
+  int slot = 0;
+  while (active) {
+    const int leader = __ffsll((long long)active) - 1;
+    const int lkey = __shfl(key, leader, 64);
+    const unsigned long long same = __ballot(key == lkey) & active;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&counter[lkey], __popcll(same));
+    base = __shfl(base, leader, 64);
+    if (key == lkey) slot = base + __popcll(same & lt);
+    active &= ~same;
+  }
+  return slot;
+}
+
 template <int DIM>
 __global__ void k_hash_count(const float* __restrict__ r, int n, Grid g,
                              int32_t* __restrict__ cell_count, int32_t* __restrict__ pcell,
@@ -86,11 +100,48 @@ __global__ void k_hash_count(const float* __restrict__ r, int n, Grid g,
     cell_coords<DIM>(load_pos<DIM>(r, i), g, c);
     h = cell_hash<DIM>(c, g);
   }
-  const int slot = atomicAdd(&cell_count[h], 1);
+  const int slot = wave_aggregated_inc(cell_count, h);
   if (pcell) {
     pcell[i] = h;
     pslot[i] = slot;
   }
+}
+
+// si_euler kick + transport velocity + drift + periodic wrap (integrator.py:26-30)
+// fused with the cell hash and histogram of the new position.
+template <int DIM>
+__global__ void k_integrate_hash(const float* __restrict__ r, const float* __restrict__ u,
+                                 const float* __restrict__ dudt, const float* __restrict__ dvdt,
+                                 float* __restrict__ r_out, float* __restrict__ u_out,
+                                 float* __restrict__ v_out, int n, float dt, float ctvf, Grid g,
+                                 float bx, float by, float bz, int32_t* __restrict__ cell_count,
+                                 int32_t* __restrict__ pcell, int32_t* __restrict__ pslot) {
+#pragma clang fp contract(off)
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float side[3] = {bx, by, bz};
+  float x[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    const int64_t k = (int64_t)i * DIM + d;
+    const float uu = u[k] + dt * dudt[k];
+    const float vv = uu + ctvf * dvdt[k];
+    float xx = r[k] + dt * vv;
+    if (xx >= side[d] || xx < 0.0f) xx = jmod(xx, side[d]);
+    u_out[k] = uu;
+    v_out[k] = vv;
+    r_out[k] = xx;
+    x[d] = xx;
+  }
+  int h = 0;
+  if (g.use_cells) {
+    int c[3];
+    cell_coords<DIM>(make_float4(x[0], x[1], x[2], 0.0f), g, c);
+    h = cell_hash<DIM>(c, g);
+  }
+  const int slot = wave_aggregated_inc(cell_count, h);
+  pcell[i] = h;
+  pslot[i] = slot;
 }
 
 __global__ void k_max_reduce(const int32_t* __restrict__ a, int64_t n, int64_t* __restrict__ out) {
@@ -278,12 +329,7 @@ __device__ __forceinline__ void test_range(const BuildArgs& a, int s, int h, con
         const float4 pb = pg[u];
         bool hit;
         if (a.fast) {
-          const float dx = sx - pb.x, dy = sy - pb.y;
-          float d2 = dx * dx + dy * dy;
-          if (DIM == 3) {
-            const float dz = sz - pb.z;
-            d2 += dz * dz;
-          }
+          const float d2 = dist2<DIM>(sx - pb.x, sy - pb.y, sz - pb.z);
           hit = d2 < a.lo;
           if (!hit && d2 < a.hi) hit = metric_sq_exact<DIM>(ps, pb, a.box, a.half) < a.cutoff_sq;
         } else {
@@ -460,24 +506,50 @@ Grid make_grid(const sph_nl_config& c) {
 }
 
 // Bins r into `bank`: hash + histogram, scan, scatter, per-cell sort, positions.
-int bin_particles(sph_ws* ws, int bank, const float* r, const int64_t* prev_info, int64_t* info,
-                  hipStream_t st) {
+__global__ void k_bin_init(int64_t* __restrict__ info, const int64_t* __restrict__ prev,
+                           int32_t* __restrict__ count, int64_t ncell) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < ncell) count[i] = 0;
+  if (i == 0) {
+    info[0] = prev ? prev[0] : 0;
+    for (int k = 1; k < 8; ++k) info[k] = 0;
+  }
+}
+
+// Resets the error block and the cell histogram.
+int bin_init(sph_ws* ws, const int64_t* prev_info, int64_t* info, hipStream_t st) {
+  SPH_LAUNCH(ws, "bin_init", st,
+             k_bin_init<<<cdiv(ws->ncell, kBlock), kBlock, 0, st>>>(info, prev_info,
+                                                                   ws->cell_count, ws->ncell));
+  return SPH_OK;
+}
+
+// Kick/drift/wrap of the state fused with the histogram of the new positions.
+int integrate_and_hash(sph_ws* ws, const float* r, const float* u, const float* dudt,
+                       const float* dvdt, float* r_out, float* u_out, float* v_out, float dt,
+                       float ctvf, const float* box, hipStream_t st) {
+  const auto& c = ws->cfg;
+  const int n = c.n;
+  Grid g = make_grid(c);
+  if (c.dim == 2)
+    SPH_LAUNCH(ws, "integrate_hash", st,
+               k_integrate_hash<2><<<cdiv(n, kBlock), kBlock, 0, st>>>(
+                   r, u, dudt, dvdt, r_out, u_out, v_out, n, dt, ctvf, g, box[0], box[1], 0.0f,
+                   ws->cell_count, ws->pcell, ws->pslot));
+  else
+    SPH_LAUNCH(ws, "integrate_hash", st,
+               k_integrate_hash<3><<<cdiv(n, kBlock), kBlock, 0, st>>>(
+                   r, u, dudt, dvdt, r_out, u_out, v_out, n, dt, ctvf, g, box[0], box[1], box[2],
+                   ws->cell_count, ws->pcell, ws->pslot));
+  return SPH_OK;
+}
+
+// Scan of the histogram, scatter, per-cell id sort; then (if r is given) the
+// sorted positions the pair build reads.
+int bin_finish(sph_ws* ws, int bank, const float* r, int64_t* info, hipStream_t st) {
   const auto& c = ws->cfg;
   const int n = c.n;
   Bank& B = ws->bank[bank];
-  Grid g = make_grid(c);
-  SPH_LAUNCH(ws, "info_init", st, k_info_init<<<1, 64, 0, st>>>(info, prev_info));
-  SPH_LAUNCH(ws, "fill", st,
-             k_fill<int32_t><<<cdiv(ws->ncell, kBlock), kBlock, 0, st>>>(ws->cell_count,
-                                                                        ws->ncell, 0));
-  if (c.dim == 2)
-    SPH_LAUNCH(ws, "hash_count", st,
-               k_hash_count<2><<<cdiv(n, kBlock), kBlock, 0, st>>>(r, n, g, ws->cell_count,
-                                                                 ws->pcell, ws->pslot));
-  else
-    SPH_LAUNCH(ws, "hash_count", st,
-               k_hash_count<3><<<cdiv(n, kBlock), kBlock, 0, st>>>(r, n, g, ws->cell_count,
-                                                                 ws->pcell, ws->pslot));
   SPH_LAUNCH(ws, "scan_cells", st,
              (k_scan_single<int32_t, int32_t><<<1, kScanThreads, 0, st>>>(
                  ws->cell_count, ws->ncell, B.cell_start, &info[2], info,
@@ -488,6 +560,7 @@ int bin_particles(sph_ws* ws, int bank, const float* r, const int64_t* prev_info
   SPH_LAUNCH(ws, "cell_sort", st,
              k_cell_sort<<<cdiv(ws->ncell * 64, 256), 256, 0, st>>>(B.cell_start, ws->unsorted,
                                                                     B.perm, B.iperm, ws->ncell));
+  if (!r) return SPH_OK;
   if (c.dim == 2)
     SPH_LAUNCH(ws, "gather_pos", st,
                k_gather_pos<2><<<cdiv(n, kBlock), kBlock, 0, st>>>(r, B.perm, B.pos, n));
@@ -495,6 +568,24 @@ int bin_particles(sph_ws* ws, int bank, const float* r, const int64_t* prev_info
     SPH_LAUNCH(ws, "gather_pos", st,
                k_gather_pos<3><<<cdiv(n, kBlock), kBlock, 0, st>>>(r, B.perm, B.pos, n));
   return SPH_OK;
+}
+
+int bin_particles(sph_ws* ws, int bank, const float* r, const int64_t* prev_info, int64_t* info,
+                  hipStream_t st) {
+  const auto& c = ws->cfg;
+  const int n = c.n;
+  Grid g = make_grid(c);
+  int rc = bin_init(ws, prev_info, info, st);
+  if (rc) return rc;
+  if (c.dim == 2)
+    SPH_LAUNCH(ws, "hash_count", st,
+               k_hash_count<2><<<cdiv(n, kBlock), kBlock, 0, st>>>(r, n, g, ws->cell_count,
+                                                                 ws->pcell, ws->pslot));
+  else
+    SPH_LAUNCH(ws, "hash_count", st,
+               k_hash_count<3><<<cdiv(n, kBlock), kBlock, 0, st>>>(r, n, g, ws->cell_count,
+                                                                 ws->pcell, ws->pslot));
+  return bin_finish(ws, bank, r, info, st);
 }
 
 int build_pairs(sph_ws* ws, int bank, int64_t max_occupancy, int64_t* info, hipStream_t st) {

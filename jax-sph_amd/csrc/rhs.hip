@@ -17,8 +17,11 @@ constexpr int kB = 256;
 static inline int64_t cdivl(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 // Bank helpers from nl.hip
-int bin_particles(sph_ws* ws, int bank, const float* r, const int64_t* prev_info, int64_t* info,
-                  hipStream_t st);
+int bin_init(sph_ws* ws, const int64_t* prev_info, int64_t* info, hipStream_t st);
+int integrate_and_hash(sph_ws* ws, const float* r, const float* u, const float* dudt,
+                       const float* dvdt, float* r_out, float* u_out, float* v_out, float dt,
+                       float ctvf, const float* box, hipStream_t st);
+int bin_finish(sph_ws* ws, int bank, const float* r, int64_t* info, hipStream_t st);
 int build_pairs(sph_ws* ws, int bank, int64_t max_occupancy, int64_t* info, hipStream_t st);
 
 // ---------------------------------------------------------- kernel dispatch
@@ -211,7 +214,7 @@ __global__ __launch_bounds__(kB) void k_density(SweepArgs a) {
   for_each_pair(a, s, h, [&](const int j) {
     const float4 Pj = a.R.P[j];
     const V3 dr = min_image<DIM>(Pi, Pj, a.geo);
-    const float d2 = dot<DIM>(dr, dr);
+    const float d2 = dist2<DIM>(dr.x, dr.y, dr.z);
     const float d = sqrtf(d2);
     const float q = d * p.one_over_h;
     if (MODE == 0) {
@@ -323,7 +326,7 @@ __global__ __launch_bounds__(kB) void k_walls(SweepArgs a) {
       if ((int)Qj.z != kFluid) return;
       const float4 Pj = a.R.P[j];
       const V3 dr = min_image<DIM>(Pi, Pj, a.geo);
-      const float d = sqrtf(dot<DIM>(dr, dr));
+      const float d = sqrtf(dist2<DIM>(dr.x, dr.y, dr.z));
       const float w = K.w(d * p.one_over_h);
       ws += w;
       if (MODE == 0) {
@@ -404,7 +407,7 @@ __global__ __launch_bounds__(kB) void k_force(SweepArgs a) {
   for_each_pair(a, s, h, [&](const int j) {
     const float4 Pj = a.R.P[j];
     const V3 dr = min_image<DIM>(Pi, Pj, a.geo);
-    const float d2 = dot<DIM>(dr, dr);
+    const float d2 = dist2<DIM>(dr.x, dr.y, dr.z);
     if (d2 == 0.0f) return;  // self pair / coincident: every term is exactly 0
     const float d = sqrtf(d2);
     const float q = d * p.one_over_h;
@@ -696,30 +699,25 @@ extern "C" int sph_step(sph_ws* ws, int bank, const sph_solver_params* prm, floa
   if (prm->dim != ws->cfg.dim) { set_error("sph_step: dim mismatch"); return SPH_ERR_ARG; }
   if (in->g_ext) { set_error("sph_step: position-dependent g_ext is not fused"); return SPH_ERR_ARG; }
   const int n = ws->cfg.n, dim = prm->dim;
-  // 1. kick/drift into the output buffers
-  const int64_t nk = (int64_t)n * dim;
-  if (dim == 2)
-    SPH_LAUNCH(ws, "integrate", st,
-               k_integrate<2><<<(unsigned)cdivl(nk, kB), kB, 0, st>>>(
-                   in->r, in->u, in->dudt, in->dvdt, r_out, out->u, out->v, n, dt, tvf_coef,
-                   prm->box[0], prm->box[1], prm->box[2]));
-  else
-    SPH_LAUNCH(ws, "integrate", st,
-               k_integrate<3><<<(unsigned)cdivl(nk, kB), kB, 0, st>>>(
-                   in->r, in->u, in->dudt, in->dvdt, r_out, out->u, out->v, n, dt, tvf_coef,
-                   prm->box[0], prm->box[1], prm->box[2]));
-  // 2. neighbour update at the new positions
-  rc = bin_particles(ws, bank, r_out, prev_info, info, st);
+  // 1. kick/drift into the output buffers, binned at the new positions
+  rc = bin_init(ws, prev_info, info, st);
   if (rc) return rc;
+  rc = integrate_and_hash(ws, in->r, in->u, in->dudt, in->dvdt, r_out, out->u, out->v, dt,
+                          tvf_coef, prm->box, st);
+  if (rc) return rc;
+  rc = bin_finish(ws, bank, nullptr, info, st);
+  if (rc) return rc;
+  // 2. sorted records (also the positions of the pair build), then the pairs
   SweepArgs a = make_args(ws, bank, prm, in, out);
   if (bc) a.bc = *bc;
   dim3 grid((unsigned)cdivl(n, kB));
+  float4* nl_pos = ws->bank[bank].pos;
   if (dim == 2)
     SPH_LAUNCH(ws, "gather_state", st,
-               k_gather_state<2><<<grid, kB, 0, st>>>(a, r_out, out->u, out->v, nullptr));
+               k_gather_state<2><<<grid, kB, 0, st>>>(a, r_out, out->u, out->v, nl_pos));
   else
     SPH_LAUNCH(ws, "gather_state", st,
-               k_gather_state<3><<<grid, kB, 0, st>>>(a, r_out, out->u, out->v, nullptr));
+               k_gather_state<3><<<grid, kB, 0, st>>>(a, r_out, out->u, out->v, nl_pos));
   rc = build_pairs(ws, bank, max_occupancy, info, st);
   if (rc) return rc;
   // 3. forward + case hook

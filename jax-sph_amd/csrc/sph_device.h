@@ -57,11 +57,23 @@ __device__ __forceinline__ float metric_sq_exact(const float4 s, const float4 r,
   return acc;
 }
 
-// Minimum-image displacement for the physics sweeps: same image as the exact
-// formula for every pair inside the cutoff, rounded differently by <= 1 ulp(side).
+// Minimum-image displacement for the physics sweeps: (a - k*side) - b with
+// k = rint((a - b)/side).  For every pair inside the cutoff this is the same
+// image as the exact formula (rounded differently by <= 1 ulp(side)) and
+// bit-identical to the pair build's (a - image) - b, so fused and separate
+// sweeps see the same distances.
 __device__ __forceinline__ float disp_fast(float a, float b, float side, float inv_side) {
-  float d = a - b;
-  return fmaf(-side, rintf(d * inv_side), d);
+#pragma clang fp contract(off)
+  const float k = rintf((a - b) * inv_side);
+  return (a - k * side) - b;
+}
+
+// x^2 + y^2 (+ z^2) with explicit fused multiply-adds (same bits everywhere).
+template <int DIM>
+__device__ __forceinline__ float dist2(float dx, float dy, float dz) {
+  float d2 = fmaf(dy, dy, dx * dx);
+  if (DIM == 3) d2 = fmaf(dz, dz, d2);
+  return d2;
 }
 
 // -------------------------------------------------------------- kernels
