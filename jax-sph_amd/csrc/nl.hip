@@ -306,10 +306,17 @@ struct BuildArgs {
 // range whose image is shifted by `img` (0 or +-box per axis).  Hits are
 // collected 32 at a time in a bit mask so the divergent stores happen once per
 // hit, in candidate order, into the lane's own sub-list.
+// The first kStage hits of each lane are staged in LDS (one column per
+// thread) and written out row by row at the end, so every pair-buffer row is
+// written by whole wavefronts at once (coalesced, no partially written lines
+// left for the L2 to evict); later hits go straight to the buffer.
+constexpr int kStage = 32;
+
 template <int DIM, int TPP, bool ORDERED>
 __device__ __forceinline__ void test_range(const BuildArgs& a, int s, int h, const float4 ps,
                                            int sid, int lo, int hi, const float* img,
-                                           int32_t* __restrict__ out, int& count) {
+                                           int32_t* __restrict__ out, int32_t* stage,
+                                           int& count) {
   const float sx = ps.x - img[0], sy = ps.y - img[1], sz = ps.z - img[2];
   constexpr int G = 8;  // candidates whose loads are issued together
   for (int c0 = lo + h; c0 < hi; c0 += 32 * TPP) {
@@ -344,7 +351,9 @@ __device__ __forceinline__ void test_range(const BuildArgs& a, int s, int h, con
     while (mask) {
       const int t = __ffs(mask) - 1;
       mask &= mask - 1u;
-      if (count < a.lane_width) out[(int64_t)count * a.npad] = c0 + t * TPP;
+      const int32_t b = c0 + t * TPP;
+      if (count < kStage) stage[count * kBlock] = b;
+      else if (count < a.lane_width) out[(int64_t)count * a.npad] = b;
       ++count;
     }
   }
@@ -355,6 +364,7 @@ __device__ __forceinline__ void test_range(const BuildArgs& a, int s, int h, con
 // the receivers with d^2 < cutoff^2 and stores them in its own sub-list.
 template <int DIM, int TPP, bool ORDERED>
 __global__ __launch_bounds__(kBlock) void k_nl_build(BuildArgs a) {
+  __shared__ int32_t stage_buf[kStage][kBlock];
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   const int s = tid / TPP, h = tid % TPP;
   int count = 0;
@@ -362,9 +372,10 @@ __global__ __launch_bounds__(kBlock) void k_nl_build(BuildArgs a) {
     const float4 ps = a.pos[s];
     const int sid = ORDERED ? a.perm[s] : 0;
     int32_t* out = a.ell + ((int64_t)h * a.lane_width) * a.npad + s;
+    int32_t* stage = &stage_buf[0][threadIdx.x];
     float img[3] = {0.0f, 0.0f, 0.0f};
     if (!a.g.use_cells) {
-      test_range<DIM, TPP, ORDERED>(a, s, h, ps, sid, 0, a.n, img, out, count);
+      test_range<DIM, TPP, ORDERED>(a, s, h, ps, sid, 0, a.n, img, out, stage, count);
     } else {
       int c[3];
       cell_coords<DIM>(ps, a.g, c);
@@ -385,7 +396,7 @@ __global__ __launch_bounds__(kBlock) void k_nl_build(BuildArgs a) {
           if (c[0] >= 1 && c[0] + 1 < nx) {
             img[0] = 0.0f;
             test_range<DIM, TPP, ORDERED>(a, s, h, ps, sid, a.cell_start[row + c[0] - 1],
-                                          a.cell_start[row + c[0] + 2], img, out, count);
+                                          a.cell_start[row + c[0] + 2], img, out, stage, count);
           } else {
             for (int dx = -1; dx <= 1; ++dx) {
               int cx = c[0] + dx;
@@ -393,12 +404,14 @@ __global__ __launch_bounds__(kBlock) void k_nl_build(BuildArgs a) {
               if (cx < 0) { cx += nx; img[0] = -a.box[0]; }
               else if (cx >= nx) { cx -= nx; img[0] = a.box[0]; }
               test_range<DIM, TPP, ORDERED>(a, s, h, ps, sid, a.cell_start[row + cx],
-                                            a.cell_start[row + cx + 1], img, out, count);
+                                            a.cell_start[row + cx + 1], img, out, stage, count);
             }
           }
         }
       }
     }
+    const int staged = count < kStage ? count : kStage;
+    for (int k = 0; k < staged; ++k) out[(int64_t)k * a.npad] = stage[k * kBlock];
     a.cnt[h * a.npad + s] = count < a.lane_width ? count : a.lane_width;
     if (count > a.lane_width) atomicAdd((unsigned long long*)&a.info[5], 1ull);
   }

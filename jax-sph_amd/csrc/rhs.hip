@@ -137,13 +137,20 @@ __device__ __forceinline__ void for_each_pair(const SweepArgs& a, int s, int h, 
   constexpr int B = 8;
   const int c = a.cnt[h * a.npad + s];
   const int32_t* lst = a.ell + ((int64_t)h * a.lane_width) * a.npad + s;
-  for (int k0 = 0; k0 < c; k0 += B) {
-    int jb[B];
+  int jb[B];
 #pragma unroll
-    for (int t = 0; t < B; ++t) jb[t] = (k0 + t < c) ? lst[(int64_t)(k0 + t) * a.npad] : -1;
+  for (int t = 0; t < B; ++t) jb[t] = (t < c) ? lst[(int64_t)t * a.npad] : -1;
+  for (int k0 = 0; k0 < c; k0 += B) {
+    // the next batch is requested before this one is consumed
+    int jn[B];
+#pragma unroll
+    for (int t = 0; t < B; ++t)
+      jn[t] = (k0 + B + t < c) ? lst[(int64_t)(k0 + B + t) * a.npad] : -1;
 #pragma unroll 2
     for (int t = 0; t < B; ++t)
       if (jb[t] >= 0) f(jb[t]);
+#pragma unroll
+    for (int t = 0; t < B; ++t) jb[t] = jn[t];
   }
 }
 
