@@ -42,6 +42,11 @@ def parse():
     ap.add_argument("--n-side", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-side", type=int, default=20)
+    ap.add_argument("--skin", default="auto",
+                    help="Verlet skin of the fused step's pair lists ('auto' = 0.1 cutoff, "
+                         "0 = exact lists rebuilt every step)")
+    ap.add_argument("--xcd-remap", type=int, default=1,
+                    help="1: XCD-contiguous block order in the pair build and sweeps")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (5 times), to "
                          "scale hardware byte counters; prints nothing else")
@@ -123,10 +128,12 @@ def main():
     (cfg, box, state, g_ext_fn, bc_fn, nw_fn, eos, key, disp, shift) = \
         cs.load_case("tgv")(cfg).initialize()
     solver = cs.make_solver(cfg, disp, eos, g_ext_fn)
+    skin = a.skin if a.skin == "auto" else float(a.skin)
     fn = partition.neighbor_list(disp, box, r_cutoff=solver._kernel_fn.cutoff,
                                  capacity_multiplier=1.25, mask_self=False,
-                                 format=partition.Sparse)
+                                 format=partition.Sparse, skin=skin)
     nb = fn.allocate(state["r"])
+    nb.workspace.tune("xcd_remap", a.xcd_remap)
     adv = integrator.si_euler(cfg.solver.tvf, solver.forward_wrapper(), shift, bc_fn, nw_fn)
     assert adv._sph_fusable(nb) is not None, "bench must run the fused step"
     N = state["r"].shape[0]
@@ -136,18 +143,25 @@ def main():
     for _ in range(a.warmup):
         st, nb = adv(dt, st, nb)
     torch.cuda.synchronize()
-    ws.prof_reset()
-    ws.prof_enable(True)
-    torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(a.steps):
         st, nb = adv(dt, st, nb)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
-    ws.prof_enable(False)
-    prof = ws.prof_read()
     overflow = int(nb.error.code)
     value = N * a.steps / el
+    # as many steps again, continuing the run, with a HIP event pair around
+    # every library kernel (on the library's stream): mean device durations
+    ws.prof_reset()
+    ws.prof_enable(True)
+    t1 = time.perf_counter()
+    for _ in range(a.steps):
+        st, nb = adv(dt, st, nb)
+    torch.cuda.synchronize()
+    el_prof = time.perf_counter() - t1
+    ws.prof_enable(False)
+    prof = ws.prof_read()
+    overflow |= int(nb.error.code)
     # dominant kernel and its roofline
     name, (tot_ms, cnt) = max(prof.items(), key=lambda kv: kv[1][0])
     avg_s = tot_ms / cnt / 1e3
@@ -174,12 +188,14 @@ def main():
         "data": "synthetic: 3D TGV lattice initial condition (sin/cos velocity field)",
         "config": {"workload": f"3D Taylor-Green vortex, {n_side}^3 = {N} particles, tvf=1, "
                                f"QSK h=dx, Re=1600, c0=10, dt={dt:.6g}, fused si_euler step",
-                   "particles": N, "pairs_per_step": pairs, "dt": dt},
+                   "particles": N, "pairs_per_step": pairs, "dt": dt,
+                   "verlet_skin": ws.skin, "xcd_remap": a.xcd_remap},
         "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None,
                      "alg_bytes_per_launch": per_launch, "us_mean": avg_s * 1e6},
         "kernels": kernels,
+        "ms_per_step_profiled_pass": el_prof / a.steps * 1e3,
         "partition_error_code": overflow,
         "fwd_adjoint": None,
     }

@@ -121,6 +121,10 @@ int sph_ws_create(const sph_nl_config *cfg, sph_ws **out);
 int sph_ws_destroy(sph_ws *ws);
 /* bytes of device memory held by the workspace */
 int64_t sph_ws_bytes(const sph_ws *ws);
+/* Launch-shape knobs of a workspace (no counterpart in the reference, whose
+ * launches XLA decides).  "xcd_remap" (default 1): the pair build and the
+ * sweeps hand each of the 8 XCDs a contiguous range of cell-sorted particles. */
+int sph_ws_tune(sph_ws *ws, const char *key, int64_t value);
 
 /* ---- neighbour search: jax_sph/partition.py:492-571 + jax_md/partition.py:723-1026 */
 /* Max particles in one cell on a grid (count_cell_filling, partition.py:159-174);
@@ -148,20 +152,59 @@ int sph_nl_scan_budget(sph_ws *ws, int bank, int64_t n_sub, int64_t budget, int6
 int sph_nl_write_idx(sph_ws *ws, int bank, int32_t *idx, int64_t max_occupancy,
                      const int64_t *info, hipStream_t stream);
 
+/* ---- Verlet-skin lists of the fused step (bank 2) ------------------------
+ * The reference rebuilds the neighbour list on every si_euler step
+ * (integrator.py:46, dr_threshold = 0).  With skin lists, sph_step on bank 2
+ * keeps pairs closer than cutoff + skin, rebuilds them on the device only
+ * after some particle has moved more than 0.45 * skin since the last build,
+ * and applies the reference predicate (partition.py:895-897) to every pair on
+ * every step.  Pair sets, occupancy info[1], info[3] and the error bits are
+ * those of a rebuild at every step; info[7] holds the largest skin list of one
+ * particle at the last rebuild.  A skin list that outgrows `width` raises
+ * SPH_NEIGHBOR_LIST_OVERFLOW, so the caller re-allocates as for any overflow.
+ * Returns 1 when enabled, 0 when this geometry cannot use them (no cell list,
+ * OrderedSparse, fewer than 3 skin cells per side, cells too close to the
+ * cutoff), < 0 on error.  Allocates; call once, after sph_ws_create. */
+int sph_skin_enable(sph_ws *ws, float skin, int32_t width);
+/* NeighborList.idx of the skin lists at positions r (orig order, those of the
+ * step that produced the list), same layout as sph_nl_write_idx. */
+int sph_nl_write_idx_skin(sph_ws *ws, const float *r, int32_t *idx, int64_t max_occupancy,
+                          hipStream_t stream);
+
 /* ---- physics ------------------------------------------------------------ */
 /* W(r) and dW/dr of one kernel, elementwise (kernel.py:9-201). */
 int sph_kernel_eval(int32_t kind, int32_t dim, float one_over_h, float sigma, const float *r,
                     int64_t n, float *w, float *dw, hipStream_t stream);
-/* WCSPH.forward (solver.py:705-949) over the pairs of `bank`. */
+/* WCSPH.forward (solver.py:705-949) over the pairs of `bank` (0, 1, or 2 for
+ * the skin lists, filtered at in->r). */
 int sph_forward(sph_ws *ws, int bank, const sph_solver_params *prm, const sph_state_in *in,
                 sph_state_out *out, hipStream_t stream);
 /* One si_euler step (integrator.py:22-56) fused with the neighbour update,
  * forward and the case boundary hook: reads `in`, writes r/u/v to r_out/out
- * and the rest of the state to `out`; the new pair set goes to `bank`. */
+ * and the rest of the state to `out`; the new pair set goes to `bank`
+ * (0 or 1: exact lists rebuilt now; 2: the skin lists, see sph_skin_enable). */
 int sph_step(sph_ws *ws, int bank, const sph_solver_params *prm, float dt, float tvf_coef,
              const sph_bc_spec *bc, const sph_state_in *in, float *r_out, sph_state_out *out,
              const int64_t *prev_info, int64_t max_occupancy, int64_t *info,
              hipStream_t stream);
+
+/* ---- adjoint ------------------------------------------------------------- */
+/* Cotangents of the step's carried state (r, u, a = dudt), (N, dim), orig order. */
+typedef struct {
+  const float *r, *u, *a;
+} sph_cotangent_in;
+typedef struct {
+  float *r, *u, *a;
+} sph_cotangent_out;
+/* Vector-Jacobian product of one si_euler step (integrator.py:22-56), the
+ * transpose that jax.grad builds for the reference's step.  Standard SPH,
+ * density summation, no walls/heat/artificial viscosity, tvf = 0, QSK or WC2K.
+ * `next` is state n+1 (r, u, rho, p, mass, eta, tag) and `bank` must hold the
+ * pairs of next->r; given the cotangents of state n+1 writes those of state n
+ * (bar may alias nothing in bar_next). */
+int sph_step_vjp(sph_ws *ws, int bank, const sph_solver_params *prm, float dt,
+                 const sph_state_in *next, const sph_cotangent_in *bar_next,
+                 sph_cotangent_out *bar, hipStream_t stream);
 
 /* si_euler kick/drift alone (integrator.py:26-30), for callers that compose
  * the step from separate update/forward calls: u' = u + dt dudt,

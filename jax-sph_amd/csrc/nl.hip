@@ -33,6 +33,8 @@ struct Grid {
   int use_cells;
 };
 
+void cut_band(const sph_nl_config& c, float* lo, float* hi);
+
 // int32(r / cell_size) per dimension (partition.py:367), clamped into the grid:
 // the reference aliases r == side into a wrong cell; clamping keeps it in the last one.
 template <int DIM>
@@ -91,7 +93,8 @@ __device__ __forceinline__ int wave_aggregated_inc(int32_t* __restrict__ counter
 template <int DIM>
 __global__ void k_hash_count(const float* __restrict__ r, int n, Grid g,
                              int32_t* __restrict__ cell_count, int32_t* __restrict__ pcell,
-                             int32_t* __restrict__ pslot) {
+                             int32_t* __restrict__ pslot, const int32_t* __restrict__ gate) {
+  if (gate && *gate == 0) return;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   int h = 0;
@@ -144,6 +147,83 @@ __global__ void k_integrate_hash(const float* __restrict__ r, const float* __res
   pslot[i] = slot;
 }
 
+// Skin-list step: kick/drift/wrap as k_integrate_hash, the histogram of the
+// new positions on the reference grid (for its cell-capacity bit), and the
+// rebuild test: a particle that moved more than the trigger distance since
+// the last build (minimum image) raises *flag.
+template <int DIM>
+__global__ void k_integrate_skin(const float* __restrict__ r, const float* __restrict__ u,
+                                 const float* __restrict__ dudt, const float* __restrict__ dvdt,
+                                 float* __restrict__ r_out, float* __restrict__ u_out,
+                                 float* __restrict__ v_out, int n, float dt, float ctvf, Grid g,
+                                 float bx, float by, float bz, int32_t* __restrict__ cell_count,
+                                 const float4* __restrict__ posb, float trigger_sq,
+                                 int32_t* __restrict__ flag) {
+#pragma clang fp contract(off)
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float side[3] = {bx, by, bz};
+  float x[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    const int64_t k = (int64_t)i * DIM + d;
+    const float uu = u[k] + dt * dudt[k];
+    const float vv = uu + ctvf * dvdt[k];
+    float xx = r[k] + dt * vv;
+    if (xx >= side[d] || xx < 0.0f) xx = jmod(xx, side[d]);
+    u_out[k] = uu;
+    v_out[k] = vv;
+    r_out[k] = xx;
+    x[d] = xx;
+  }
+  int c[3];
+  cell_coords<DIM>(make_float4(x[0], x[1], x[2], 0.0f), g, c);
+  wave_aggregated_inc(cell_count, cell_hash<DIM>(c, g));
+  const float4 b = posb[i];
+  const float pb[3] = {b.x, b.y, b.z};
+  float m2 = 0.0f;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    const float dd = disp_fast(x[d], pb[d], side[d], 1.0f / side[d]);
+    m2 += dd * dd;
+  }
+  if (m2 > trigger_sq) *flag = 1;  // same value from every writer
+}
+
+// Largest reference-grid cell occupancy of this step (one workgroup): sets
+// info[2] and the cell-capacity bit, resets the other info entries (keeping
+// the sticky bits of prev[0]) and clears the histogram for the next step.
+constexpr int kCellsThreads = 1024;
+__global__ __launch_bounds__(kCellsThreads) void k_cells_max(int32_t* __restrict__ count,
+                                                             int64_t ncell,
+                                                             int64_t* __restrict__ info,
+                                                             const int64_t* __restrict__ prev,
+                                                             int64_t capacity) {
+  __shared__ int wm[kCellsThreads / 64];
+  int m = 0;
+  for (int64_t i = threadIdx.x; i < ncell; i += kCellsThreads) {
+    const int c = count[i];
+    m = c > m ? c : m;
+    count[i] = 0;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const int y = __shfl_down(m, o, 64);
+    m = y > m ? y : m;
+  }
+  if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int mm = 0;
+    for (int w = 0; w < kCellsThreads / 64; ++w) mm = wm[w] > mm ? wm[w] : mm;
+    int64_t bits = prev ? prev[0] : 0;
+    if (capacity >= 0 && mm > capacity) bits |= SPH_CELL_LIST_OVERFLOW;
+    info[0] = bits;
+    for (int k = 1; k < 8; ++k) info[k] = 0;
+    info[2] = mm;
+  }
+}
+
 __global__ void k_max_reduce(const int32_t* __restrict__ a, int64_t n, int64_t* __restrict__ out) {
   int m = 0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -163,14 +243,20 @@ constexpr int kScanThreads = 1024;
 constexpr int kScanItems = 8;
 
 template <typename TIn, typename TOut>
-__global__ __launch_bounds__(kScanThreads) void k_scan_single(const TIn* __restrict__ in, int64_t n,
+__global__ __launch_bounds__(kScanThreads) void k_scan_single(const TIn* in, int64_t n,
                                                               TOut* __restrict__ out,
                                                               int64_t* __restrict__ max_out,
                                                               int64_t* __restrict__ info,
-                                                              int64_t capacity) {
+                                                              int64_t capacity,
+                                                              TIn* clear = nullptr,
+                                                              const int32_t* __restrict__ gate =
+                                                                  nullptr) {
+  // `clear` (the input array itself, or null) is zeroed as it is read;
+  // `gate` (or null) skips the whole scan when *gate == 0.
   __shared__ TOut wave_tot[kScanThreads / 64];
   __shared__ TOut carry_s;
   __shared__ int64_t wmax[kScanThreads / 64];
+  if (gate && *gate == 0) return;
   const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
   TOut carry = 0;
   int64_t mx = 0;
@@ -183,6 +269,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_single(const TIn* __restr
     for (int k = 0; k < kScanItems; ++k) {
       int64_t i = i0 + k;
       TOut x = i < n ? (TOut)in[i] : (TOut)0;
+      if (clear && i < n) clear[i] = 0;
       v[k] = x;
       s += x;
       mx = (int64_t)x > mx ? (int64_t)x : mx;
@@ -237,7 +324,8 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_single(const TIn* __restr
 
 __global__ void k_scatter(const int32_t* __restrict__ pcell, const int32_t* __restrict__ pslot,
                           const int32_t* __restrict__ cell_start, int32_t* __restrict__ unsorted,
-                          int n) {
+                          int n, const int32_t* __restrict__ gate = nullptr) {
+  if (gate && *gate == 0) return;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   unsorted[cell_start[pcell[i]] + pslot[i]] = i;
@@ -248,7 +336,9 @@ __global__ void k_scatter(const int32_t* __restrict__ pcell, const int32_t* __re
 __global__ __launch_bounds__(256) void k_cell_sort(const int32_t* __restrict__ cell_start,
                                                    const int32_t* __restrict__ unsorted,
                                                    int32_t* __restrict__ perm,
-                                                   int32_t* __restrict__ iperm, int64_t ncell) {
+                                                   int32_t* __restrict__ iperm, int64_t ncell,
+                                                   const int32_t* __restrict__ gate = nullptr) {
+  if (gate && *gate == 0) return;
   const int lane = threadIdx.x & 63;
   const int64_t cell = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   if (cell >= ncell) return;
@@ -300,6 +390,11 @@ struct BuildArgs {
   float lo, hi;
   int fast;
   int n, npad, lane_width, mask_self;
+  int64_t* tot_out;       // += pairs found (nullable)
+  int64_t* max_out;       // max pairs of one particle (nullable)
+  int64_t trunc_bits;     // error bits raised when a sub-list overflows
+  const int32_t* gate;    // nullable: skip the build when *gate == 0
+  int remap;              // XCD-contiguous block order (grid a multiple of 8)
 };
 
 // Lane h (of TPP) tests candidates lo + h, lo + h + TPP, ... < hi of one cell
@@ -365,7 +460,8 @@ __device__ __forceinline__ void test_range(const BuildArgs& a, int s, int h, con
 template <int DIM, int TPP, bool ORDERED>
 __global__ __launch_bounds__(kBlock) void k_nl_build(BuildArgs a) {
   __shared__ int32_t stage_buf[kStage][kBlock];
-  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a.gate && *a.gate == 0) return;  // uniform over the grid
+  const int tid = logical_block(a.remap) * blockDim.x + threadIdx.x;
   const int s = tid / TPP, h = tid % TPP;
   int count = 0;
   if (s < a.n) {
@@ -410,10 +506,14 @@ __global__ __launch_bounds__(kBlock) void k_nl_build(BuildArgs a) {
         }
       }
     }
-    const int staged = count < kStage ? count : kStage;
+    int staged = count < kStage ? count : kStage;
+    staged = staged < a.lane_width ? staged : a.lane_width;  // never past the sub-list
     for (int k = 0; k < staged; ++k) out[(int64_t)k * a.npad] = stage[k * kBlock];
     a.cnt[h * a.npad + s] = count < a.lane_width ? count : a.lane_width;
-    if (count > a.lane_width) atomicAdd((unsigned long long*)&a.info[5], 1ull);
+    if (count > a.lane_width) {
+      atomicAdd((unsigned long long*)&a.info[5], 1ull);
+      if (a.trunc_bits) atomicOr((unsigned long long*)&a.info[0], (unsigned long long)a.trunc_bits);
+    }
   }
   // pairs of the particle (sum over its TPP lanes), then block reductions:
   // total pairs and the largest per-particle count
@@ -437,13 +537,22 @@ __global__ __launch_bounds__(kBlock) void k_nl_build(BuildArgs a) {
     unsigned long long t = 0;
     int mm = 0;
     for (int w = 0; w < kBlock / 64; ++w) { t += tot[w]; mm = mx[w] > mm ? mx[w] : mm; }
-    if (t) atomicAdd((unsigned long long*)&a.info[1], t);
-    atomicMax((unsigned long long*)&a.info[3], (unsigned long long)mm);
+    if (t && a.tot_out) atomicAdd((unsigned long long*)a.tot_out, t);
+    if (a.max_out) atomicMax((unsigned long long*)a.max_out, (unsigned long long)mm);
   }
 }
 
 __global__ void k_finalize(int64_t* __restrict__ info, int64_t max_occ) {
   if (threadIdx.x == 0 && info[1] > max_occ) info[0] |= SPH_NEIGHBOR_LIST_OVERFLOW;
+}
+
+// End of a skin-list step: the occupancy bit, and the rebuild request is consumed.
+__global__ void k_finalize_skin(int64_t* __restrict__ info, int64_t max_occ,
+                                int32_t* __restrict__ flag) {
+  if (threadIdx.x == 0) {
+    if (max_occ >= 0 && info[1] > max_occ) info[0] |= SPH_NEIGHBOR_LIST_OVERFLOW;
+    *flag = 0;
+  }
 }
 
 template <int DIM>
@@ -494,6 +603,45 @@ __global__ void k_pad_idx(int32_t* __restrict__ idx, int64_t max_occ,
     idx[p] = n;
     idx[max_occ + p] = n;
   }
+}
+
+// idx of a skin list: the pairs of each sender (sub-list order) that pass the
+// exact reference predicate at positions r.  WRITE = false counts them per
+// original id; WRITE = true stores them from the scanned offsets.
+struct SkinIdxArgs {
+  const float* r;
+  const int32_t *perm, *cnt, *ell;
+  int64_t* cnt64;
+  const int64_t* off;
+  int32_t* idx;
+  int64_t max_occ;
+  float box[3], half[3];
+  float cutoff_sq;
+  int n, npad, tpp, lane_width;
+};
+
+template <int DIM, bool WRITE>
+__global__ void k_skin_pairs(SkinIdxArgs a) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.n) return;
+  const int i = a.perm[s];
+  const float4 ri = load_pos<DIM>(a.r, i);
+  int64_t p = WRITE ? a.off[i] : 0;
+  for (int h = 0; h < a.tpp; ++h) {
+    const int c = a.cnt[h * a.npad + s];
+    const int32_t* lst = a.ell + ((int64_t)h * a.lane_width) * a.npad + s;
+    for (int k = 0; k < c; ++k) {
+      const int j = a.perm[lst[(int64_t)k * a.npad]];
+      if (!(metric_sq_exact<DIM>(ri, load_pos<DIM>(a.r, j), a.box, a.half) < a.cutoff_sq))
+        continue;
+      if (WRITE && p < a.max_occ) {
+        a.idx[p] = j;
+        a.idx[a.max_occ + p] = i;
+      }
+      ++p;
+    }
+  }
+  if (!WRITE) a.cnt64[i] = p;
 }
 
 __global__ void k_chunk_budget(const int64_t* __restrict__ off, int n, int64_t n_sub,
@@ -566,7 +714,7 @@ int bin_finish(sph_ws* ws, int bank, const float* r, int64_t* info, hipStream_t 
   SPH_LAUNCH(ws, "scan_cells", st,
              (k_scan_single<int32_t, int32_t><<<1, kScanThreads, 0, st>>>(
                  ws->cell_count, ws->ncell, B.cell_start, &info[2], info,
-                 c.use_cells ? (int64_t)c.cell_capacity : (int64_t)-1)));
+                 c.use_cells ? (int64_t)c.cell_capacity : (int64_t)-1, ws->cell_count)));
   SPH_LAUNCH(ws, "scatter", st,
              k_scatter<<<cdiv(n, kBlock), kBlock, 0, st>>>(ws->pcell, ws->pslot, B.cell_start,
                                                          ws->unsorted, n));
@@ -593,11 +741,11 @@ int bin_particles(sph_ws* ws, int bank, const float* r, const int64_t* prev_info
   if (c.dim == 2)
     SPH_LAUNCH(ws, "hash_count", st,
                k_hash_count<2><<<cdiv(n, kBlock), kBlock, 0, st>>>(r, n, g, ws->cell_count,
-                                                                 ws->pcell, ws->pslot));
+                                                                 ws->pcell, ws->pslot, nullptr));
   else
     SPH_LAUNCH(ws, "hash_count", st,
                k_hash_count<3><<<cdiv(n, kBlock), kBlock, 0, st>>>(r, n, g, ws->cell_count,
-                                                                 ws->pcell, ws->pslot));
+                                                                 ws->pcell, ws->pslot, nullptr));
   return bin_finish(ws, bank, r, info, st);
 }
 
@@ -621,30 +769,28 @@ int build_pairs(sph_ws* ws, int bank, int64_t max_occupancy, int64_t* info, hipS
   a.npad = ws->npad;
   a.lane_width = ws->lane_width;
   a.mask_self = c.mask_self;
+  a.tot_out = &info[1];
+  a.max_out = &info[3];
+  a.trunc_bits = 0;
+  a.gate = nullptr;
+  a.remap = ws->xcd_remap;
   // The cell-adjacent image is the minimum image of every pair within the
   // cutoff when each side has >= 5 cells (2 cells < box/2).  Both the fast and
   // the exact squared distance are then within a few ulp(box) per component of
-  // the true one; `band` bounds their difference with a wide margin.
-  float maxbox = 0.0f;
+  // the true one (see cut_band).
   a.fast = c.use_cells ? 1 : 0;
-  for (int d = 0; d < c.dim; ++d) {
-    maxbox = c.box[d] > maxbox ? c.box[d] : maxbox;
+  for (int d = 0; d < c.dim; ++d)
     if (c.cps[d] < 5) a.fast = 0;
-  }
-  const float ulp_box = std::nextafter(maxbox, INFINITY) - maxbox;
-  const float ulp_c2 = std::nextafter(c.cutoff_sq, INFINITY) - c.cutoff_sq;
-  const float band = 64.0f * c.cutoff * ulp_box + 16.0f * ulp_c2;
-  a.lo = c.cutoff_sq - band;
-  a.hi = c.cutoff_sq + band;
+  cut_band(c, &a.lo, &a.hi);
   constexpr int T2 = tpp_of(2), T3 = tpp_of(3);
   if (c.dim == 2) {
-    dim3 grid(cdiv((int64_t)c.n * T2, kBlock));
+    dim3 grid(xcd_grid(cdiv((int64_t)c.n * T2, kBlock), a.remap));
     if (c.ordered)
       SPH_LAUNCH(ws, "nl_build", st, (k_nl_build<2, T2, true><<<grid, kBlock, 0, st>>>(a)));
     else
       SPH_LAUNCH(ws, "nl_build", st, (k_nl_build<2, T2, false><<<grid, kBlock, 0, st>>>(a)));
   } else {
-    dim3 grid(cdiv((int64_t)c.n * T3, kBlock));
+    dim3 grid(xcd_grid(cdiv((int64_t)c.n * T3, kBlock), a.remap));
     if (c.ordered)
       SPH_LAUNCH(ws, "nl_build", st, (k_nl_build<3, T3, true><<<grid, kBlock, 0, st>>>(a)));
     else
@@ -652,6 +798,126 @@ int build_pairs(sph_ws* ws, int bank, int64_t max_occupancy, int64_t* info, hipS
   }
   if (max_occupancy >= 0)
     SPH_LAUNCH(ws, "finalize", st, k_finalize<<<1, 64, 0, st>>>(info, max_occupancy));
+  return SPH_OK;
+}
+
+// The exact-predicate band of the pair build: squared distances (fp32, minimum
+// image, explicit FMA) below lo are inside the cutoff, at or above hi outside;
+// in between the reference metric decides.  The two differ by a few ulp(box)
+// per component, which `band` bounds with a wide margin.
+void cut_band(const sph_nl_config& c, float* lo, float* hi) {
+  float maxbox = 0.0f;
+  for (int d = 0; d < c.dim; ++d) maxbox = c.box[d] > maxbox ? c.box[d] : maxbox;
+  const float ulp_box = std::nextafter(maxbox, INFINITY) - maxbox;
+  const float ulp_c2 = std::nextafter(c.cutoff_sq, INFINITY) - c.cutoff_sq;
+  const float band = 64.0f * c.cutoff * ulp_box + 16.0f * ulp_c2;
+  *lo = c.cutoff_sq - band;
+  *hi = c.cutoff_sq + band;
+}
+
+static Grid skin_grid(const sph_ws* ws) {
+  Grid g;
+  for (int d = 0; d < 3; ++d) {
+    g.cs[d] = ws->sk.cell_size[d];
+    g.cps[d] = ws->sk.cps[d];
+  }
+  g.use_cells = 1;
+  return g;
+}
+
+// First half of a skin-list step: kick/drift with the rebuild test and the
+// reference-grid occupancy; then, only if a rebuild was requested, the
+// skin-grid binning (hash, scan, scatter, per-cell sort).
+int skin_begin(sph_ws* ws, const float* r, const float* u, const float* dudt, const float* dvdt,
+               float* r_out, float* u_out, float* v_out, float dt, float ctvf, const float* box,
+               const int64_t* prev_info, int64_t* info, hipStream_t st) {
+  const auto& c = ws->cfg;
+  auto& S = ws->sk;
+  const int n = c.n;
+  const Grid g = make_grid(c);
+  const Grid gs = skin_grid(ws);
+  const unsigned nb = (unsigned)cdiv(n, kBlock);
+  if (c.dim == 2)
+    SPH_LAUNCH(ws, "integrate_skin", st,
+               k_integrate_skin<2><<<nb, kBlock, 0, st>>>(
+                   r, u, dudt, dvdt, r_out, u_out, v_out, n, dt, ctvf, g, box[0], box[1], 0.0f,
+                   ws->cell_count, S.posb, S.trigger_sq, S.flag));
+  else
+    SPH_LAUNCH(ws, "integrate_skin", st,
+               k_integrate_skin<3><<<nb, kBlock, 0, st>>>(
+                   r, u, dudt, dvdt, r_out, u_out, v_out, n, dt, ctvf, g, box[0], box[1], box[2],
+                   ws->cell_count, S.posb, S.trigger_sq, S.flag));
+  SPH_LAUNCH(ws, "cells_max", st,
+             k_cells_max<<<1, kCellsThreads, 0, st>>>(ws->cell_count, ws->ncell, info, prev_info,
+                                                      (int64_t)c.cell_capacity));
+  if (c.dim == 2)
+    SPH_LAUNCH(ws, "skin_hash", st,
+               k_hash_count<2><<<nb, kBlock, 0, st>>>(r_out, n, gs, S.cell_count, ws->pcell,
+                                                      ws->pslot, S.flag));
+  else
+    SPH_LAUNCH(ws, "skin_hash", st,
+               k_hash_count<3><<<nb, kBlock, 0, st>>>(r_out, n, gs, S.cell_count, ws->pcell,
+                                                      ws->pslot, S.flag));
+  SPH_LAUNCH(ws, "skin_scan", st,
+             (k_scan_single<int32_t, int32_t><<<1, kScanThreads, 0, st>>>(
+                 S.cell_count, S.ncell, S.b.cell_start, nullptr, nullptr, (int64_t)-1,
+                 S.cell_count, S.flag)));
+  SPH_LAUNCH(ws, "skin_scatter", st,
+             k_scatter<<<nb, kBlock, 0, st>>>(ws->pcell, ws->pslot, S.b.cell_start, ws->unsorted,
+                                              n, S.flag));
+  SPH_LAUNCH(ws, "skin_sort", st,
+             k_cell_sort<<<cdiv(S.ncell * 64, 256), 256, 0, st>>>(S.b.cell_start, ws->unsorted,
+                                                                  S.b.perm, S.b.iperm, S.ncell,
+                                                                  S.flag));
+  return SPH_OK;
+}
+
+// Second half: the skin pair lists (cutoff + skin) from the sorted build
+// positions, only if a rebuild was requested.  Pair totals are not counted
+// here (the sweeps count the pairs inside the cutoff); info[7] receives the
+// largest list of one particle, and an overflowing sub-list raises
+// SPH_NEIGHBOR_LIST_OVERFLOW so the caller re-allocates.
+int skin_build(sph_ws* ws, int64_t* info, hipStream_t st) {
+  const auto& c = ws->cfg;
+  auto& S = ws->sk;
+  BuildArgs a;
+  a.pos = S.b.pos;
+  a.cell_start = S.b.cell_start;
+  a.perm = S.b.perm;
+  a.ell = S.b.ell;
+  a.cnt = S.b.cnt;
+  a.info = info;
+  a.g = skin_grid(ws);
+  for (int d = 0; d < 3; ++d) {
+    a.box[d] = c.box[d];
+    a.half[d] = c.box[d] * 0.5f;
+  }
+  a.cutoff_sq = S.reach_sq;
+  a.fast = 1;
+  a.lo = a.hi = S.reach_sq;
+  a.n = c.n;
+  a.npad = ws->npad;
+  a.lane_width = S.lane_width;
+  a.mask_self = c.mask_self;
+  a.tot_out = nullptr;
+  a.max_out = &info[7];
+  a.trunc_bits = SPH_NEIGHBOR_LIST_OVERFLOW;
+  a.gate = S.flag;
+  a.remap = ws->xcd_remap;
+  constexpr int T2 = tpp_of(2), T3 = tpp_of(3);
+  if (c.dim == 2) {
+    dim3 grid(xcd_grid(cdiv((int64_t)c.n * T2, kBlock), a.remap));
+    SPH_LAUNCH(ws, "skin_build", st, (k_nl_build<2, T2, false><<<grid, kBlock, 0, st>>>(a)));
+  } else {
+    dim3 grid(xcd_grid(cdiv((int64_t)c.n * T3, kBlock), a.remap));
+    SPH_LAUNCH(ws, "skin_build", st, (k_nl_build<3, T3, false><<<grid, kBlock, 0, st>>>(a)));
+  }
+  return SPH_OK;
+}
+
+int skin_finish(sph_ws* ws, int64_t max_occupancy, int64_t* info, hipStream_t st) {
+  SPH_LAUNCH(ws, "finalize", st,
+             k_finalize_skin<<<1, 64, 0, st>>>(info, max_occupancy, ws->sk.flag));
   return SPH_OK;
 }
 
@@ -701,11 +967,11 @@ extern "C" int sph_nl_fill_max(int32_t dim, int32_t n, const float* r, const flo
     if (dim == 2)
       SPH_LAUNCH(ws, "hash_count", st,
                  k_hash_count<2><<<cdiv(n, kBlock), kBlock, 0, st>>>(r, n, g, counts, nullptr,
-                                                                   nullptr));
+                                                                   nullptr, nullptr));
     else
       SPH_LAUNCH(ws, "hash_count", st,
                  k_hash_count<3><<<cdiv(n, kBlock), kBlock, 0, st>>>(r, n, g, counts, nullptr,
-                                                                   nullptr));
+                                                                   nullptr, nullptr));
   }
   int64_t blocks = cdiv(nc, kBlock);
   blocks = blocks > 1024 ? 1024 : blocks;
@@ -778,5 +1044,54 @@ extern "C" int sph_nl_write_idx(sph_ws* ws, int bank, int32_t* idx, int64_t max_
              k_pad_idx<<<cdiv(max_occupancy, kBlock), kBlock, 0, st>>>(idx, max_occupancy,
                                                                       ws->off64 + n, n));
   (void)info;
+  return SPH_OK;
+}
+
+extern "C" int sph_nl_write_idx_skin(sph_ws* ws, const float* r, int32_t* idx,
+                                     int64_t max_occupancy, hipStream_t st) {
+  if (!ws || !r || !idx || max_occupancy < 0) {
+    set_error("sph_nl_write_idx_skin: bad argument");
+    return SPH_ERR_ARG;
+  }
+  if (!ws->sk.on || !ws->sk.stepped) {
+    set_error("sph_nl_write_idx_skin: no skin lists built by sph_step on this workspace");
+    return SPH_ERR_ARG;
+  }
+  if (max_occupancy == 0) return SPH_OK;
+  const auto& c = ws->cfg;
+  const int n = c.n;
+  SkinIdxArgs a;
+  a.r = r;
+  a.perm = ws->sk.b.perm;
+  a.cnt = ws->sk.b.cnt;
+  a.ell = ws->sk.b.ell;
+  a.cnt64 = ws->cnt64;
+  a.off = ws->off64;
+  a.idx = idx;
+  a.max_occ = max_occupancy;
+  for (int d = 0; d < 3; ++d) {
+    a.box[d] = c.box[d];
+    a.half[d] = c.box[d] * 0.5f;
+  }
+  a.cutoff_sq = c.cutoff_sq;
+  a.n = n;
+  a.npad = ws->npad;
+  a.tpp = ws->tpp;
+  a.lane_width = ws->sk.lane_width;
+  const unsigned nb = (unsigned)cdiv(n, kBlock);
+  if (c.dim == 2)
+    SPH_LAUNCH(ws, "skin_idx_count", st, (k_skin_pairs<2, false><<<nb, kBlock, 0, st>>>(a)));
+  else
+    SPH_LAUNCH(ws, "skin_idx_count", st, (k_skin_pairs<3, false><<<nb, kBlock, 0, st>>>(a)));
+  SPH_LAUNCH(ws, "scan_pairs", st,
+             (k_scan_single<int64_t, int64_t><<<1, kScanThreads, 0, st>>>(
+                 ws->cnt64, n, ws->off64, nullptr, nullptr, -1)));
+  if (c.dim == 2)
+    SPH_LAUNCH(ws, "skin_idx_write", st, (k_skin_pairs<2, true><<<nb, kBlock, 0, st>>>(a)));
+  else
+    SPH_LAUNCH(ws, "skin_idx_write", st, (k_skin_pairs<3, true><<<nb, kBlock, 0, st>>>(a)));
+  SPH_LAUNCH(ws, "pad_idx", st,
+             k_pad_idx<<<cdiv(max_occupancy, kBlock), kBlock, 0, st>>>(idx, max_occupancy,
+                                                                      ws->off64 + n, n));
   return SPH_OK;
 }

@@ -8,13 +8,14 @@
 // and the per-step case hooks tgv.py:56-57, ldc.py:130-144, db.py:299-307.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+#include <string>
+
 #include "sph_device.h"
 #include "sph_internal.h"
+#include "sweep.h"
 
 namespace sph {
-
-constexpr int kB = 256;
-static inline int64_t cdivl(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 // Bank helpers from nl.hip
 int bin_init(sph_ws* ws, const int64_t* prev_info, int64_t* info, hipStream_t st);
@@ -23,289 +24,11 @@ int integrate_and_hash(sph_ws* ws, const float* r, const float* u, const float* 
                        float ctvf, const float* box, hipStream_t st);
 int bin_finish(sph_ws* ws, int bank, const float* r, int64_t* info, hipStream_t st);
 int build_pairs(sph_ws* ws, int bank, int64_t max_occupancy, int64_t* info, hipStream_t st);
-
-// ---------------------------------------------------------- kernel dispatch
-// KIND >= 0: compile-time kernel; KIND == -1: runtime switch over the rest.
-template <int KIND, int DIM>
-struct KernelEval {
-  int kind;
-  float s, ih;
-  __device__ __forceinline__ float w(float q) const { return Kern<KIND, DIM>::w(q, s); }
-  __device__ __forceinline__ float dw(float q) const { return Kern<KIND, DIM>::dw(q, s, ih); }
-};
-
-template <int DIM>
-struct KernelEval<-1, DIM> {
-  int kind;
-  float s, ih;
-  __device__ __forceinline__ float w(float q) const {
-    switch (kind) {
-      case SPH_KERNEL_CSK: return Kern<SPH_KERNEL_CSK, DIM>::w(q, s);
-      case SPH_KERNEL_QSK: return Kern<SPH_KERNEL_QSK, DIM>::w(q, s);
-      case SPH_KERNEL_WC2K: return Kern<SPH_KERNEL_WC2K, DIM>::w(q, s);
-      case SPH_KERNEL_WC4K: return Kern<SPH_KERNEL_WC4K, DIM>::w(q, s);
-      case SPH_KERNEL_WC6K: return Kern<SPH_KERNEL_WC6K, DIM>::w(q, s);
-      case SPH_KERNEL_GK: return Kern<SPH_KERNEL_GK, DIM>::w(q, s);
-      default: return Kern<SPH_KERNEL_SGK, DIM>::w(q, s);
-    }
-  }
-  __device__ __forceinline__ float dw(float q) const {
-    switch (kind) {
-      case SPH_KERNEL_CSK: return Kern<SPH_KERNEL_CSK, DIM>::dw(q, s, ih);
-      case SPH_KERNEL_QSK: return Kern<SPH_KERNEL_QSK, DIM>::dw(q, s, ih);
-      case SPH_KERNEL_WC2K: return Kern<SPH_KERNEL_WC2K, DIM>::dw(q, s, ih);
-      case SPH_KERNEL_WC4K: return Kern<SPH_KERNEL_WC4K, DIM>::dw(q, s, ih);
-      case SPH_KERNEL_WC6K: return Kern<SPH_KERNEL_WC6K, DIM>::dw(q, s, ih);
-      case SPH_KERNEL_GK: return Kern<SPH_KERNEL_GK, DIM>::dw(q, s, ih);
-      default: return Kern<SPH_KERNEL_SGK, DIM>::dw(q, s, ih);
-    }
-  }
-};
-
-// --------------------------------------------------------------- vectors
-struct V3 { float x, y, z; };
-__device__ __forceinline__ V3 v3(float4 a) { return V3{a.x, a.y, a.z}; }
-__device__ __forceinline__ V3 operator-(V3 a, V3 b) { return V3{a.x - b.x, a.y - b.y, a.z - b.z}; }
-__device__ __forceinline__ V3 operator+(V3 a, V3 b) { return V3{a.x + b.x, a.y + b.y, a.z + b.z}; }
-__device__ __forceinline__ V3 operator*(float s, V3 a) { return V3{s * a.x, s * a.y, s * a.z}; }
-template <int DIM>
-__device__ __forceinline__ float dot(V3 a, V3 b) {
-  float d = a.x * b.x + a.y * b.y;
-  if (DIM == 3) d += a.z * b.z;
-  return d;
-}
-
-struct Geo {
-  float box[3], inv[3];
-};
-
-template <int DIM>
-__device__ __forceinline__ V3 min_image(float4 a, float4 b, const Geo& g) {
-  V3 d;
-  d.x = disp_fast(a.x, b.x, g.box[0], g.inv[0]);
-  d.y = disp_fast(a.y, b.y, g.box[1], g.inv[1]);
-  d.z = DIM == 3 ? disp_fast(a.z, b.z, g.box[2], g.inv[2]) : 0.0f;
-  return d;
-}
-
-template <int DIM>
-__device__ __forceinline__ V3 load_vec(const float* __restrict__ a, int64_t i) {
-  V3 v;
-  v.x = a[i * DIM];
-  v.y = a[i * DIM + 1];
-  v.z = DIM == 3 ? a[i * DIM + 2] : 0.0f;
-  return v;
-}
-
-template <int DIM>
-__device__ __forceinline__ void store_vec(float* __restrict__ a, int64_t i, V3 v) {
-  a[i * DIM] = v.x;
-  a[i * DIM + 1] = v.y;
-  if (DIM == 3) a[i * DIM + 2] = v.z;
-}
-
-__device__ __forceinline__ float4 f4(V3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
-
-// ------------------------------------------------------------- arguments
-struct Recs {
-  float4 *P, *U0, *V0, *Q0, *NW, *U1, *Q1, *U2, *V2, *Q2, *UT;
-  float* drho;
-};
-
-struct SweepArgs {
-  sph_solver_params prm;
-  sph_state_in in;
-  sph_state_out out;
-  Recs R;
-  const int32_t* perm;
-  const int32_t* ell;
-  const int32_t* cnt;
-  Geo geo;
-  int n, npad, lane_width;
-  // which record set the force sweep reads
-  const float4 *Uf, *Vf, *Qf;
-  int heat_in_density;  // apply T += dt*dTdt in the density epilogue
-  sph_bc_spec bc;
-};
-
-// Visits the pairs of sorted particle s.  Pair indices are fetched in batches
-// of 8 independent (lane-coalesced) loads so their latency is paid once per
-// batch; the body is unrolled by two so gathers of one pair overlap the
-// arithmetic of the other.
-template <typename F>
-__device__ __forceinline__ void for_each_pair(const SweepArgs& a, int s, int h, F&& f) {
-  constexpr int B = 8;
-  const int c = a.cnt[h * a.npad + s];
-  const int32_t* lst = a.ell + ((int64_t)h * a.lane_width) * a.npad + s;
-  int jb[B];
-#pragma unroll
-  for (int t = 0; t < B; ++t) jb[t] = (t < c) ? lst[(int64_t)t * a.npad] : -1;
-  for (int k0 = 0; k0 < c; k0 += B) {
-    // the next batch is requested before this one is consumed
-    int jn[B];
-#pragma unroll
-    for (int t = 0; t < B; ++t)
-      jn[t] = (k0 + B + t < c) ? lst[(int64_t)(k0 + B + t) * a.npad] : -1;
-#pragma unroll 2
-    for (int t = 0; t < B; ++t)
-      if (jb[t] >= 0) f(jb[t]);
-#pragma unroll
-    for (int t = 0; t < B; ++t) jb[t] = jn[t];
-  }
-}
-
-// Sum of x over the TPP consecutive lanes serving one particle.
-template <int TPP>
-__device__ __forceinline__ float lane_sum(float x) {
-#pragma unroll
-  for (int o = 1; o < TPP; o <<= 1) x += __shfl_xor(x, o, 64);
-  return x;
-}
-
-template <int TPP>
-__device__ __forceinline__ V3 lane_sum(V3 v) {
-  return V3{lane_sum<TPP>(v.x), lane_sum<TPP>(v.y), lane_sum<TPP>(v.z)};
-}
-
-// ------------------------------------------------------------- gather
-// Original-order state -> cell-sorted records (P, U0, V0, Q0, NW).
-template <int DIM>
-__global__ void k_gather_state(SweepArgs a, const float* __restrict__ r, const float* __restrict__ u,
-                               const float* __restrict__ v, float4* __restrict__ nl_pos) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= a.n) return;
-  const int i = a.perm[s];
-  const sph_state_in& in = a.in;
-  V3 rr = load_vec<DIM>(r, i);
-  a.R.P[s] = f4(rr, in.mass[i]);
-  if (nl_pos) nl_pos[s] = f4(rr, 0.0f);
-  a.R.U0[s] = f4(load_vec<DIM>(u, i), in.rho[i]);
-  a.R.V0[s] = f4(load_vec<DIM>(v, i), in.kappa ? in.kappa[i] : 0.0f);
-  a.R.Q0[s] = make_float4(in.p[i], in.eta[i], (float)in.tag[i], in.T ? in.T[i] : 0.0f);
-  if (in.nw || in.Cp) {
-    V3 n = in.nw ? load_vec<DIM>(in.nw, i) : V3{0.f, 0.f, 0.f};
-    a.R.NW[s] = f4(n, in.Cp ? in.Cp[i] : 0.0f);
-  }
-}
-
-template <int DIM>
-__device__ __forceinline__ V3 g_ext_of(const SweepArgs& a, int i) {
-  if (a.in.g_ext) return load_vec<DIM>(a.in.g_ext, i);
-  return V3{a.prm.g_const[0], a.prm.g_const[1], a.prm.g_const[2]};
-}
-
-// ------------------------------------------------------------- density
-// MODE 0: summation (solver.py:176-178, 794-796)
-// MODE 1: SPH continuity (solver.py:24-30, 750-756)
-// MODE 2: Riemann continuity (solver.py:108-164, 772-793)
-// plus the Riemann wall velocity u_tilde (solver.py:547-552) for wall targets,
-// then the EOS (solver.py:801).
-template <int DIM, int KIND, int MODE>
-__global__ __launch_bounds__(kB) void k_density(SweepArgs a) {
-  constexpr int TPP = tpp_of(DIM);
-  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int s = tid / TPP, h = tid % TPP;
-  if (s >= a.n) return;  // whole lane groups leave together
-  const sph_solver_params& p = a.prm;
-  const KernelEval<KIND, DIM> K{p.kernel, p.sigma, p.one_over_h};
-  const Eos eos = make_eos(p);
-  const float4 Pi = a.R.P[s], Ui = a.R.U0[s], Qi = a.R.Q0[s];
-  const int tag_i = (int)Qi.z;
-  const bool fluid_i = tag_i == kFluid;
-  const bool want_ut = p.solver == 1 && p.bc_trick && !p.free_slip && is_wall(tag_i);
-  const V3 ui = v3(Ui);
-  float wsum = 0.0f, acc = 0.0f, utw = 0.0f;
-  V3 ut = {0.f, 0.f, 0.f};
-  V3 gi = {0.f, 0.f, 0.f};
-  if (MODE == 2) gi = g_ext_of<DIM>(a, a.perm[s]);
-  for_each_pair(a, s, h, [&](const int j) {
-    const float4 Pj = a.R.P[j];
-    const V3 dr = min_image<DIM>(Pi, Pj, a.geo);
-    const float d2 = dist2<DIM>(dr.x, dr.y, dr.z);
-    const float d = sqrtf(d2);
-    const float q = d * p.one_over_h;
-    if (MODE == 0) {
-      wsum += K.w(q);
-    } else {
-      const float4 Uj = a.R.U0[j];
-      const float gw = K.dw(q);
-      const V3 e = (1.0f / (d + kEps)) * dr;
-      const V3 kg = gw * e;
-      if (MODE == 1) {
-        const float vj = Pj.w / Uj.w;
-        acc += vj * dot<DIM>(ui - v3(Uj), kg);
-      } else {
-        const float4 Qj = a.R.Q0[j];
-        const bool wall_j = is_wall((int)Qj.z);
-        const V3 uj = v3(Uj);
-        float uL, uR, pR, rhoR;
-        const float pL = Qi.x, rhoL = Ui.w;
-        const V3 me = (-1.0f) * e;
-        if (wall_j) {
-          const float4 Nj = a.R.NW[j];
-          const V3 nj = v3(Nj);
-          uL = -dot<DIM>(ui, nj);
-          uR = -uL + 2.0f * dot<DIM>(uj, nj);
-          pR = pL + rhoL * dot<DIM>(gi, (-1.0f) * dr);
-          rhoR = eos.rho_fn(pR);
-        } else {
-          uL = dot<DIM>(ui, me);
-          uR = dot<DIM>(uj, me);
-          pR = Qj.x;
-          rhoR = Uj.w;
-        }
-        const float Uavg = (uL + uR) * 0.5f;
-        const V3 vavg = 0.5f * (ui + uj);
-        const float rhoavg = (rhoL + rhoR) * 0.5f;
-        const float Ustar = Uavg + 0.5f * (pL - pR) / (rhoavg * p.c_ref);
-        const V3 vstar = Ustar * me + (vavg - Uavg * me);
-        acc += 2.0f * Ui.w * Pj.w / Uj.w * dot<DIM>(ui - vstar, kg);
-      }
-    }
-    if (want_ut) {
-      const float4 Qj = a.R.Q0[j];
-      if ((int)Qj.z == kFluid) {
-        const float w = K.w(q);
-        utw += w;
-        ut = ut + w * v3(a.R.U0[j]);
-      }
-    }
-  });
-  if (MODE == 0) {
-    wsum = lane_sum<TPP>(wsum);
-  } else {
-    acc = lane_sum<TPP>(acc);
-  }
-  if (p.solver == 1 && p.bc_trick) {
-    utw = lane_sum<TPP>(utw);
-    ut = lane_sum<TPP>(ut);
-  }
-  if (h != 0) return;
-  float rho = Ui.w, drhodt;
-  const int i = a.perm[s];
-  if (MODE == 0) {
-    if (fluid_i) rho = Pi.w * wsum;
-    drhodt = a.in.drhodt ? a.in.drhodt[i] : 0.0f;
-  } else if (MODE == 1) {
-    drhodt = Ui.w * acc;
-    rho = Ui.w + p.dt * drhodt;
-  } else {
-    drhodt = acc * (fluid_i ? 1.0f : 0.0f);
-    rho = Ui.w + p.dt * drhodt;
-  }
-  const float pr = eos.p_fn(rho);
-  float T = Qi.w;
-  if (a.heat_in_density) T = T + p.dt * a.in.dTdt[i];
-  a.R.U1[s] = make_float4(Ui.x, Ui.y, Ui.z, rho);
-  a.R.Q1[s] = make_float4(pr, Qi.y, Qi.z, T);
-  a.R.drho[s] = drhodt;
-  if (p.solver == 1 && p.bc_trick) {
-    // u_tilde of every particle: Shepard average for no-slip walls, u otherwise
-    // (only wall neighbours read it, solver.py:378)
-    V3 o = want_ut ? (1.0f / (utw + kEps)) * ut : ui;
-    a.R.UT[s] = f4(o, 0.0f);
-  }
-}
+int skin_begin(sph_ws* ws, const float* r, const float* u, const float* dudt, const float* dvdt,
+               float* r_out, float* u_out, float* v_out, float dt, float ctvf, const float* box,
+               const int64_t* prev_info, int64_t* info, hipStream_t st);
+int skin_build(sph_ws* ws, int64_t* info, hipStream_t st);
+int skin_finish(sph_ws* ws, int64_t max_occupancy, int64_t* info, hipStream_t st);
 
 // ------------------------------------------------------------- walls
 // MODE 0: generalized wall BC of the SPH solver (solver.py:431-530)
@@ -314,7 +37,7 @@ __global__ __launch_bounds__(kB) void k_density(SweepArgs a) {
 template <int DIM, int KIND, int MODE>
 __global__ __launch_bounds__(kB) void k_walls(SweepArgs a) {
   constexpr int TPP = tpp_of(DIM);
-  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int tid = logical_block(a.remap) * blockDim.x + threadIdx.x;
   const int s = tid / TPP, h = tid % TPP;
   if (s >= a.n) return;  // whole lane groups leave together
   const sph_solver_params& p = a.prm;
@@ -333,7 +56,9 @@ __global__ __launch_bounds__(kB) void k_walls(SweepArgs a) {
       if ((int)Qj.z != kFluid) return;
       const float4 Pj = a.R.P[j];
       const V3 dr = min_image<DIM>(Pi, Pj, a.geo);
-      const float d = sqrtf(dist2<DIM>(dr.x, dr.y, dr.z));
+      const float d2 = dist2<DIM>(dr.x, dr.y, dr.z);
+      if (!in_cutoff<DIM>(a, d2, Pi, Pj)) return;
+      const float d = sqrtf(d2);
       const float w = K.w(d * p.one_over_h);
       ws += w;
       if (MODE == 0) {
@@ -385,7 +110,7 @@ __global__ __launch_bounds__(kB) void k_walls(SweepArgs a) {
 template <int DIM, int KIND, int SOLVER, bool TVF>
 __global__ __launch_bounds__(kB) void k_force(SweepArgs a) {
   constexpr int TPP = tpp_of(DIM);
-  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int tid = logical_block(a.remap) * blockDim.x + threadIdx.x;
   const int s = tid / TPP, h = tid % TPP;
   if (s >= a.n) return;  // whole lane groups leave together
   const sph_solver_params& p = a.prm;
@@ -416,6 +141,7 @@ __global__ __launch_bounds__(kB) void k_force(SweepArgs a) {
     const V3 dr = min_image<DIM>(Pi, Pj, a.geo);
     const float d2 = dist2<DIM>(dr.x, dr.y, dr.z);
     if (d2 == 0.0f) return;  // self pair / coincident: every term is exactly 0
+    if (!in_cutoff<DIM>(a, d2, Pi, Pj)) return;
     const float d = sqrtf(d2);
     const float q = d * p.one_over_h;
     const float gw = K.dw(q);
@@ -573,19 +299,11 @@ using namespace sph;
 
 namespace {
 
-Geo make_geo(const float* box) {
-  Geo g;
-  for (int d = 0; d < 3; ++d) {
-    g.box[d] = box[d];
-    g.inv[d] = box[d] != 0.0f ? 1.0f / box[d] : 0.0f;
-  }
-  return g;
-}
 
 template <int DIM, int KIND>
 int launch_sweeps(sph_ws* ws, SweepArgs& a, hipStream_t st) {
   const int n = a.n;
-  dim3 grid((unsigned)cdivl((int64_t)n * tpp_of(DIM), kB));
+  dim3 grid(xcd_grid(cdivl((int64_t)n * tpp_of(DIM), kB), a.remap));
   const auto& p = a.prm;
   // density
   if (!p.density_evolution)
@@ -627,68 +345,26 @@ int launch_kind(sph_ws* ws, SweepArgs& a, hipStream_t st) {
   }
 }
 
-int check_state(const sph_solver_params* prm, const sph_state_in* in, const sph_state_out* out) {
-  if (!prm || !in || !out) { set_error("null argument"); return SPH_ERR_ARG; }
-  if (!in->r || !in->u || !in->v || !in->rho || !in->p || !in->mass || !in->eta || !in->tag) {
-    set_error("state is missing r/u/v/rho/p/mass/eta/tag");
-    return SPH_ERR_ARG;
-  }
-  if (!out->u || !out->v || !out->dudt || !out->dvdt || !out->rho || !out->p || !out->drhodt) {
-    set_error("output is missing u/v/dudt/dvdt/rho/p/drhodt");
-    return SPH_ERR_ARG;
-  }
-  if (prm->heat_conduction && (!in->T || !in->dTdt || !in->kappa || !in->Cp || !out->T ||
-                               !out->dTdt)) {
-    set_error("heat conduction needs T, dTdt, kappa, Cp in and T, dTdt out");
-    return SPH_ERR_ARG;
-  }
-  if ((prm->solver == 1 || (prm->free_slip && prm->bc_trick)) && !in->nw) {
-    set_error("wall normals nw are required for this boundary treatment");
-    return SPH_ERR_ARG;
-  }
-  if (prm->kernel < 0 || prm->kernel > 6) { set_error("unknown kernel"); return SPH_ERR_ARG; }
-  return SPH_OK;
-}
-
-SweepArgs make_args(sph_ws* ws, int bank, const sph_solver_params* prm, const sph_state_in* in,
-                    sph_state_out* out) {
-  SweepArgs a;
-  a.prm = *prm;
-  a.in = *in;
-  a.out = *out;
-  a.R = Recs{ws->P, ws->U0, ws->V0, ws->Q0, ws->NW, ws->U1, ws->Q1, ws->U2, ws->V2, ws->Q2,
-             ws->UT, ws->drho};
-  a.perm = ws->bank[bank].perm;
-  a.ell = ws->bank[bank].ell;
-  a.cnt = ws->bank[bank].cnt;
-  a.geo = make_geo(prm->box);
-  a.n = ws->cfg.n;
-  a.npad = ws->npad;
-  a.lane_width = ws->lane_width;
-  a.Uf = a.R.U1; a.Vf = a.R.V0; a.Qf = a.R.Q1;
-  a.heat_in_density = prm->heat_conduction && !prm->bc_trick;
-  a.bc.kind = 0;
-  a.bc.u_lid[0] = a.bc.u_lid[1] = a.bc.u_lid[2] = 0.0f;
-  return a;
-}
 
 }  // namespace
 
 extern "C" int sph_forward(sph_ws* ws, int bank, const sph_solver_params* prm,
                            const sph_state_in* in, sph_state_out* out, hipStream_t st) {
-  if (!ws || bank < 0 || bank > 1) { set_error("sph_forward: bad workspace/bank"); return SPH_ERR_ARG; }
   int rc = check_state(prm, in, out);
   if (rc) return rc;
-  if (prm->dim != ws->cfg.dim) { set_error("sph_forward: dim mismatch"); return SPH_ERR_ARG; }
+  rc = check_bank(ws, bank, prm, "sph_forward");
+  if (rc) return rc;
   SweepArgs a = make_args(ws, bank, prm, in, out);
   dim3 grid((unsigned)cdivl(a.n, kB));
   if (prm->dim == 2) {
     SPH_LAUNCH(ws, "gather_state", st,
-               k_gather_state<2><<<grid, kB, 0, st>>>(a, in->r, in->u, in->v, nullptr));
+               k_gather_state<2><<<grid, kB, 0, st>>>(a, in->r, in->u, in->v, nullptr, nullptr,
+                                                      nullptr));
     return launch_kind<2>(ws, a, st);
   }
   SPH_LAUNCH(ws, "gather_state", st,
-             k_gather_state<3><<<grid, kB, 0, st>>>(a, in->r, in->u, in->v, nullptr));
+             k_gather_state<3><<<grid, kB, 0, st>>>(a, in->r, in->u, in->v, nullptr, nullptr,
+                                                    nullptr));
   return launch_kind<3>(ws, a, st);
 }
 
@@ -696,16 +372,46 @@ extern "C" int sph_step(sph_ws* ws, int bank, const sph_solver_params* prm, floa
                         const sph_bc_spec* bc, const sph_state_in* in, float* r_out,
                         sph_state_out* out, const int64_t* prev_info, int64_t max_occupancy,
                         int64_t* info, hipStream_t st) {
-  if (!ws || bank < 0 || bank > 1 || !r_out || !info) {
+  if (!r_out || !info) {
     set_error("sph_step: bad argument");
     return SPH_ERR_ARG;
   }
   int rc = check_state(prm, in, out);
   if (rc) return rc;
+  rc = check_bank(ws, bank, prm, "sph_step");
+  if (rc) return rc;
   if (!in->dudt || !in->dvdt) { set_error("sph_step: dudt/dvdt required"); return SPH_ERR_ARG; }
-  if (prm->dim != ws->cfg.dim) { set_error("sph_step: dim mismatch"); return SPH_ERR_ARG; }
   if (in->g_ext) { set_error("sph_step: position-dependent g_ext is not fused"); return SPH_ERR_ARG; }
   const int n = ws->cfg.n, dim = prm->dim;
+  if (bank == kSkinBank) {
+    // 1. kick/drift + rebuild test + reference-grid occupancy; skin binning if needed
+    rc = skin_begin(ws, in->r, in->u, in->dudt, in->dvdt, r_out, out->u, out->v, dt, tvf_coef,
+                    prm->box, prev_info, info, st);
+    if (rc) return rc;
+    ws->sk.stepped = true;
+    // 2. sorted records (and, on a rebuild, the build positions), then the lists
+    SweepArgs a = make_args(ws, bank, prm, in, out);
+    if (bc) a.bc = *bc;
+    dim3 grid((unsigned)cdivl(n, kB));
+    if (dim == 2)
+      SPH_LAUNCH(ws, "gather_state", st,
+                 k_gather_state<2><<<grid, kB, 0, st>>>(a, r_out, out->u, out->v, ws->sk.b.pos,
+                                                        ws->sk.flag, ws->sk.posb));
+    else
+      SPH_LAUNCH(ws, "gather_state", st,
+                 k_gather_state<3><<<grid, kB, 0, st>>>(a, r_out, out->u, out->v, ws->sk.b.pos,
+                                                        ws->sk.flag, ws->sk.posb));
+    rc = skin_build(ws, info, st);
+    if (rc) return rc;
+    // 3. forward + case hook over the pairs inside the cutoff, counted on the way
+    a.in.r = r_out;
+    a.in.u = out->u;
+    a.in.v = out->v;
+    a.info = info;
+    rc = dim == 2 ? launch_kind<2>(ws, a, st) : launch_kind<3>(ws, a, st);
+    if (rc) return rc;
+    return skin_finish(ws, max_occupancy, info, st);
+  }
   // 1. kick/drift into the output buffers, binned at the new positions
   rc = bin_init(ws, prev_info, info, st);
   if (rc) return rc;
@@ -721,10 +427,12 @@ extern "C" int sph_step(sph_ws* ws, int bank, const sph_solver_params* prm, floa
   float4* nl_pos = ws->bank[bank].pos;
   if (dim == 2)
     SPH_LAUNCH(ws, "gather_state", st,
-               k_gather_state<2><<<grid, kB, 0, st>>>(a, r_out, out->u, out->v, nl_pos));
+               k_gather_state<2><<<grid, kB, 0, st>>>(a, r_out, out->u, out->v, nl_pos, nullptr,
+                                                      nullptr));
   else
     SPH_LAUNCH(ws, "gather_state", st,
-               k_gather_state<3><<<grid, kB, 0, st>>>(a, r_out, out->u, out->v, nl_pos));
+               k_gather_state<3><<<grid, kB, 0, st>>>(a, r_out, out->u, out->v, nl_pos, nullptr,
+                                                      nullptr));
   rc = build_pairs(ws, bank, max_occupancy, info, st);
   if (rc) return rc;
   // 3. forward + case hook

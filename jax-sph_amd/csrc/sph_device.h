@@ -96,6 +96,11 @@ struct Kern<SPH_KERNEL_QSK, DIM> {
     float q1 = fmaxf(0.0f, 1.0f - q), q2 = fmaxf(0.0f, 2.0f - q), q3 = fmaxf(0.0f, 3.0f - q);
     return -(5.0f * s * ih) * (p4(q3) - 6.0f * p4(q2) + 15.0f * p4(q1));
   }
+  // d^2W/dr^2 = 20 sigma / h^2 [(3-q)^3 - 6 (2-q)^3 + 15 (1-q)^3]_+ (for the adjoint)
+  __device__ __forceinline__ static float d2w(float q, float s, float ih) {
+    float q1 = fmaxf(0.0f, 1.0f - q), q2 = fmaxf(0.0f, 2.0f - q), q3 = fmaxf(0.0f, 3.0f - q);
+    return (20.0f * s * ih * ih) * (q3 * q3 * q3 - 6.0f * q2 * q2 * q2 + 15.0f * q1 * q1 * q1);
+  }
   static constexpr float kSupport = 3.0f;
 };
 
@@ -110,6 +115,11 @@ struct Kern<SPH_KERNEL_WC2K, DIM> {
     float q1 = fmaxf(0.0f, 1.0f - 0.5f * q);
     if (DIM == 1) return s * ih * (-3.0f * q * q1 * q1);
     return -(5.0f * s * ih) * q * (q1 * q1 * q1);
+  }
+  // d^2W/dr^2 (dim >= 2) = -5 sigma / h^2 (1 - q/2)_+^2 (1 - 2q) (for the adjoint)
+  __device__ __forceinline__ static float d2w(float q, float s, float ih) {
+    float q1 = fmaxf(0.0f, 1.0f - 0.5f * q);
+    return -(5.0f * s * ih * ih) * (q1 * q1) * (1.0f - 2.0f * q);
   }
   static constexpr float kSupport = 2.0f;
 };
@@ -221,6 +231,12 @@ struct Eos {
     if (gamma != 1.0f) x = powf(x, gamma);
     return p_ref * (x - 1.0f) + p_bg;
   }
+  // dp/drho (for the adjoint)
+  __device__ __forceinline__ float dp_drho(float rho) const {
+    if (kind == 1) return rk;
+    if (gamma == 1.0f) return p_ref / rho_ref;
+    return p_ref * gamma * powf(rho / rho_ref, gamma - 1.0f) / rho_ref;
+  }
   __device__ __forceinline__ float rho_fn(float p) const {
     if (kind == 1) return (p - p_bg) / rk + rho_ref;
     float t = p + p_ref - p_bg;
@@ -236,6 +252,23 @@ __device__ __forceinline__ Eos make_eos(const sph_solver_params& p) {
   e.p_ref = p.p_ref; e.rho_ref = p.rho_ref; e.p_bg = p.p_bg;
   e.gamma = p.gamma; e.inv_gamma = p.inv_gamma; e.rk = p.riemann_k;
   return e;
+}
+
+// ------------------------------------------------------------ XCD mapping
+// MI355X dispatches the workgroups of a launch to its 8 XCDs round-robin by
+// workgroup id, and each XCD has its own L2.  With remap on (and a grid that
+// is a multiple of 8), XCD x runs the contiguous logical blocks
+// [x * per, (x + 1) * per): cell-sorted particles that share neighbours then
+// share one L2 instead of being spread over all eight.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int logical_block(int remap) {
+  const int b = blockIdx.x;
+  if (!remap) return b;
+  const int per = gridDim.x / kXcds;
+  return (b % kXcds) * per + b / kXcds;
+}
+__host__ __device__ inline unsigned xcd_grid(long long blocks, int remap) {
+  return (unsigned)(remap ? (blocks + kXcds - 1) / kXcds * kXcds : blocks);
 }
 
 // tags (utils.py:22-32)

@@ -48,6 +48,7 @@ struct sph_ws {
   int32_t npad = 0;        // N rounded up to 64 (pair-buffer column stride)
   int32_t tpp = 1;         // lanes per particle (tpp_of(dim))
   int32_t lane_width = 0;  // capacity of one sub-list
+  int32_t xcd_remap = 1;   // XCD-contiguous block order of the pair build and sweeps
   int64_t bytes = 0;
   // binning scratch (shared by both banks)
   int32_t* cell_count = nullptr;  // ncell
@@ -58,6 +59,24 @@ struct sph_ws {
   int64_t* scan_tmp = nullptr;    // block carries of the scans
   int64_t* cnt64 = nullptr;       // N: pair count per original id
   sph::Bank bank[2];
+  // Verlet-skin lists of the fused step: pairs closer than cutoff + skin on a
+  // grid whose cells are >= cutoff + skin, rebuilt only after some particle
+  // has moved more than `trigger` since the last build.
+  struct Skin {
+    bool on = false;
+    bool stepped = false;     // a step has built the lists (perm etc. are defined)
+    float skin = 0.0f;
+    float reach_sq = 0.0f;    // (cutoff + skin)^2 rounded up: build predicate
+    float trigger_sq = 0.0f;  // squared displacement that forces a rebuild
+    int32_t cps[3] = {1, 1, 1};
+    float cell_size[3] = {1.0f, 1.0f, 1.0f};
+    int64_t ncell = 0;
+    int32_t lane_width = 0;
+    int32_t* cell_count = nullptr;  // ncell, all zero between builds
+    int32_t* flag = nullptr;        // device: 1 = rebuild the lists this step
+    float4* posb = nullptr;         // N: position of each original id at the last build
+    sph::Bank b;                    // perm, iperm, cell_start, pos, ell, cnt of the lists
+  } sk;
   // sorted particle records used by the physics sweeps
   float4 *P = nullptr;   // x, y, z, mass
   float4 *U0 = nullptr, *U1 = nullptr, *U2 = nullptr;  // u, rho
@@ -66,6 +85,8 @@ struct sph_ws {
   float4* NW = nullptr;  // wall normal
   float4* UT = nullptr;  // Riemann wall velocity u_tilde
   float* drho = nullptr; // sorted drhodt
+  // adjoint sweeps: sorted cotangent of dudt, then the per-particle partials
+  float4 *LA = nullptr, *AB0 = nullptr, *AB1 = nullptr;
   // profiling
   bool prof_on = false;
   std::vector<sph::ProfEntry> prof_pending;

@@ -1,6 +1,7 @@
 // Workspace lifetime, error reporting and the profiling hooks of libsph.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstring>
 #include <string>
 
@@ -131,10 +132,94 @@ extern "C" int sph_ws_create(const sph_nl_config* cfg, sph_ws** out) {
   A(ws->Q0, N); A(ws->Q1, N); A(ws->Q2, N);
   A(ws->NW, N); A(ws->UT, N);
   A(ws->drho, N);
+  A(ws->LA, N); A(ws->AB0, N); A(ws->AB1, N);
 #undef A
+  // the cell histogram is kept all-zero between launches (scans clear it)
+  if (rc == SPH_OK) {
+    hipError_t e = hipMemset(ws->cell_count, 0, sizeof(int32_t) * (size_t)(ncell ? ncell : 1));
+    if (e != hipSuccess) rc = hip_fail(e, "hipMemset");
+  }
   if (rc != SPH_OK) { sph_ws_destroy(ws); return rc; }
   *out = ws;
   return SPH_OK;
+}
+
+extern "C" int sph_skin_enable(sph_ws* ws, float skin, int32_t width) {
+  if (!ws) { set_error("sph_skin_enable: null workspace"); return SPH_ERR_ARG; }
+  auto& S = ws->sk;
+  if (S.on) { set_error("sph_skin_enable: skin lists are already enabled"); return SPH_ERR_ARG; }
+  if (!(skin > 0.0f) || !std::isfinite(skin) || width <= 0) {
+    set_error("sph_skin_enable: skin must be > 0 and width positive");
+    return SPH_ERR_ARG;
+  }
+  const sph_nl_config& c = ws->cfg;
+  if (!c.use_cells || c.ordered) return 0;
+  double maxbox = 0.0;
+  for (int d = 0; d < c.dim; ++d) maxbox = c.box[d] > maxbox ? c.box[d] : maxbox;
+  // displacements are tracked to ~1e-7 box; the skin must dwarf that
+  if ((double)skin < 1e-4 * maxbox) return 0;
+  const double reach = (double)c.cutoff + (double)skin;
+  int32_t cps[3] = {1, 1, 1};
+  float cs[3] = {1.0f, 1.0f, 1.0f};
+  int64_t ncell = 1;
+  for (int d = 0; d < c.dim; ++d) {
+    // pairs inside the cutoff must lie in adjacent reference cells, as the
+    // reference's candidate search assumes, with room for the rounding of r/cell
+    if ((double)c.cell_size[d] * (1.0 - 1e-4) < (double)c.cutoff || c.cps[d] > 100000) return 0;
+    // skin cells strictly wider than cutoff + skin, at least 3 per side
+    const int k = (int)std::floor((double)c.box[d] / (reach * (1.0 + 1e-5)));
+    if (k < 3) return 0;
+    cps[d] = k;
+    cs[d] = (float)((double)c.box[d] / k);
+    ncell *= k;
+  }
+  const int64_t N = c.n, npad = ws->npad;
+  const int64_t nr = c.dim == 3 ? 27 : 9;
+  int64_t lw = ((int64_t)width + ws->tpp - 1) / ws->tpp + nr;
+  const int64_t lw_n = (N + ws->tpp - 1) / ws->tpp + 1;
+  lw = lw < lw_n ? lw : lw_n;
+  if ((double)lw * ws->tpp * (double)npad > 4.0e10) {
+    set_error("sph_skin_enable: skin pair buffer would exceed 160 GB");
+    return SPH_ERR_SIZE;
+  }
+  int rc = SPH_OK;
+#define A(ptr, n) if (rc == SPH_OK) rc = alloc(ws, &(ptr), (size_t)(n))
+  A(S.cell_count, ncell);
+  A(S.flag, 1);
+  A(S.posb, N);
+  A(S.b.perm, N);
+  A(S.b.iperm, N);
+  A(S.b.cell_start, ncell + 1);
+  A(S.b.pos, N);
+  A(S.b.ell, ws->tpp * lw * npad);
+  A(S.b.cnt, ws->tpp * npad);
+#undef A
+  if (rc == SPH_OK) {
+    const int32_t one = 1;  // the first step builds the lists
+    hipError_t e = hipMemset(S.cell_count, 0, sizeof(int32_t) * (size_t)ncell);
+    if (e == hipSuccess) e = hipMemcpy(S.flag, &one, sizeof(one), hipMemcpyHostToDevice);
+    if (e != hipSuccess) rc = hip_fail(e, "sph_skin_enable");
+  }
+  if (rc != SPH_OK) {
+    void* ps[] = {S.cell_count, S.flag, S.posb, S.b.perm, S.b.iperm, S.b.cell_start, S.b.pos,
+                  S.b.ell, S.b.cnt};
+    hipDeviceSynchronize();
+    for (void* p : ps)
+      if (p) hipFree(p);
+    S = sph_ws::Skin();
+    return rc;
+  }
+  for (int d = 0; d < 3; ++d) {
+    S.cps[d] = cps[d];
+    S.cell_size[d] = cs[d];
+  }
+  S.skin = skin;
+  S.reach_sq = (float)(reach * reach * (1.0 + 1e-6));
+  S.trigger_sq = (float)((0.45 * skin) * (0.45 * skin));
+  S.ncell = ncell;
+  S.lane_width = (int32_t)lw;
+  S.on = true;
+  return 1;
 }
 
 extern "C" int sph_ws_destroy(sph_ws* ws) {
@@ -142,12 +227,19 @@ extern "C" int sph_ws_destroy(sph_ws* ws) {
   hipDeviceSynchronize();
   void* ptrs[] = {ws->cell_count, ws->pcell, ws->pslot, ws->unsorted, ws->off64, ws->scan_tmp,
                   ws->cnt64, ws->P, ws->U0, ws->U1, ws->U2, ws->V0, ws->V2, ws->Q0, ws->Q1,
-                  ws->Q2, ws->NW, ws->UT, ws->drho};
+                  ws->Q2, ws->NW, ws->UT, ws->drho, ws->LA, ws->AB0, ws->AB1};
   for (void* p : ptrs)
     if (p) hipFree(p);
   for (auto& b : ws->bank) {
     void* bp[] = {b.perm, b.iperm, b.cell_start, b.pos, b.ell, b.cnt};
     for (void* p : bp)
+      if (p) hipFree(p);
+  }
+  {
+    auto& S = ws->sk;
+    void* ps[] = {S.cell_count, S.flag, S.posb, S.b.perm, S.b.iperm, S.b.cell_start, S.b.pos,
+                  S.b.ell, S.b.cnt};
+    for (void* p : ps)
       if (p) hipFree(p);
   }
   for (auto& e : ws->prof_pending) { ws->prof_pool.push_back(e.a); ws->prof_pool.push_back(e.b); }
@@ -157,6 +249,16 @@ extern "C" int sph_ws_destroy(sph_ws* ws) {
 }
 
 extern "C" int64_t sph_ws_bytes(const sph_ws* ws) { return ws ? ws->bytes : 0; }
+
+extern "C" int sph_ws_tune(sph_ws* ws, const char* key, int64_t value) {
+  if (!ws || !key) { set_error("sph_ws_tune: null argument"); return SPH_ERR_ARG; }
+  if (std::strcmp(key, "xcd_remap") == 0) {
+    ws->xcd_remap = value ? 1 : 0;
+    return SPH_OK;
+  }
+  set_error(std::string("sph_ws_tune: unknown key ") + key);
+  return SPH_ERR_ARG;
+}
 
 extern "C" int sph_prof_enable(sph_ws* ws, int on) {
   if (!ws) return SPH_ERR_ARG;

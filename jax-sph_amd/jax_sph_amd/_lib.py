@@ -89,6 +89,7 @@ def lib():
         "sph_ws_create": (c_int, [POINTER(NLConfig), POINTER(c_void_p)]),
         "sph_ws_destroy": (c_int, [vp]),
         "sph_ws_bytes": (c_int64, [vp]),
+        "sph_ws_tune": (c_int, [vp, c_char_p, c_int64]),
         "sph_nl_fill_max": (c_int, [c_int32, c_int32, vp, POINTER(c_float), POINTER(c_int32), vp,
                                     vp, vp]),
         "sph_nl_needs_rebuild": (c_int, [vp, vp, vp, c_float, vp, vp]),
@@ -96,6 +97,8 @@ def lib():
         "sph_nl_finalize": (c_int, [vp, c_int64, vp, vp]),
         "sph_nl_scan_budget": (c_int, [vp, c_int, c_int64, c_int64, vp, vp]),
         "sph_nl_write_idx": (c_int, [vp, c_int, vp, c_int64, vp, vp]),
+        "sph_skin_enable": (c_int, [vp, c_float, c_int32]),
+        "sph_nl_write_idx_skin": (c_int, [vp, vp, vp, c_int64, vp]),
         "sph_kernel_eval": (c_int, [c_int32, c_int32, c_float, c_float, vp, c_int64, vp, vp, vp]),
         "sph_forward": (c_int, [vp, c_int, POINTER(SolverParams), POINTER(StateIn),
                                 POINTER(StateOut), vp]),
@@ -158,15 +161,38 @@ def as_i32(x, dev=None) -> torch.Tensor:
     return t
 
 
+SKIN_BANK = 2
+
+
 class Workspace:
-    """Owns one ``sph_ws`` (cell arrays, two banks of pairs, sorted records)."""
+    """Owns one ``sph_ws`` (cell arrays, two banks of exact pairs, optional
+    skin lists as bank 2, sorted records)."""
 
     def __init__(self, cfg: NLConfig):
         self.cfg = cfg
         h = c_void_p()
         check(lib().sph_ws_create(ctypes.byref(cfg), ctypes.byref(h)), "sph_ws_create")
         self.handle = h
-        self.generation = [0, 0]
+        self.generation = [0, 0, 0]
+        self.skin = 0.0
+
+    def enable_skin(self, skin: float, width: int) -> bool:
+        """Skin lists for the fused step (sph_skin_enable); False if this
+        geometry cannot use them."""
+        rc = lib().sph_skin_enable(self.handle, float(skin), int(width))
+        if rc < 0:
+            check(rc, "sph_skin_enable")
+        if rc == 1:
+            self.skin = float(skin)
+        return rc == 1
+
+    @property
+    def skin_on(self) -> bool:
+        return self.skin > 0.0
+
+    def tune(self, key: str, value: int):
+        """Launch-shape knob of the library (sph_ws_tune), e.g. ``xcd_remap``."""
+        check(lib().sph_ws_tune(self.handle, key.encode(), int(value)), "sph_ws_tune")
 
     @property
     def bytes(self) -> int:

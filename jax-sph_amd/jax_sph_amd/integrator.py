@@ -9,7 +9,8 @@ When every piece is one this package understands -- ``model`` from
 ``WCSPH.forward_wrapper()``, ``shift_fn`` from ``space.periodic``, a
 ``CaseBC`` (or None) for ``bc_fn``, no moving-wall normals, a constant body
 force and a neighbour list rebuilt every step (``dr_threshold == 0``) -- the
-whole step is one ``sph_step`` call into the HIP library.  Otherwise the same
+whole step is one ``sph_step`` call into the HIP library, on the list's
+Verlet-skin buffers when it has them (same pair sets as a per-step rebuild).  Otherwise the same
 HIP kernels run one call at a time and Python callables are applied between
 them.  Either way the input dict is left untouched and a new dict is returned.
 """
@@ -68,7 +69,7 @@ def si_euler(tvf: float, model: Callable, shift_fn: Callable, bc_fn: Callable,
             raise ValueError("state size does not match the neighbour list")
         out = solver.new_outputs(s)
         r_out = torch.empty_like(s["r"])
-        bank = 1 - neighbors.bank
+        bank = neighbors._search.step_bank(neighbors)
         ws.generation[bank] += 1
         info = torch.empty(8, dtype=torch.int64, device=r_out.device)
         dtf = _f(dt)

@@ -20,6 +20,16 @@ bit-exact); a particle exactly on the upper box face is binned into the last
 cell instead of being aliased into a wrong cell.  Each ``update`` writes into
 the other of two buffer banks, so the list an update started from stays
 readable; a list two updates old raises ``StaleNeighborListError``.
+
+Skin lists (extra keyword ``skin``, default ``"auto"`` = 0.1 * cutoff; 0
+turns them off): when the fused ``si_euler`` step advances a list, it keeps
+pairs within ``cutoff + skin`` and rebuilds them on the device only after a
+particle has moved more than ``0.45 * skin``; the reference predicate is
+applied to every pair on every step, so pair sets, occupancy and error bits
+are those of a per-step rebuild.  A list produced that way is stale after the
+next fused step.  If a skin list outgrows its buffer the step reports
+``NEIGHBOR_LIST_OVERFLOW`` and the re-allocation sizes the buffer from the
+observed count.
 """
 from __future__ import annotations
 
@@ -198,7 +208,7 @@ class _NeighborSearch:
     def __init__(self, displacement_or_metric, box, r_cutoff, dr_threshold=0.0,
                  capacity_multiplier=1.25, disable_cell_list=False, mask_self=True,
                  custom_mask_function=None, fractional_coordinates=False, format=Sparse,
-                 backend="jaxmd_vmap", num_partitions=1):
+                 backend="jaxmd_vmap", num_partitions=1, skin="auto"):
         if not isinstance(format, NeighborListFormat):
             raise ValueError(f"Neighbor list format must be a member of NeighborListFormat "
                              f"found {format}.")
@@ -225,6 +235,13 @@ class _NeighborSearch:
         self.backend = backend
         self.num_partitions = int(num_partitions)
         self.use_cells = None
+        if skin == "auto":
+            skin = 0.1 * float(self.cutoff)
+        self.skin = float(skin or 0.0)
+        if self.skin < 0.0:
+            raise ValueError("skin must be >= 0")
+        self._skin_hint = 0
+        self._last_step_info = None
 
     # ------------------------------------------------------------ geometry
     def _geometry(self, dim):
@@ -302,7 +319,25 @@ class _NeighborSearch:
         max_occ = self._max_occupancy(int(info[1]), ws, extra_capacity)
         _lib.check(L.sph_nl_finalize(ws.handle, max_occ, _lib.ptr(info), st), "sph_nl_finalize")
         self._scan_bits(ws, bank, info)
+        self._enable_skin(ws, int(info[3]))
         return NeighborList(self, ws, bank, info, r.clone(), cap, max_occ, self.update)
+
+    def _enable_skin(self, ws, max_pairs):
+        """Skin lists for the fused step, sized from the largest exact list."""
+        if (self.skin <= 0.0 or not self.use_cells or self.format is not Sparse
+                or self.backend != "jaxmd_vmap" or self.dr_threshold != 0.0):
+            return
+        if self._last_step_info is not None:   # re-allocation: keep what was seen
+            self._skin_hint = max(self._skin_hint, int(self._last_step_info[7]))
+            self._last_step_info = None
+        grow = ((float(self.cutoff) + self.skin) / float(self.cutoff)) ** ws.dim
+        width = int(np.ceil(max_pairs * grow * 1.3)) + 8
+        width = max(width, int(np.ceil(self._skin_hint * 1.2)) + 8)
+        if self._skin_width_override is not None:   # tests: force a small buffer
+            width = int(self._skin_width_override)
+        ws.enable_skin(self.skin, min(width, ws.n))
+
+    _skin_width_override = None
 
     def _max_occupancy(self, occ, ws, extra_capacity):
         n = ws.n
@@ -352,7 +387,7 @@ class _NeighborSearch:
             if not int(neighbors.info[4]):
                 return neighbors
         info = torch.empty(8, dtype=torch.int64, device=r.device)
-        bank = 1 - neighbors.bank
+        bank = self.next_exact_bank(neighbors)
         ws.generation[bank] += 1
         _lib.check(L.sph_nl_build(ws.handle, bank, _lib.ptr(r), _lib.ptr(neighbors.info),
                                   neighbors.max_occupancy, _lib.ptr(info), st), "sph_nl_build")
@@ -361,10 +396,23 @@ class _NeighborSearch:
                             neighbors.cell_list_capacity, neighbors.max_occupancy,
                             neighbors.update_fn)
 
+    @staticmethod
+    def next_exact_bank(neighbors: NeighborList) -> int:
+        """Bank for the next exact list: the one ``neighbors`` does not use."""
+        return 1 - neighbors.bank if neighbors.bank < 2 else 0
+
+    def step_bank(self, neighbors: NeighborList) -> int:
+        """Bank the fused step writes: the skin lists when enabled."""
+        ws = neighbors.workspace
+        return _lib.SKIN_BANK if ws.skin_on else self.next_exact_bank(neighbors)
+
     def adopt_step(self, neighbors: NeighborList, bank: int, info: torch.Tensor,
                    r: torch.Tensor) -> NeighborList:
         """NeighborList for a bank just written by the fused step."""
-        self._scan_bits(neighbors.workspace, bank, info)
+        if bank == _lib.SKIN_BANK:
+            self._last_step_info = info
+        else:
+            self._scan_bits(neighbors.workspace, bank, info)
         return NeighborList(self, neighbors.workspace, bank, info, r,
                             neighbors.cell_list_capacity, neighbors.max_occupancy,
                             neighbors.update_fn)
@@ -372,11 +420,30 @@ class _NeighborSearch:
     def write_idx(self, nbrs: NeighborList) -> torch.Tensor:
         m = nbrs.max_occupancy
         idx = torch.empty((2, m), dtype=torch.int32, device=nbrs.info.device)
-        if m:
+        if m and nbrs.bank == _lib.SKIN_BANK:
+            r = _lib.as_f32(nbrs.reference_position)
+            _lib.check(_lib.lib().sph_nl_write_idx_skin(nbrs.workspace.handle, _lib.ptr(r),
+                                                        _lib.ptr(idx), m, _lib.stream()),
+                       "sph_nl_write_idx_skin")
+        elif m:
             _lib.check(_lib.lib().sph_nl_write_idx(nbrs.workspace.handle, nbrs.bank, _lib.ptr(idx),
                                                    m, _lib.ptr(nbrs.info), _lib.stream()),
                        "sph_nl_write_idx")
         return idx
+
+    def exact_at_reference(self, nbrs: NeighborList) -> NeighborList:
+        """An exact list (bank 0/1) of the pairs of a skin list, built at its
+        reference position; used when a skin list meets other positions."""
+        ws = nbrs.workspace
+        info = torch.empty(8, dtype=torch.int64, device=nbrs.info.device)
+        bank = self.next_exact_bank(nbrs)
+        ws.generation[bank] += 1
+        r = _lib.as_f32(nbrs.reference_position)
+        _lib.check(_lib.lib().sph_nl_build(ws.handle, bank, _lib.ptr(r), _lib.ptr(nbrs.info),
+                                           nbrs.max_occupancy, _lib.ptr(info), _lib.stream()),
+                   "sph_nl_build")
+        return NeighborList(self, ws, bank, info, r, nbrs.cell_list_capacity,
+                            nbrs.max_occupancy, nbrs.update_fn)
 
 
 def neighbor_list(displacement_or_metric, box, r_cutoff: float, dr_threshold: float = 0.0,
@@ -389,7 +456,8 @@ def neighbor_list(displacement_or_metric, box, r_cutoff: float, dr_threshold: fl
                         capacity_multiplier, disable_cell_list, mask_self,
                         custom_mask_function, fractional_coordinates, format,
                         backend=static_kwargs.get("backend", "jaxmd_vmap"),
-                        num_partitions=static_kwargs.get("num_partitions", 1))
+                        num_partitions=static_kwargs.get("num_partitions", 1),
+                        skin=static_kwargs.get("skin", "auto"))
 
     def allocate_fn(position, extra_capacity: int = 0, **kwargs):
         return s.allocate(position, extra_capacity, **kwargs)

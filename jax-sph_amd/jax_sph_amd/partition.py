@@ -44,10 +44,10 @@ def _scan_neighbor_list(displacement_or_metric, box, r_cutoff, dr_threshold=0.0,
 def _vmap_neighbor_list(displacement_or_metric, box, r_cutoff, dr_threshold=0.0,
                         capacity_multiplier=1.25, disable_cell_list=False, mask_self=True,
                         custom_mask_function=None, fractional_coordinates=False,
-                        format=NeighborListFormat.Sparse, **static_kwargs):
+                        format=NeighborListFormat.Sparse, skin="auto", **static_kwargs):
     return vmap_neighbor_list(displacement_or_metric, box, r_cutoff, dr_threshold,
                               capacity_multiplier, disable_cell_list, mask_self,
-                              custom_mask_function, fractional_coordinates, format)
+                              custom_mask_function, fractional_coordinates, format, skin=skin)
 
 
 def _matscipy_neighbor_list(*args, **kwargs):
@@ -69,11 +69,16 @@ def neighbor_list(displacement_or_metric, box_size, r_cutoff: float,
                   mask_self: bool = True, custom_mask_function=None,
                   fractional_coordinates: bool = False,
                   format: NeighborListFormat = NeighborListFormat.Sparse,
-                  num_particles_max: int = None, num_partitions: int = 1, pbc=None):
+                  num_particles_max: int = None, num_partitions: int = 1, pbc=None,
+                  skin="auto"):
     """partition.py:492-571: same arguments; ``num_particles_max``/``pbc`` only
-    matter to the matscipy backend."""
+    matter to the matscipy backend.  ``skin`` (not in the reference) sets the
+    Verlet skin of the fused step's lists for ``jaxmd_vmap`` ("auto" = 0.1 *
+    cutoff, 0 = rebuild exact lists every step); results are the same."""
     assert backend in BACKENDS, f"Unknown backend {backend}"
+    kw = dict(num_particles_max=num_particles_max, num_partitions=num_partitions, pbc=pbc)
+    if backend == "jaxmd_vmap":
+        kw["skin"] = skin
     return BACKENDS[backend](
         displacement_or_metric, box_size, r_cutoff, dr_threshold, capacity_multiplier,
-        disable_cell_list, mask_self, custom_mask_function, fractional_coordinates, format,
-        num_particles_max=num_particles_max, num_partitions=num_partitions, pbc=pbc)
+        disable_cell_list, mask_self, custom_mask_function, fractional_coordinates, format, **kw)

@@ -194,6 +194,11 @@ class WCSPH:
             if s["r"].shape[0] != ws.n:
                 raise ValueError("state size does not match the neighbour list")
             neighbors._check_live()
+            if neighbors.bank == _lib.SKIN_BANK and \
+                    s["r"].data_ptr() != neighbors.reference_position.data_ptr():
+                # a skin list is filtered at the positions it belongs to; other
+                # positions get an exact list of those pairs
+                neighbors = neighbors._search.exact_at_reference(neighbors)
             out = self.new_outputs(s)
             g = self.g_ext_tensor(s["r"])
             prm = self.params(tvf_stress=True)

@@ -43,11 +43,11 @@ class _Ref:
         return st
 
 
-def _make(case, ov, backend="jaxmd_vmap"):
+def _make(case, ov, backend="jaxmd_vmap", skin="auto"):
     cfg, box, state, g, bc, eos, disp, shift, solver = G.setup(case, ov)
     fn = partition.neighbor_list(disp, box, r_cutoff=solver._kernel_fn.cutoff, backend=backend,
                                  capacity_multiplier=1.25, mask_self=False,
-                                 format=partition.Sparse)
+                                 format=partition.Sparse, skin=skin)
     nb = fn.allocate(state["r"])
     adv = integrator.si_euler(cfg.solver.tvf, solver.forward_wrapper(), shift, bc, None)
     return cfg, state, nb, adv, _Ref(cfg, box, solver, bc), fn, shift, solver, bc
@@ -56,6 +56,7 @@ def _make(case, ov, backend="jaxmd_vmap"):
 STATE = ("r", "u", "v", "rho")
 
 
+@pytest.mark.parametrize("skin", ["auto", 0.0])
 @pytest.mark.parametrize("case,ov", [
     ("tgv", {"case": {"dim": 2, "dx": 0.02}}),
     ("tgv", {"case": {"dim": 2, "dx": 0.02}, "solver": {"tvf": 1.0}}),
@@ -63,8 +64,8 @@ STATE = ("r", "u", "v", "rho")
     ("ldc", {"case": {"dx": 0.04}}),
     ("db", {"case": {"dx": 0.04}, "solver": {"name": "RIE"}, "kernel": {"name": "WC2K"}}),
 ])
-def test_fused_steps_match_oracle(case, ov):
-    cfg, state, nb, adv, R, *_ = _make(case, ov)
+def test_fused_steps_match_oracle(case, ov, skin):
+    cfg, state, nb, adv, R, *_ = _make(case, ov, skin=skin)
     assert adv._sph_fusable(nb) is not None
     st = state
     steps = 5
@@ -83,8 +84,9 @@ def test_fused_steps_match_oracle(case, ov):
 
 
 def test_fused_equals_composed():
+    # exact lists on both sides: the same pairs in the same order, so bitwise equal
     cfg, state, nb, adv, *_, fn, shift, solver, bc = _make(
-        "ldc", {"case": {"dx": 0.04}, "solver": {"tvf": 1.0}})
+        "ldc", {"case": {"dx": 0.04}, "solver": {"tvf": 1.0}}, skin=0.0)
     composed = integrator.si_euler(cfg.solver.tvf, solver.forward_wrapper(), shift,
                                    lambda s: bc(s), None)
     assert composed._sph_fusable(nb) is None
@@ -145,3 +147,28 @@ def test_overflow_reallocate_and_rerun():
     assert reallocs >= 1  # lattice ties open up as soon as particles move
     ref, ref64 = R.run(state, 20), R.run(state, 20, f64=True)
     G.assert_within_noise_floor(st, ref, ref64, 1e-5, fields=STATE, label="realloc")
+
+
+def test_simulate_loop_matches_oracle():
+    """simulate.py:109-134: warm-up (discarded), then sequence_length + 2 steps,
+    with write_state(step - 1) before each; the trajectory written equals the
+    oracle's rollout."""
+    from jax_sph_amd import case_setup as cs
+    from jax_sph_amd.simulate import simulate
+    ov = {"case": {"dim": 2, "dx": 0.02}, "solver": {"dt": 0.0004, "t_end": 0.004}}
+    cfg = cs.load_config("tgv", ov)
+    written = {}
+
+    def write(step, st):
+        if step in (-1, 5):
+            written[step] = {k: v.clone() for k, v in st.items()}
+
+    res = simulate(cfg, write_fn=write, verbose=False)
+    assert res["cfg"].solver.sequence_length == 10
+    _, state, _, _, R, *_ = _make("tgv", ov)
+    for k in ("r", "u"):
+        assert torch.equal(written[-1][k], state[k])
+    ref5, ref5_64 = R.run(state, 6), R.run(state, 6, f64=True)
+    G.assert_within_noise_floor(written[5], ref5, ref5_64, 1e-5, fields=STATE, label="write 5")
+    ref, ref64 = R.run(state, 12), R.run(state, 12, f64=True)
+    G.assert_within_noise_floor(res["state"], ref, ref64, 1e-5, fields=STATE, label="simulate")
