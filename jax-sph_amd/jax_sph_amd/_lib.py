@@ -65,6 +65,14 @@ class BCSpec(Structure):
     _fields_ = [("kind", c_int32), ("u_lid", c_float * 3)]
 
 
+class CotangentIn(Structure):
+    _fields_ = [("r", c_void_p), ("u", c_void_p), ("a", c_void_p)]
+
+
+class CotangentOut(Structure):
+    _fields_ = [("r", c_void_p), ("u", c_void_p), ("a", c_void_p)]
+
+
 class SphError(RuntimeError):
     """A libsph call returned an error code."""
 
@@ -107,6 +115,8 @@ def lib():
                              c_int64, vp, vp]),
         "sph_integrate": (c_int, [c_int32, c_int64, vp, vp, vp, vp, vp, vp, vp, c_float, c_float,
                                   POINTER(c_float), vp]),
+        "sph_step_vjp": (c_int, [vp, c_int, POINTER(SolverParams), c_float, POINTER(StateIn),
+                                 POINTER(CotangentIn), POINTER(CotangentOut), vp]),
         "sph_prof_enable": (c_int, [vp, c_int]),
         "sph_prof_reset": (c_int, [vp]),
         "sph_prof_read": (c_int, [vp, c_char_p, c_int, POINTER(c_double), i64p, c_int]),
