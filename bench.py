@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n-side", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the 2D TGV forward + adjoint leg")
     ap.add_argument("--cpu-sample-side", type=int, default=20)
     ap.add_argument("--skin", default="auto",
                     help="Verlet skin of the fused step's pair lists ('auto' = 0.1 cutoff, "
@@ -62,6 +64,85 @@ def copy_calibration(mb):
         dst.copy_(src)
     torch.cuda.synchronize()
     print(json.dumps({"copy_calibration_mib": mb, "launches": 5}))
+
+
+def fwd_adjoint_leg(steps=100, every=10, n_dirs=32, eps=1e-3, seed=0):
+    """2D Taylor-Green inverse problem: dx = 0.01 (10,000 particles), tvf = 0,
+    a rollout of `steps` si_euler steps; loss = MSE of the final velocities
+    against the rollout from the analytic initial field; gradient with respect
+    to the initial velocity (perturbed by N(0, 0.05^2)) through the hand-written
+    adjoint with a checkpoint every `every` steps.  Returns the cost ratio
+    t(forward with checkpoints + adjoint) / t(plain forward) and the gradient
+    error against central differences (eps) on `n_dirs` random directions."""
+    import torch
+    from jax_sph_amd import adjoint, integrator, partition
+    from jax_sph_amd import case_setup as cs
+    cfg = cs.load_config("tgv", {"case": {"dim": 2, "dx": 0.01}, "solver": {"tvf": 0.0}})
+    (cfg, box, state, g_ext_fn, bc_fn, nw_fn, eos, key, disp, shift) = \
+        cs.load_case("tgv")(cfg).initialize()
+    solver = cs.make_solver(cfg, disp, eos, g_ext_fn)
+    dt = cfg.solver.dt
+    cut = solver._kernel_fn.cutoff
+    fast_fn = partition.neighbor_list(disp, box, r_cutoff=cut, mask_self=False,
+                                      format=partition.Sparse)
+    exact_fn = partition.neighbor_list(disp, box, r_cutoff=cut, mask_self=False,
+                                       format=partition.Sparse, skin=0.0)
+    adv = integrator.si_euler(0.0, solver.forward_wrapper(), shift, bc_fn, nw_fn)
+
+    def plain(u0, fn=fast_fn):
+        st = dict(state)
+        st["u"], st["v"] = u0, u0
+        nb = fn.allocate(st["r"])
+        for _ in range(steps):
+            st, nb = adv(dt, st, nb)
+        assert not bool(nb.did_buffer_overflow)
+        return st["u"]
+
+    n = state["r"].shape[0]
+    gen = torch.Generator().manual_seed(seed)
+    u_tgt = plain(state["u"])
+    u0 = (state["u"].cpu() + 0.05 * torch.randn(n, 2, generator=gen)).cuda()
+
+    def loss_of(u):
+        return ((u.double() - u_tgt.double()) ** 2).mean()
+
+    plain(u0)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plain(u0)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    st = dict(state)
+    t0 = time.perf_counter()
+    uu = u0.clone().requires_grad_(True)
+    st["u"] = uu
+    u_n, _, runner = adjoint.rollout(st, solver, exact_fn, shift, dt, steps,
+                                     checkpoint_every=every)
+    loss = loss_of(u_n)
+    loss.backward()
+    torch.cuda.synchronize()
+    t_fa = time.perf_counter() - t0
+    g = uu.grad.double()
+    errs, ad_all, fd_all = [], [], []
+    for k in range(n_dirs):
+        d = torch.randn(n, 2, generator=gen).cuda()
+        lp = float(loss_of(plain(u0 + eps * d, exact_fn)))
+        lm = float(loss_of(plain(u0 - eps * d, exact_fn)))
+        fd = (lp - lm) / (2 * eps)
+        ad = float((g * d.double()).sum())
+        ad_all.append(ad)
+        fd_all.append(fd)
+        errs.append(abs(ad - fd) / max(abs(fd), 1e-30))
+    ad_all, fd_all = np.array(ad_all), np.array(fd_all)
+    return {"config": f"2D TGV dx=0.01 ({n} particles), tvf=0, {steps} steps, checkpoint "
+                      f"every {every}, loss = MSE(u_final, target rollout)",
+            "t_forward_s": t_fwd, "t_forward_plus_adjoint_s": t_fa,
+            "cost_ratio": t_fa / t_fwd, "loss": float(loss),
+            "grad_check": {"directions": n_dirs, "eps": eps,
+                           "rel_err_median": float(np.median(errs)),
+                           "rel_err_max": float(np.max(errs)),
+                           "rel_err_vector": float(np.linalg.norm(ad_all - fd_all) /
+                                                   np.linalg.norm(fd_all))}}
 
 
 def cpu_baseline(n_side, seconds=15.0):
@@ -199,6 +280,8 @@ def main():
         "partition_error_code": overflow,
         "fwd_adjoint": None,
     }
+    if not a.no_adjoint:
+        out["fwd_adjoint"] = fwd_adjoint_leg()
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(a.cpu_sample_side)
     print(json.dumps(out), flush=True)

@@ -390,8 +390,7 @@ struct BuildArgs {
   float lo, hi;
   int fast;
   int n, npad, lane_width, mask_self;
-  int64_t* tot_out;       // += pairs found (nullable)
-  int64_t* max_out;       // max pairs of one particle (nullable)
+  int64_t* part;          // [2 * gridDim.x]: pairs found and largest list, per workgroup
   int64_t trunc_bits;     // error bits raised when a sub-list overflows
   const int32_t* gate;    // nullable: skip the build when *gate == 0
   int remap;              // XCD-contiguous block order (grid a multiple of 8)
@@ -537,8 +536,8 @@ __global__ __launch_bounds__(kBlock) void k_nl_build(BuildArgs a) {
     unsigned long long t = 0;
     int mm = 0;
     for (int w = 0; w < kBlock / 64; ++w) { t += tot[w]; mm = mx[w] > mm ? mx[w] : mm; }
-    if (t && a.tot_out) atomicAdd((unsigned long long*)a.tot_out, t);
-    if (a.max_out) atomicMax((unsigned long long*)a.max_out, (unsigned long long)mm);
+    a.part[2 * blockIdx.x] = (int64_t)t;
+    a.part[2 * blockIdx.x + 1] = mm;
   }
 }
 
@@ -546,13 +545,44 @@ __global__ void k_finalize(int64_t* __restrict__ info, int64_t max_occ) {
   if (threadIdx.x == 0 && info[1] > max_occ) info[0] |= SPH_NEIGHBOR_LIST_OVERFLOW;
 }
 
-// End of a skin-list step: the occupancy bit, and the rebuild request is consumed.
-__global__ void k_finalize_skin(int64_t* __restrict__ info, int64_t max_occ,
-                                int32_t* __restrict__ flag) {
-  if (threadIdx.x == 0) {
-    if (max_occ >= 0 && info[1] > max_occ) info[0] |= SPH_NEIGHBOR_LIST_OVERFLOW;
-    *flag = 0;
+// Reduces the per-workgroup (pairs, largest list) partials of one launch:
+// *tot_out = sum, *max_out = max (each nullable); with max_occ >= 0 also the
+// NEIGHBOR_LIST_OVERFLOW bit (partition.py:976).  With a gate, runs only when
+// *gate != 0; with clear_flag, zeroes *clear_flag at the end (the skin step's
+// rebuild request is consumed).
+constexpr int kReduceThreads = 1024;
+__global__ __launch_bounds__(kReduceThreads) void k_reduce_parts(
+    const int64_t* __restrict__ part, int nblocks, int64_t* __restrict__ tot_out,
+    int64_t* __restrict__ max_out, int64_t* __restrict__ info, int64_t max_occ,
+    const int32_t* __restrict__ gate, int32_t* __restrict__ clear_flag) {
+  __shared__ int64_t ws_t[kReduceThreads / 64], ws_m[kReduceThreads / 64];
+  const bool run = !gate || *gate != 0;  // uniform
+  if (run) {
+    int64_t t = 0, m = 0;
+    for (int b = threadIdx.x; b < nblocks; b += kReduceThreads) {
+      t += part[2 * b];
+      m = part[2 * b + 1] > m ? part[2 * b + 1] : m;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      t += __shfl_down(t, o, 64);
+      const int64_t y = __shfl_down(m, o, 64);
+      m = y > m ? y : m;
+    }
+    if ((threadIdx.x & 63) == 0) { ws_t[threadIdx.x >> 6] = t; ws_m[threadIdx.x >> 6] = m; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int64_t tt = 0, mm = 0;
+      for (int w = 0; w < kReduceThreads / 64; ++w) {
+        tt += ws_t[w];
+        mm = ws_m[w] > mm ? ws_m[w] : mm;
+      }
+      if (tot_out) *tot_out = tt;
+      if (max_out) *max_out = mm;
+      if (max_occ >= 0 && tt > max_occ) info[0] |= SPH_NEIGHBOR_LIST_OVERFLOW;
+    }
   }
+  if (clear_flag && threadIdx.x == 0) *clear_flag = 0;
 }
 
 template <int DIM>
@@ -769,8 +799,7 @@ int build_pairs(sph_ws* ws, int bank, int64_t max_occupancy, int64_t* info, hipS
   a.npad = ws->npad;
   a.lane_width = ws->lane_width;
   a.mask_self = c.mask_self;
-  a.tot_out = &info[1];
-  a.max_out = &info[3];
+  a.part = ws->part_build;
   a.trunc_bits = 0;
   a.gate = nullptr;
   a.remap = ws->xcd_remap;
@@ -783,21 +812,23 @@ int build_pairs(sph_ws* ws, int bank, int64_t max_occupancy, int64_t* info, hipS
     if (c.cps[d] < 5) a.fast = 0;
   cut_band(c, &a.lo, &a.hi);
   constexpr int T2 = tpp_of(2), T3 = tpp_of(3);
+  const dim3 grid(xcd_grid(cdiv((int64_t)c.n * (c.dim == 2 ? T2 : T3), kBlock), a.remap));
   if (c.dim == 2) {
-    dim3 grid(xcd_grid(cdiv((int64_t)c.n * T2, kBlock), a.remap));
     if (c.ordered)
       SPH_LAUNCH(ws, "nl_build", st, (k_nl_build<2, T2, true><<<grid, kBlock, 0, st>>>(a)));
     else
       SPH_LAUNCH(ws, "nl_build", st, (k_nl_build<2, T2, false><<<grid, kBlock, 0, st>>>(a)));
   } else {
-    dim3 grid(xcd_grid(cdiv((int64_t)c.n * T3, kBlock), a.remap));
     if (c.ordered)
       SPH_LAUNCH(ws, "nl_build", st, (k_nl_build<3, T3, true><<<grid, kBlock, 0, st>>>(a)));
     else
       SPH_LAUNCH(ws, "nl_build", st, (k_nl_build<3, T3, false><<<grid, kBlock, 0, st>>>(a)));
   }
-  if (max_occupancy >= 0)
-    SPH_LAUNCH(ws, "finalize", st, k_finalize<<<1, 64, 0, st>>>(info, max_occupancy));
+  // occupancy info[1], largest list info[3], and the overflow bit when max_occupancy >= 0
+  SPH_LAUNCH(ws, "reduce_pairs", st,
+             k_reduce_parts<<<1, kReduceThreads, 0, st>>>(ws->part_build, (int)grid.x, &info[1],
+                                                          &info[3], info, max_occupancy,
+                                                          nullptr, nullptr));
   return SPH_OK;
 }
 
@@ -899,25 +930,31 @@ int skin_build(sph_ws* ws, int64_t* info, hipStream_t st) {
   a.npad = ws->npad;
   a.lane_width = S.lane_width;
   a.mask_self = c.mask_self;
-  a.tot_out = nullptr;
-  a.max_out = &info[7];
+  a.part = ws->part_build;
   a.trunc_bits = SPH_NEIGHBOR_LIST_OVERFLOW;
   a.gate = S.flag;
   a.remap = ws->xcd_remap;
   constexpr int T2 = tpp_of(2), T3 = tpp_of(3);
-  if (c.dim == 2) {
-    dim3 grid(xcd_grid(cdiv((int64_t)c.n * T2, kBlock), a.remap));
+  const dim3 grid(xcd_grid(cdiv((int64_t)c.n * (c.dim == 2 ? T2 : T3), kBlock), a.remap));
+  if (c.dim == 2)
     SPH_LAUNCH(ws, "skin_build", st, (k_nl_build<2, T2, false><<<grid, kBlock, 0, st>>>(a)));
-  } else {
-    dim3 grid(xcd_grid(cdiv((int64_t)c.n * T3, kBlock), a.remap));
+  else
     SPH_LAUNCH(ws, "skin_build", st, (k_nl_build<3, T3, false><<<grid, kBlock, 0, st>>>(a)));
-  }
+  // largest skin list of one particle -> info[7] (only on a rebuild)
+  SPH_LAUNCH(ws, "reduce_skin", st,
+             k_reduce_parts<<<1, kReduceThreads, 0, st>>>(ws->part_build, (int)grid.x, nullptr,
+                                                          &info[7], info, -1, S.flag, nullptr));
   return SPH_OK;
 }
 
+// End of a skin-list step: occupancy and largest list from the counting
+// density sweep's partials, the overflow bit, and the rebuild request is consumed.
 int skin_finish(sph_ws* ws, int64_t max_occupancy, int64_t* info, hipStream_t st) {
-  SPH_LAUNCH(ws, "finalize", st,
-             k_finalize_skin<<<1, 64, 0, st>>>(info, max_occupancy, ws->sk.flag));
+  const int64_t blocks = xcd_grid(cdiv((int64_t)ws->cfg.n * ws->tpp, kBlock), ws->xcd_remap);
+  SPH_LAUNCH(ws, "reduce_pairs", st,
+             k_reduce_parts<<<1, kReduceThreads, 0, st>>>(ws->part_sweep, (int)blocks, &info[1],
+                                                          &info[3], info, max_occupancy,
+                                                          nullptr, ws->sk.flag));
   return SPH_OK;
 }
 

@@ -58,6 +58,11 @@ struct sph_ws {
   int64_t* off64 = nullptr;       // N + 1: pair offsets of the idx writer
   int64_t* scan_tmp = nullptr;    // block carries of the scans
   int64_t* cnt64 = nullptr;       // N: pair count per original id
+  // per-workgroup (pairs, largest list) partials of the pair build and of the
+  // counting density sweep, reduced by one workgroup (no same-address atomics)
+  int64_t* part_build = nullptr;  // 2 * part_blocks
+  int64_t* part_sweep = nullptr;  // 2 * part_blocks
+  int32_t part_blocks = 0;
   sph::Bank bank[2];
   // Verlet-skin lists of the fused step: pairs closer than cutoff + skin on a
   // grid whose cells are >= cutoff + skin, rebuilt only after some particle

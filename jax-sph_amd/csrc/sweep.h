@@ -128,7 +128,8 @@ struct SweepArgs {
   // cutoff, at or above cut_hi outside, in between the exact reference metric
   // decides.  Exact lists pass every pair (cut_lo = cut_hi = +inf).
   float cut_lo, cut_hi, cut_sq;
-  int64_t* info;  // non-null: the density sweep counts pairs into info[1], info[3]
+  int64_t* info;  // non-null: the density sweep counts pairs (per-workgroup partials in part)
+  int64_t* part;  // [2 * gridDim.x] pairs and largest list of each workgroup
   int remap;      // XCD-contiguous block order (grid a multiple of 8)
 };
 
@@ -364,8 +365,8 @@ __global__ __launch_bounds__(kB) void k_density(SweepArgs a) {
     unsigned long long t = 0;
     int m = 0;
     for (int w = 0; w < kB / 64; ++w) { t += btot[w]; m = bmax[w] > m ? bmax[w] : m; }
-    if (t) atomicAdd((unsigned long long*)&a.info[1], t);
-    if (m) atomicMax((unsigned long long*)&a.info[3], (unsigned long long)m);
+    a.part[2 * blockIdx.x] = (int64_t)t;  // reduced by one workgroup afterwards
+    a.part[2 * blockIdx.x + 1] = m;
   }
 }
 
@@ -427,6 +428,7 @@ inline SweepArgs make_args(sph_ws* ws, int bank, const sph_solver_params* prm, c
     a.cut_lo = a.cut_hi = INFINITY;
   }
   a.info = nullptr;
+  a.part = ws->part_sweep;
   a.remap = ws->xcd_remap;
   a.Uf = a.R.U1; a.Vf = a.R.V0; a.Qf = a.R.Q1;
   a.heat_in_density = prm->heat_conduction && !prm->bc_trick;

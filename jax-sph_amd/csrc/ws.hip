@@ -118,6 +118,10 @@ extern "C" int sph_ws_create(const sph_nl_config* cfg, sph_ws** out) {
   A(ws->off64, N + 1);
   A(ws->scan_tmp, 1024);
   A(ws->cnt64, N);
+  // workgroups of the widest launch (npad * tpp lanes of 256, rounded to 8 XCDs)
+  ws->part_blocks = (int32_t)(((npad * ws->tpp + 255) / 256 + 7) / 8 * 8);
+  A(ws->part_build, 2 * (int64_t)ws->part_blocks);
+  A(ws->part_sweep, 2 * (int64_t)ws->part_blocks);
   for (int b = 0; b < 2; ++b) {
     A(ws->bank[b].perm, N);
     A(ws->bank[b].iperm, N);
@@ -226,7 +230,7 @@ extern "C" int sph_ws_destroy(sph_ws* ws) {
   if (!ws) return SPH_OK;
   hipDeviceSynchronize();
   void* ptrs[] = {ws->cell_count, ws->pcell, ws->pslot, ws->unsorted, ws->off64, ws->scan_tmp,
-                  ws->cnt64, ws->P, ws->U0, ws->U1, ws->U2, ws->V0, ws->V2, ws->Q0, ws->Q1,
+                  ws->cnt64, ws->part_build, ws->part_sweep, ws->P, ws->U0, ws->U1, ws->U2, ws->V0, ws->V2, ws->Q0, ws->Q1,
                   ws->Q2, ws->NW, ws->UT, ws->drho, ws->LA, ws->AB0, ws->AB1};
   for (void* p : ptrs)
     if (p) hipFree(p);
