@@ -251,6 +251,14 @@ def main():
     alg["nl_build"] = 12.0 + 4.0 * pairs / N
     per_launch = alg.get(name, float("nan")) * N
     achieved = per_launch / avg_s / 1e9
+    # HBM bytes per launch of that kernel: rocprofv3 FETCH_SIZE / WRITE_SIZE of
+    # this workload, scaled by a plain copy's counter readings (profiles/)
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "hbm_traffic_tgv3d_64cubed_fused_step.json")
+    if n_side == 64 and a.skin == "auto" and os.path.exists(tfile):
+        for k, v in json.load(open(tfile))["kernels"].items():
+            if k == f"k_{name}" or k.startswith(f"k_{name}<"):
+                traffic = v["traffic_bytes"]
     kernels = {k: {"ms_total": round(v[0], 4), "launches": v[1],
                    "us_mean": round(v[0] / v[1] * 1e3, 2)} for k, v in
                sorted(prof.items(), key=lambda kv: -kv[1][0])}
@@ -273,7 +281,7 @@ def main():
                    "verlet_skin": ws.skin, "xcd_remap": a.xcd_remap},
         "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None,
+                     "traffic": traffic,
                      "alg_bytes_per_launch": per_launch, "us_mean": avg_s * 1e6},
         "kernels": kernels,
         "ms_per_step_profiled_pass": el_prof / a.steps * 1e3,
