@@ -66,17 +66,20 @@ def copy_calibration(mb):
     print(json.dumps({"copy_calibration_mib": mb, "launches": 5}))
 
 
-def fwd_adjoint_leg(steps=100, every=10, n_dirs=32, eps=1e-3, seed=0):
+def fwd_adjoint_leg(steps=100, every=10, seed=0):
     """2D Taylor-Green inverse problem: dx = 0.01 (10,000 particles), tvf = 0,
     a rollout of `steps` si_euler steps; loss = MSE of the final velocities
     against the rollout from the analytic initial field; gradient with respect
-    to the initial velocity (perturbed by N(0, 0.05^2)) through the hand-written
-    adjoint with a checkpoint every `every` steps.  Returns the cost ratio
-    t(forward with checkpoints + adjoint) / t(plain forward) and the gradient
-    error against central differences (eps) on `n_dirs` random directions."""
+    to the initial velocity (analytic + N(0, 0.05^2)) through the hand-written
+    adjoint with a checkpoint every `every` steps, both passes replayed as HIP
+    graphs.  Reports t(forward with checkpoints + adjoint) / t(forward), the
+    forward being the same rollout as graph replays of the fused step, and the
+    gradient error against the float64 oracle's central differences committed
+    in tests/golden/c4_fd.json (same u0 and directions)."""
     import torch
     from jax_sph_amd import adjoint, integrator, partition
     from jax_sph_amd import case_setup as cs
+    from jax_sph_amd.graph import StepGraph
     cfg = cs.load_config("tgv", {"case": {"dim": 2, "dx": 0.01}, "solver": {"tvf": 0.0}})
     (cfg, box, state, g_ext_fn, bc_fn, nw_fn, eos, key, disp, shift) = \
         cs.load_case("tgv")(cfg).initialize()
@@ -88,61 +91,65 @@ def fwd_adjoint_leg(steps=100, every=10, n_dirs=32, eps=1e-3, seed=0):
     exact_fn = partition.neighbor_list(disp, box, r_cutoff=cut, mask_self=False,
                                        format=partition.Sparse, skin=0.0)
     adv = integrator.si_euler(0.0, solver.forward_wrapper(), shift, bc_fn, nw_fn)
-
-    def plain(u0, fn=fast_fn):
-        st = dict(state)
-        st["u"], st["v"] = u0, u0
-        nb = fn.allocate(st["r"])
-        for _ in range(steps):
-            st, nb = adv(dt, st, nb)
-        assert not bool(nb.did_buffer_overflow)
-        return st["u"]
-
     n = state["r"].shape[0]
     gen = torch.Generator().manual_seed(seed)
-    u_tgt = plain(state["u"])
     u0 = (state["u"].cpu() + 0.05 * torch.randn(n, 2, generator=gen)).cuda()
 
-    def loss_of(u):
-        return ((u.double() - u_tgt.double()) ** 2).mean()
+    # plain forward: `every` fused steps per graph replay
+    sg = StepGraph(adv, state, fast_fn.allocate(state["r"]), dt, steps_per_replay=every)
+    base = {k: v.clone() for k, v in sg.state.items()}
 
-    plain(u0)  # warm-up
+    def plain(u_init):
+        for k in sg.state:
+            sg.state[k].copy_(base[k])
+        sg.state["u"].copy_(u_init)
+        sg.state["v"].copy_(u_init)
+        sg.replay(steps // every)
+        return sg.state["u"].clone()
+
+    u_tgt = plain(state["u"]).double()
+    plain(u0)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     plain(u0)
     torch.cuda.synchronize()
     t_fwd = time.perf_counter() - t0
-    st = dict(state)
+    assert not sg.did_buffer_overflow
+
+    gr = adjoint.GraphRollout(state, solver, exact_fn, shift, dt, steps, checkpoint_every=every)
+
+    def fwd_adj():
+        uu = u0.clone().requires_grad_(True)
+        u_n, _ = gr(uu, state["r"])
+        loss = ((u_n.double() - u_tgt) ** 2).mean()
+        loss.backward()
+        return loss, uu.grad
+
+    fwd_adj()
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    uu = u0.clone().requires_grad_(True)
-    st["u"] = uu
-    u_n, _, runner = adjoint.rollout(st, solver, exact_fn, shift, dt, steps,
-                                     checkpoint_every=every)
-    loss = loss_of(u_n)
-    loss.backward()
+    loss, grad = fwd_adj()
     torch.cuda.synchronize()
     t_fa = time.perf_counter() - t0
-    g = uu.grad.double()
-    errs, ad_all, fd_all = [], [], []
-    for k in range(n_dirs):
-        d = torch.randn(n, 2, generator=gen).cuda()
-        lp = float(loss_of(plain(u0 + eps * d, exact_fn)))
-        lm = float(loss_of(plain(u0 - eps * d, exact_fn)))
-        fd = (lp - lm) / (2 * eps)
-        ad = float((g * d.double()).sum())
-        ad_all.append(ad)
-        fd_all.append(fd)
-        errs.append(abs(ad - fd) / max(abs(fd), 1e-30))
-    ad_all, fd_all = np.array(ad_all), np.array(fd_all)
-    return {"config": f"2D TGV dx=0.01 ({n} particles), tvf=0, {steps} steps, checkpoint "
-                      f"every {every}, loss = MSE(u_final, target rollout)",
-            "t_forward_s": t_fwd, "t_forward_plus_adjoint_s": t_fa,
-            "cost_ratio": t_fa / t_fwd, "loss": float(loss),
-            "grad_check": {"directions": n_dirs, "eps": eps,
-                           "rel_err_median": float(np.median(errs)),
-                           "rel_err_max": float(np.max(errs)),
-                           "rel_err_vector": float(np.linalg.norm(ad_all - fd_all) /
-                                                   np.linalg.norm(fd_all))}}
+    out = {"config": f"2D TGV dx=0.01 ({n} particles), tvf=0, {steps} steps, checkpoint "
+                     f"every {every}, loss = MSE(u_final, target rollout)",
+           "t_forward_s": t_fwd, "t_forward_plus_adjoint_s": t_fa,
+           "cost_ratio": t_fa / t_fwd, "loss": float(loss)}
+    fx_path = os.path.join(ROOT, "tests", "golden", "c4_fd.json")
+    if os.path.exists(fx_path):
+        fx = json.load(open(fx_path))
+        g = grad.double()
+        ad = np.array([float((g * torch.randn(n, 2, generator=gen).cuda().double()).sum())
+                       for _ in range(len(fx["fd"]))])
+        fd = np.array(fx["fd"])
+        rel = np.abs(ad - fd) / np.abs(fd)
+        out["grad_check"] = {"against": "float64 oracle central differences, eps=1e-3, "
+                                        "tests/golden/c4_fd.json",
+                             "directions": len(fd),
+                             "rel_err_vector": float(np.linalg.norm(ad - fd) / np.linalg.norm(fd)),
+                             "rel_err_median": float(np.median(rel)),
+                             "rel_err_max": float(np.max(rel))}
+    return out
 
 
 def cpu_baseline(n_side, seconds=15.0):

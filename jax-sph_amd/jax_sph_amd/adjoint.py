@@ -159,6 +159,125 @@ class _Rollout(torch.autograd.Function):
         return ub0, rb0, None
 
 
+EVOLVING = ("r", "u", "v", "dudt", "dvdt", "rho", "p", "drhodt")
+
+
+class GraphRollout:
+    """The same differentiable rollout with both passes as HIP graphs.
+
+    Each of the ``k = checkpoint_every`` steps of a segment has a workspace of
+    its own.  The exact pair list that step builds for its new positions
+    therefore survives the segment, and the reverse sweep needs no rebuild.
+
+    * Graph A: one forward segment of ``k`` fused steps, in place.
+    * Graph B: recompute a segment from its checkpoint, then apply ``k`` step
+      transposes to the cotangent buffers.
+
+    A call costs ``n_steps / k`` replays of each graph, plus the checkpoint
+    copies.  ``n_steps`` must be a multiple of ``k``.  The gradients equal
+    :func:`rollout`'s bit for bit.
+    """
+
+    def __init__(self, state: Dict[str, torch.Tensor], solver: WCSPH, neighbor_fn, shift_fn,
+                 dt: float, n_steps: int, tvf: float = 0.0, checkpoint_every: int = 10):
+        _check(solver, tvf)
+        if getattr(getattr(neighbor_fn, "search", None), "skin", 0.0):
+            raise ValueError("the differentiable rollout needs a neighbor_list with skin=0")
+        self.k = int(checkpoint_every)
+        if n_steps % self.k:
+            raise ValueError("n_steps must be a multiple of checkpoint_every")
+        self.n_steps = int(n_steps)
+        self.solver = solver
+        self.dt = float(_f(dt))
+        self.prm = solver.params(tvf_stress=False)
+        self.advance = si_euler(0.0, solver.forward_wrapper(), shift_fn, None, None)
+        base = solver.state_in(state)
+        self.base = {key: base[key].clone() for key in EVOLVING}
+        self.nbs = [neighbor_fn.allocate(base["r"]) for _ in range(self.k)]
+        self.S = {key: v.clone() for key, v in base.items()}   # forward segment state
+        self.C = {key: v.clone() for key, v in base.items()}   # checkpoint being reversed
+        self.bar = [torch.zeros_like(base["u"]) for _ in range(3)]  # r, u, a cotangents
+        self.bits = torch.zeros((), dtype=torch.int64, device=base["u"].device)
+        torch.cuda.synchronize()
+        self.gA = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.gA):
+            st = self.S
+            bits = None
+            for j in range(self.k):
+                st, nb = self.advance(self.dt, st, self.nbs[j])
+                b = nb.info[0]
+                bits = b if bits is None else torch.bitwise_or(bits, b)
+            for key in EVOLVING:
+                self.S[key].copy_(st[key])
+            self.bits.copy_(torch.bitwise_or(self.bits, bits))
+        self.gB = torch.cuda.CUDAGraph()
+        L = _lib.lib()
+        with torch.cuda.graph(self.gB):
+            st, steps = self.C, []
+            for j in range(self.k):
+                st, nb = self.advance(self.dt, st, self.nbs[j])
+                steps.append((st, nb))
+            rb, ub, ab = self.bar
+            for st_j, nb_j in reversed(steps):
+                out = [torch.empty_like(rb) for _ in range(3)]
+                bar_in = _lib.CotangentIn(rb.data_ptr(), ub.data_ptr(), ab.data_ptr())
+                bar_out = _lib.CotangentOut(*(t.data_ptr() for t in out))
+                _lib.check(L.sph_step_vjp(nb_j.workspace.handle, nb_j.bank, ctypes.byref(self.prm),
+                                          self.dt, ctypes.byref(self.solver.c_state_in(st_j)),
+                                          ctypes.byref(bar_in), ctypes.byref(bar_out),
+                                          _lib.stream()), "sph_step_vjp")
+                rb, ub, ab = out
+            for dst, src in zip(self.bar, (rb, ub, ab)):
+                dst.copy_(src)
+
+    def forward(self, u0, r0):
+        """Runs the rollout; returns (u_final, r_final, checkpoints)."""
+        for key in EVOLVING:
+            self.S[key].copy_(self.base[key])
+        self.S["u"].copy_(u0)
+        self.S["v"].copy_(u0)
+        self.S["r"].copy_(r0)
+        self.bits.zero_()
+        ckpts = []
+        for _ in range(self.n_steps // self.k):
+            ckpts.append({key: self.S[key].clone() for key in EVOLVING})
+            self.gA.replay()
+        if int(self.bits) & 3:
+            raise RuntimeError("neighbour-list overflow inside the graph rollout; use "
+                               "adjoint.rollout (re-allocates) or a larger capacity")
+        return self.S["u"].clone(), self.S["r"].clone(), ckpts
+
+    def backward(self, ckpts, ub_final, rb_final):
+        self.bar[0].copy_(rb_final)
+        self.bar[1].copy_(ub_final)
+        self.bar[2].zero_()
+        for ck in reversed(ckpts):
+            for key in EVOLVING:
+                self.C[key].copy_(ck[key])
+            self.gB.replay()
+        return self.bar[0].clone(), self.bar[1].clone()
+
+    def __call__(self, u0: torch.Tensor, r0: torch.Tensor):
+        """Differentiable (u_final, r_final) of the rollout from (u0, r0)."""
+        return _GraphFn.apply(u0, r0, self)
+
+
+class _GraphFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, u0, r0, runner: GraphRollout):
+        u, r, ckpts = runner.forward(_lib.as_f32(u0.detach()), _lib.as_f32(r0.detach()))
+        ctx.runner, ctx.ckpts = runner, ckpts
+        return u, r
+
+    @staticmethod
+    def backward(ctx, gu, gr):
+        runner = ctx.runner
+        ub = gu if gu is not None else torch.zeros_like(runner.S["u"])
+        rb = gr if gr is not None else torch.zeros_like(runner.S["r"])
+        rb0, ub0 = runner.backward(ctx.ckpts, _lib.as_f32(ub), _lib.as_f32(rb))
+        return ub0, rb0, None
+
+
 def rollout(state: Dict[str, torch.Tensor], solver: WCSPH, neighbor_fn, shift_fn, dt: float,
             n_steps: int, tvf: float = 0.0, checkpoint_every: int = 10
             ) -> Tuple[torch.Tensor, torch.Tensor, "_Runner"]:

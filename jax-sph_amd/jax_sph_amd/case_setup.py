@@ -296,7 +296,9 @@ class SimulationSetup:
         return CaseBC(0)
 
     # ----------------------------------------------------------------- main
-    def initialize(self):
+    def initialize(self, device=None):
+        """case_setup.py:44-231.  ``device`` (default: the HIP device) places
+        the state tensors; ``"cpu"`` gives the same values on the host."""
         cfg = self.cfg
         dx, dim = cfg.case.dx, cfg.case.dim
         rho_ref, viscosity, u_ref = cfg.case.rho_ref, cfg.case.viscosity, cfg.case.u_ref
@@ -340,7 +342,7 @@ class SimulationSetup:
                               lambda d, n: self._walls(d, n, dim))
         else:
             nw = np.zeros_like(r)
-        dev = _lib.device()
+        dev = _lib.device() if device is None else torch.device(device)
 
         def T_(x, dtype=torch.float32):
             return torch.as_tensor(np.ascontiguousarray(x), dtype=dtype, device=dev)

@@ -1,0 +1,100 @@
+"""Writes tests/golden/c4_fd.json: float64 central differences of the 2D
+Taylor-Green inverse-problem loss, computed with the oracle (a numpy
+restatement of jax-sph's si_euler + WCSPH.forward + neighbour list).
+
+Problem: dx = 0.01 (10,000 particles), tvf = 0, 100 si_euler steps; loss =
+mean((u_final(u0) - u_target)^2) with u_target the rollout from the analytic
+initial field and u0 = analytic + 0.05 N(0, 1) (torch CPU generator, seed 0,
+first draw); 32 directions = the next 32 draws of the same generator; eps =
+1e-3.  The values pin the gradient of the hand-written adjoint at this size
+(the reference would give it with jax.grad, which cannot run here).
+
+Run from the repository root: ``python tests/golden/make_c4_fd.py`` (about
+40 CPU-minutes, spread over worker processes).
+"""
+import json
+import multiprocessing as mp
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "jax-sph_amd"))
+
+STEPS, EPS, N_DIRS, SEED = 100, 1e-3, 32, 0
+
+
+def setup():
+    import torch
+    from jax_sph_amd import case_setup as cs
+    from tests import gpu_helpers as G
+    cfg = cs.load_config("tgv", {"case": {"dim": 2, "dx": 0.01}, "solver": {"tvf": 0.0}})
+    (cfg, box, state, g, bc, nw, eos, key, disp, shift) = \
+        cs.load_case("tgv")(cfg).initialize("cpu")
+    solver = cs.make_solver(cfg, disp, eos, g)
+    n = state["r"].shape[0]
+    gen = torch.Generator().manual_seed(SEED)
+    u0 = (state["u"] + 0.05 * torch.randn(n, 2, generator=gen)).numpy().astype(np.float32)
+    dirs = [torch.randn(n, 2, generator=gen).numpy().astype(np.float32) for _ in range(N_DIRS)]
+    base = G.to_f64(G.to_np(state))
+    return cfg, box, solver, base, u0, dirs
+
+
+def rollout(u0_64):
+    from oracle import nl_oracle as nlo
+    from oracle import sph_oracle as so
+    from tests import gpu_helpers as G
+    cfg, box, solver, base, _, _ = CTX
+    ns = nlo.NeighborSearch(box, solver._kernel_fn.cutoff, mask_self=False, dtype=np.float64)
+    adv = so.si_euler(0.0, G.oracle_solver64(solver, box).forward, box, lambda s: s, ns.update,
+                      np.float64)
+    st = dict(base)
+    st["u"] = u0_64
+    st["v"] = u0_64.copy()
+    nb = ns.allocate(st["r"])
+    for _ in range(STEPS):
+        st, nb = adv(cfg.solver.dt, st, nb)
+    return st["u"]
+
+
+def job(k):
+    _, _, _, base, u0, dirs = CTX
+    if k < 0:
+        return k, rollout(base["u"])          # target from the analytic field
+    kk, sign = divmod(k, 2)
+    u = u0.astype(np.float64) + (EPS if sign == 0 else -EPS) * dirs[kk].astype(np.float64)
+    return k, rollout(u)
+
+
+CTX = None
+
+
+def init():
+    global CTX
+    CTX = setup()
+
+
+if __name__ == "__main__":
+    init()
+    t0 = time.time()
+    jobs = [-1] + list(range(2 * N_DIRS))
+    with mp.Pool(int(os.environ.get("WORKERS", "7")), initializer=init) as pool:
+        res = dict(pool.map(job, jobs, chunksize=1))
+    target = res[-1]
+    fd = []
+    for k in range(N_DIRS):
+        lp = float(np.mean((res[2 * k] - target) ** 2))
+        lm = float(np.mean((res[2 * k + 1] - target) ** 2))
+        fd.append((lp - lm) / (2 * EPS))
+    out = {"problem": "2D TGV dx=0.01 (10000 particles), tvf=0, QSK, Tait c0=10, nu=0.01, "
+                      f"{STEPS} si_euler steps, dt={CTX[0].solver.dt!r}",
+           "loss": "mean((u_final - u_target)^2) over all particles and components",
+           "u0": "analytic + 0.05 * torch.randn(n, 2, generator=torch.Generator().manual_seed(0))",
+           "directions": f"next {N_DIRS} torch.randn(n, 2) draws of the same generator",
+           "eps": EPS, "steps": STEPS, "seed": SEED, "dtype": "float64 oracle",
+           "fd": fd, "seconds": time.time() - t0}
+    json.dump(out, open(os.path.join(ROOT, "tests", "golden", "c4_fd.json"), "w"), indent=1)
+    print(json.dumps({k: v for k, v in out.items() if k != "fd"}), fd[:4])

@@ -105,3 +105,65 @@ def test_unsupported_configurations_raise():
                                       format=partition.Sparse)
     with pytest.raises(ValueError):
         adjoint.rollout(st, solver, skin_fn, shift, cfg.solver.dt, 2)
+
+
+def test_c4_gradient_matches_float64_finite_differences():
+    """2D TGV inverse problem at full size (10,000 particles, 100 steps,
+    checkpoint every 10): the adjoint gradient of the velocity MSE against 32
+    central differences (eps = 1e-3) of the float64 oracle rollout, committed
+    in tests/golden/c4_fd.json (made by tests/golden/make_c4_fd.py)."""
+    import json
+    import os
+    from jax_sph_amd import case_setup as cs
+    from jax_sph_amd import integrator
+    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "c4_fd.json")))
+    cfg = cs.load_config("tgv", {"case": {"dim": 2, "dx": 0.01}, "solver": {"tvf": 0.0}})
+    (cfg, box, state, g, bc, nw, eos, key, disp, shift) = cs.load_case("tgv")(cfg).initialize()
+    solver = cs.make_solver(cfg, disp, eos, g)
+    assert repr(cfg.solver.dt) in fx["problem"]
+    fn = partition.neighbor_list(disp, box, r_cutoff=solver._kernel_fn.cutoff, mask_self=False,
+                                 format=partition.Sparse, skin=0.0)
+    adv = integrator.si_euler(0.0, solver.forward_wrapper(), shift, bc, None)
+    n = state["r"].shape[0]
+    st = dict(state)
+    nb = fn.allocate(st["r"])
+    for _ in range(fx["steps"]):
+        st, nb = adv(cfg.solver.dt, st, nb)
+    target = st["u"].double()
+    gen = torch.Generator().manual_seed(fx["seed"])
+    u0 = (state["u"].cpu() + 0.05 * torch.randn(n, 2, generator=gen)).cuda()
+    dirs = [torch.randn(n, 2, generator=gen).cuda() for _ in range(len(fx["fd"]))]
+    uu = u0.clone().requires_grad_(True)
+    s = dict(state)
+    s["u"] = uu
+    u, _, _ = adjoint.rollout(s, solver, fn, shift, cfg.solver.dt, fx["steps"],
+                              checkpoint_every=10)
+    ((u.double() - target) ** 2).mean().backward()
+    gd = uu.grad.double()
+    ad = np.array([float((gd * d.double()).sum()) for d in dirs])
+    fd = np.array(fx["fd"])
+    vec = np.linalg.norm(ad - fd) / np.linalg.norm(fd)
+    assert vec <= 1e-3, (vec, ad[:4], fd[:4])
+
+
+def test_graph_rollout_equals_eager_rollout():
+    """GraphRollout (both passes as HIP graphs, one workspace per step of a
+    segment) gives the eager rollout's values and gradients bit for bit, also
+    when called twice."""
+    cfg, box, st, solver, fn, shift, gen = _setup()
+    n = st["r"].shape[0]
+    w = torch.randn(n, 2, generator=gen).cuda()
+    u0 = st["u"].clone().requires_grad_(True)
+    s = dict(st)
+    s["u"] = u0
+    u, r, _ = adjoint.rollout(s, solver, fn, shift, cfg.solver.dt, 8, checkpoint_every=4)
+    ((w * u).sum() + (w * r).sum()).backward()
+    g_eager = u0.grad.clone()
+    gr = adjoint.GraphRollout(st, solver, fn, shift, cfg.solver.dt, 8, checkpoint_every=4)
+    for _ in range(2):
+        uu = st["u"].clone().requires_grad_(True)
+        ug, rg = gr(uu, st["r"])
+        ((w * ug).sum() + (w * rg).sum()).backward()
+        torch.cuda.synchronize()
+        assert torch.equal(ug, u) and torch.equal(rg, r)
+        assert torch.equal(uu.grad, g_eager)
