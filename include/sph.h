@@ -201,10 +201,12 @@ typedef struct {
  * density summation, no walls/heat/artificial viscosity, tvf = 0, QSK or WC2K.
  * `next` is state n+1 (r, u, rho, p, mass, eta, tag) and `bank` must hold the
  * pairs of next->r; given the cotangents of state n+1 writes those of state n
- * (bar may alias nothing in bar_next). */
+ * (bar may alias nothing in bar_next).  records_ready != 0 promises that the
+ * last kernels on this workspace were the sph_step that produced `next` into
+ * `bank` (its sorted records are reused instead of gathered again). */
 int sph_step_vjp(sph_ws *ws, int bank, const sph_solver_params *prm, float dt,
                  const sph_state_in *next, const sph_cotangent_in *bar_next,
-                 sph_cotangent_out *bar, hipStream_t stream);
+                 sph_cotangent_out *bar, int32_t records_ready, hipStream_t stream);
 
 /* si_euler kick/drift alone (integrator.py:26-30), for callers that compose
  * the step from separate update/forward calls: u' = u + dt dudt,

@@ -149,10 +149,11 @@ __global__ __launch_bounds__(kAB) void k_adj_density(AdjArgs aa, const float4* _
 }
 
 template <int DIM, int KIND>
-int launch_adjoint(sph_ws* ws, AdjArgs& a, hipStream_t st) {
+int launch_adjoint(sph_ws* ws, AdjArgs& a, bool records_ready, hipStream_t st) {
   const int n = a.s.n;
   dim3 grid(xcd_grid(cdivl((int64_t)n * tpp_of(DIM), kAB), a.s.remap));
-  SPH_LAUNCH(ws, "density", st, (k_density<DIM, KIND, 0><<<grid, kAB, 0, st>>>(a.s)));
+  if (!records_ready)
+    SPH_LAUNCH(ws, "density", st, (k_density<DIM, KIND, 0><<<grid, kAB, 0, st>>>(a.s)));
   SPH_LAUNCH(ws, "adj_gather", st,
              k_adj_gather<DIM><<<(unsigned)cdivl(n, kAB), kAB, 0, st>>>(a));
   SPH_LAUNCH(ws, "adj_forces", st,
@@ -168,7 +169,7 @@ using namespace sph;
 
 extern "C" int sph_step_vjp(sph_ws* ws, int bank, const sph_solver_params* prm, float dt,
                             const sph_state_in* next, const sph_cotangent_in* bar_next,
-                            sph_cotangent_out* bar, hipStream_t st) {
+                            sph_cotangent_out* bar, int32_t records_ready, hipStream_t st) {
   if (!prm || !next || !bar_next || !bar) { set_error("sph_step_vjp: null argument"); return SPH_ERR_ARG; }
   int rc = check_bank(ws, bank, prm, "sph_step_vjp");
   if (rc) return rc;
@@ -201,18 +202,21 @@ extern "C" int sph_step_vjp(sph_ws* ws, int bank, const sph_solver_params* prm, 
   a.la = ws->LA;
   a.dt = dt;
   const int n = a.s.n;
+  const bool ready = records_ready != 0;
   dim3 g1((unsigned)cdivl(n, kAB));
+  if (!ready) {
+    if (prm->dim == 2)
+      SPH_LAUNCH(ws, "gather_state", st,
+                 k_gather_state<2><<<g1, kAB, 0, st>>>(a.s, in.r, in.u, in.v, nullptr, nullptr,
+                                                       nullptr));
+    else
+      SPH_LAUNCH(ws, "gather_state", st,
+                 k_gather_state<3><<<g1, kAB, 0, st>>>(a.s, in.r, in.u, in.v, nullptr, nullptr,
+                                                       nullptr));
+  }
   if (prm->dim == 2)
-    SPH_LAUNCH(ws, "gather_state", st,
-               k_gather_state<2><<<g1, kAB, 0, st>>>(a.s, in.r, in.u, in.v, nullptr, nullptr,
-                                                     nullptr));
-  else
-    SPH_LAUNCH(ws, "gather_state", st,
-               k_gather_state<3><<<g1, kAB, 0, st>>>(a.s, in.r, in.u, in.v, nullptr, nullptr,
-                                                     nullptr));
-  if (prm->dim == 2)
-    return prm->kernel == SPH_KERNEL_QSK ? launch_adjoint<2, SPH_KERNEL_QSK>(ws, a, st)
-                                         : launch_adjoint<2, SPH_KERNEL_WC2K>(ws, a, st);
-  return prm->kernel == SPH_KERNEL_QSK ? launch_adjoint<3, SPH_KERNEL_QSK>(ws, a, st)
-                                       : launch_adjoint<3, SPH_KERNEL_WC2K>(ws, a, st);
+    return prm->kernel == SPH_KERNEL_QSK ? launch_adjoint<2, SPH_KERNEL_QSK>(ws, a, ready, st)
+                                         : launch_adjoint<2, SPH_KERNEL_WC2K>(ws, a, ready, st);
+  return prm->kernel == SPH_KERNEL_QSK ? launch_adjoint<3, SPH_KERNEL_QSK>(ws, a, ready, st)
+                                       : launch_adjoint<3, SPH_KERNEL_WC2K>(ws, a, ready, st);
 }

@@ -116,7 +116,7 @@ def lib():
         "sph_integrate": (c_int, [c_int32, c_int64, vp, vp, vp, vp, vp, vp, vp, c_float, c_float,
                                   POINTER(c_float), vp]),
         "sph_step_vjp": (c_int, [vp, c_int, POINTER(SolverParams), c_float, POINTER(StateIn),
-                                 POINTER(CotangentIn), POINTER(CotangentOut), vp]),
+                                 POINTER(CotangentIn), POINTER(CotangentOut), c_int32, vp]),
         "sph_prof_enable": (c_int, [vp, c_int]),
         "sph_prof_reset": (c_int, [vp]),
         "sph_prof_read": (c_int, [vp, c_char_p, c_int, POINTER(c_double), i64p, c_int]),

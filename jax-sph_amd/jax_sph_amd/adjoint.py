@@ -134,7 +134,7 @@ class _Runner:
                 bar_out = _lib.CotangentOut(rb2.data_ptr(), ub2.data_ptr(), ab2.data_ptr())
                 _lib.check(L.sph_step_vjp(ws.handle, nbn.bank, ctypes.byref(self.prm), self.dt,
                                           ctypes.byref(cin), ctypes.byref(bar_in),
-                                          ctypes.byref(bar_out), _lib.stream()),
+                                          ctypes.byref(bar_out), 0, _lib.stream()),
                            "sph_step_vjp")
                 bufs = [rb, ub, ab]
                 rb, ub, ab = rb2, ub2, ab2
@@ -160,6 +160,10 @@ class _Rollout(torch.autograd.Function):
 
 
 EVOLVING = ("r", "u", "v", "dudt", "dvdt", "rho", "p", "drhodt")
+# what one step of the differentiated configuration reads from the state it
+# starts from: positions, velocity and acceleration (v = u + 0 * dvdt; the
+# fluid density and pressure are recomputed by the summation and the EOS)
+CARRIED = ("r", "u", "dudt")
 
 
 class GraphRollout:
@@ -168,18 +172,24 @@ class GraphRollout:
     Each of the ``k = checkpoint_every`` steps of a segment has a workspace of
     its own.  The exact pair list that step builds for its new positions
     therefore survives the segment, and the reverse sweep needs no rebuild.
+    ``workspace_sets`` such sets rotate over the segments.  Graphs:
 
-    * Graph A: one forward segment of ``k`` fused steps, in place.
-    * Graph B: recompute a segment from its checkpoint, then apply ``k`` step
-      transposes to the cotangent buffers.
+    * A: one forward segment of ``k`` fused steps, in place;
+    * R[s]: recompute a segment from its checkpoint into set s, keeping each
+      step's state;
+    * V[s]: the ``k`` step transposes over set s, updating the cotangent
+      buffers in place.
 
-    A call costs ``n_steps / k`` replays of each graph, plus the checkpoint
-    copies.  ``n_steps`` must be a multiple of ``k``.  The gradients equal
-    :func:`rollout`'s bit for bit.
+    A recompute needs only its checkpoint.  The backward pass therefore runs
+    the recomputes of later-reversed segments on ``side_streams`` streams
+    while the calling stream applies V, segment after segment.  A set is
+    reused only after its V has run.  ``n_steps`` must be a multiple of
+    ``k``.  The gradients equal :func:`rollout`'s bit for bit.
     """
 
     def __init__(self, state: Dict[str, torch.Tensor], solver: WCSPH, neighbor_fn, shift_fn,
-                 dt: float, n_steps: int, tvf: float = 0.0, checkpoint_every: int = 10):
+                 dt: float, n_steps: int, tvf: float = 0.0, checkpoint_every: int = 10,
+                 workspace_sets: Optional[int] = None, side_streams: int = 3):
         _check(solver, tvf)
         if getattr(getattr(neighbor_fn, "search", None), "skin", 0.0):
             raise ValueError("the differentiable rollout needs a neighbor_list with skin=0")
@@ -192,69 +202,138 @@ class GraphRollout:
         self.prm = solver.params(tvf_stress=False)
         self.advance = si_euler(0.0, solver.forward_wrapper(), shift_fn, None, None)
         base = solver.state_in(state)
-        self.base = {key: base[key].clone() for key in EVOLVING}
-        self.nbs = [neighbor_fn.allocate(base["r"]) for _ in range(self.k)]
+        n_seg = self.n_steps // self.k
+        if workspace_sets is None:   # concurrency pays at small sizes; memory rules large ones
+            workspace_sets = 4 if base["r"].shape[0] <= 100_000 else 2
+        self.n_sets = max(1, min(int(workspace_sets), n_seg))
+        self.sets = [[neighbor_fn.allocate(base["r"]) for _ in range(self.k)]
+                     for _ in range(self.n_sets)]
+        # carried fields live in one (3, N, d) buffer: a checkpoint is one copy
+        self.base_buf = torch.stack([base[key] for key in CARRIED]).contiguous()
+        self.S_buf = self.base_buf.clone()
         self.S = {key: v.clone() for key, v in base.items()}   # forward segment state
-        self.C = {key: v.clone() for key, v in base.items()}   # checkpoint being reversed
+        for j, key in enumerate(CARRIED):
+            self.S[key] = self.S_buf[j]
+        self.C_bufs, self.C = [], []
+        for _ in range(self.n_sets):
+            cb = self.base_buf.clone()
+            c = {key: v.clone() for key, v in base.items()}
+            for j, key in enumerate(CARRIED):
+                c[key] = cb[j]
+            self.C_bufs.append(cb)
+            self.C.append(c)
         self.bar = [torch.zeros_like(base["u"]) for _ in range(3)]  # r, u, a cotangents
         self.bits = torch.zeros((), dtype=torch.int64, device=base["u"].device)
+        self.sides = [torch.cuda.Stream() for _ in range(max(1, int(side_streams)))]
+        self.generation = 0
         torch.cuda.synchronize()
         self.gA = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self.gA):
             st = self.S
             bits = None
             for j in range(self.k):
-                st, nb = self.advance(self.dt, st, self.nbs[j])
+                st, nb = self.advance(self.dt, st, self.sets[0][j])
                 b = nb.info[0]
                 bits = b if bits is None else torch.bitwise_or(bits, b)
             for key in EVOLVING:
                 self.S[key].copy_(st[key])
             self.bits.copy_(torch.bitwise_or(self.bits, bits))
-        self.gB = torch.cuda.CUDAGraph()
         L = _lib.lib()
-        with torch.cuda.graph(self.gB):
-            st, steps = self.C, []
-            for j in range(self.k):
-                st, nb = self.advance(self.dt, st, self.nbs[j])
-                steps.append((st, nb))
-            rb, ub, ab = self.bar
-            for st_j, nb_j in reversed(steps):
-                out = [torch.empty_like(rb) for _ in range(3)]
-                bar_in = _lib.CotangentIn(rb.data_ptr(), ub.data_ptr(), ab.data_ptr())
-                bar_out = _lib.CotangentOut(*(t.data_ptr() for t in out))
-                _lib.check(L.sph_step_vjp(nb_j.workspace.handle, nb_j.bank, ctypes.byref(self.prm),
-                                          self.dt, ctypes.byref(self.solver.c_state_in(st_j)),
-                                          ctypes.byref(bar_in), ctypes.byref(bar_out),
-                                          _lib.stream()), "sph_step_vjp")
-                rb, ub, ab = out
-            for dst, src in zip(self.bar, (rb, ub, ab)):
-                dst.copy_(src)
+        self.gR, self.gV, self._steps = [], [], []
+        for s in range(self.n_sets):
+            gR = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(gR):
+                st, steps = self.C[s], []
+                bits = None
+                for j in range(self.k):
+                    st, nb = self.advance(self.dt, st, self.sets[s][j])
+                    steps.append((st, nb))
+                    b = nb.info[0]
+                    bits = b if bits is None else torch.bitwise_or(bits, b)
+                self.bits.copy_(torch.bitwise_or(self.bits, bits))
+            gV = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(gV):
+                rb, ub, ab = self.bar
+                for st_j, nb_j in reversed(steps):
+                    out = [torch.empty_like(rb) for _ in range(3)]
+                    bar_in = _lib.CotangentIn(rb.data_ptr(), ub.data_ptr(), ab.data_ptr())
+                    bar_out = _lib.CotangentOut(*(t.data_ptr() for t in out))
+                    _lib.check(L.sph_step_vjp(nb_j.workspace.handle, nb_j.bank,
+                                              ctypes.byref(self.prm), self.dt,
+                                              ctypes.byref(self.solver.c_state_in(st_j)),
+                                              ctypes.byref(bar_in), ctypes.byref(bar_out),
+                                              1,  # records left by this step's own R pass
+                                              _lib.stream()), "sph_step_vjp")
+                    rb, ub, ab = out
+                for dst, src in zip(self.bar, (rb, ub, ab)):
+                    dst.copy_(src)
+            self.gR.append(gR)
+            self.gV.append(gV)
+            self._steps.append(steps)   # keeps the recomputed states alive
 
     def forward(self, u0, r0):
         """Runs the rollout; returns (u_final, r_final, checkpoints)."""
-        for key in EVOLVING:
-            self.S[key].copy_(self.base[key])
+        self.S_buf.copy_(self.base_buf)
         self.S["u"].copy_(u0)
         self.S["v"].copy_(u0)
         self.S["r"].copy_(r0)
         self.bits.zero_()
         ckpts = []
-        for _ in range(self.n_steps // self.k):
-            ckpts.append({key: self.S[key].clone() for key in EVOLVING})
+        n = self.n_steps // self.k
+        for _ in range(n - 1):
+            ckpts.append(self.S_buf.clone())
             self.gA.replay()
+        # the last segment runs as a recompute graph, so its states and pair
+        # lists are already in place when the backward pass starts
+        ckpts.append(self.S_buf.clone())
+        s = (n - 1) % self.n_sets
+        self.C_bufs[s].copy_(self.S_buf)
+        self.gR[s].replay()
         if int(self.bits) & 3:
             raise RuntimeError("neighbour-list overflow inside the graph rollout; use "
                                "adjoint.rollout (re-allocates) or a larger capacity")
-        return self.S["u"].clone(), self.S["r"].clone(), ckpts
+        self.generation += 1
+        last = self._steps[s][-1][0]
+        return last["u"].clone(), last["r"].clone(), ckpts, self.generation
 
-    def backward(self, ckpts, ub_final, rb_final):
+    def backward(self, ckpts, ub_final, rb_final, generation=None):
+        """Cotangents of (r0, u0).  When no other forward ran since the one
+        that made `ckpts` (same generation), its last segment is still in
+        place and is not recomputed."""
+        main = torch.cuda.current_stream()
         self.bar[0].copy_(rb_final)
         self.bar[1].copy_(ub_final)
         self.bar[2].zero_()
-        for ck in reversed(ckpts):
-            for key in EVOLVING:
-                self.C[key].copy_(ck[key])
-            self.gB.replay()
+        n = len(ckpts)
+        ev_r = [torch.cuda.Event() for _ in range(n)]
+        ev_v = [torch.cuda.Event() for _ in range(n)]
+        for side in self.sides:
+            side.wait_stream(main)
+        ns = self.n_sets
+
+        def recompute(seg):
+            s = seg % ns
+            side = self.sides[seg % len(self.sides)]
+            with torch.cuda.stream(side):
+                if seg + ns < n:
+                    side.wait_event(ev_v[seg + ns])   # set s was last read by V(seg + ns)
+                self.C_bufs[s].copy_(ckpts[seg])
+                self.gR[s].replay()
+                ev_r[seg].record(side)
+            ckpts[seg].record_stream(side)
+
+        if generation == self.generation:
+            ev_r[n - 1].record(main)    # the forward left segment n-1 in its set
+        else:
+            recompute(n - 1)
+        for seg in range(n - 2, max(-1, n - 1 - ns), -1):
+            recompute(seg)              # sets not yet used by any V
+        for seg in reversed(range(n)):
+            main.wait_event(ev_r[seg])
+            self.gV[seg % ns].replay()
+            ev_v[seg].record(main)
+            if seg - ns >= 0:
+                recompute(seg - ns)     # reuses the set V(seg) just released
         return self.bar[0].clone(), self.bar[1].clone()
 
     def __call__(self, u0: torch.Tensor, r0: torch.Tensor):
@@ -265,8 +344,8 @@ class GraphRollout:
 class _GraphFn(torch.autograd.Function):
     @staticmethod
     def forward(ctx, u0, r0, runner: GraphRollout):
-        u, r, ckpts = runner.forward(_lib.as_f32(u0.detach()), _lib.as_f32(r0.detach()))
-        ctx.runner, ctx.ckpts = runner, ckpts
+        u, r, ckpts, gen = runner.forward(_lib.as_f32(u0.detach()), _lib.as_f32(r0.detach()))
+        ctx.runner, ctx.ckpts, ctx.generation = runner, ckpts, gen
         return u, r
 
     @staticmethod
@@ -274,7 +353,7 @@ class _GraphFn(torch.autograd.Function):
         runner = ctx.runner
         ub = gu if gu is not None else torch.zeros_like(runner.S["u"])
         rb = gr if gr is not None else torch.zeros_like(runner.S["r"])
-        rb0, ub0 = runner.backward(ctx.ckpts, _lib.as_f32(ub), _lib.as_f32(rb))
+        rb0, ub0 = runner.backward(ctx.ckpts, _lib.as_f32(ub), _lib.as_f32(rb), ctx.generation)
         return ub0, rb0, None
 
 

