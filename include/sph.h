@@ -73,7 +73,7 @@ typedef struct {
 typedef struct {
   int32_t dim;
   int32_t kernel;            /* enum sph_kernel_kind */
-  int32_t solver;            /* 0 = SPH, 1 = RIE */
+  int32_t solver;            /* 0 = SPH, 1 = RIE, 2 = DELTA (solver.py:684-694) */
   int32_t density_evolution; /* 0 = summation, 1 = continuity */
   int32_t bc_trick;
   int32_t free_slip;
@@ -85,6 +85,10 @@ typedef struct {
   float box[3];
   float g_const[3];          /* body force used when g_ext == NULL */
   int32_t tvf_stress;        /* 1: evaluate the A = rho u (v-u) term; 0: caller proves v == u */
+  int32_t rho_renorm;        /* density renormalisation after continuity (solver.py:167-173) */
+  float diff_delta;          /* DELTA: density diffusion coefficient (solver.py:693) */
+  float diff_alpha;          /* DELTA: velocity diffusion coefficient (solver.py:687) */
+  float support;             /* DELTA: h_fac * dx (solver.py:688, 693) */
   int32_t pad_;
 } sph_solver_params;
 

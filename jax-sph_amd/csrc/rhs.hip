@@ -30,6 +30,147 @@ int skin_begin(sph_ws* ws, const float* r, const float* u, const float* dudt, co
 int skin_build(sph_ws* ws, int64_t* info, hipStream_t st);
 int skin_finish(sph_ws* ws, int64_t max_occupancy, int64_t* info, hipStream_t st);
 
+// ------------------------------------------------------------- delta-SPH
+// Renormalisation matrix of solver.py:42-58: M = sum_j (-r_ij) (x) grad W_ij V_j
+// over the pairs of i, L = inv(M).  The reference's second matrix (grouped by
+// the other index, solver.py:57) is the termwise negation of M, so its
+// inverse is -L: k_delta_grad uses -L of the neighbour.
+template <int DIM>
+__device__ __forceinline__ void store_mat(const SweepArgs& a, int s, const float* m) {
+  if (DIM == 2) {
+    a.R.DL0[s] = make_float4(m[0], m[1], m[2], m[3]);
+  } else {
+    a.R.DL0[s] = make_float4(m[0], m[1], m[2], m[3]);
+    a.R.DL1[s] = make_float4(m[4], m[5], m[6], m[7]);
+    a.R.DL2[s] = make_float4(m[8], 0.f, 0.f, 0.f);
+  }
+}
+
+template <int DIM>
+__device__ __forceinline__ V3 mat_vec(const SweepArgs& a, int s, V3 x) {
+  const float4 m0 = a.R.DL0[s];
+  if (DIM == 2) return V3{m0.x * x.x + m0.y * x.y, m0.z * x.x + m0.w * x.y, 0.f};
+  const float4 m1 = a.R.DL1[s], m2 = a.R.DL2[s];
+  return V3{m0.x * x.x + m0.y * x.y + m0.z * x.z,  // row 0: m0.x m0.y m0.z
+            m0.w * x.x + m1.x * x.y + m1.y * x.z,  // row 1: m0.w m1.x m1.y
+            m1.z * x.x + m1.w * x.y + m2.x * x.z};  // row 2: m1.z m1.w m2.x
+}
+
+template <int DIM, int KIND>
+__global__ __launch_bounds__(kB) void k_delta_L(SweepArgs a) {
+  constexpr int TPP = tpp_of(DIM);
+  const int tid = logical_block(a.remap) * blockDim.x + threadIdx.x;
+  const int s = tid / TPP, h = tid % TPP;
+  if (s >= a.n) return;
+  const sph_solver_params& p = a.prm;
+  const KernelEval<KIND, DIM> K{p.kernel, p.sigma, p.one_over_h};
+  const float4 Pi = a.R.P[s];
+  float M[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  for_each_pair(a, s, h, [&](const int j) {
+    const float4 Pj = a.R.P[j];
+    const V3 dr = min_image<DIM>(Pi, Pj, a.geo);
+    const float d2 = dist2<DIM>(dr.x, dr.y, dr.z);
+    if (!in_cutoff<DIM>(a, d2, Pi, Pj)) return;
+    const float d = sqrtf(d2);
+    const float vj = Pj.w / a.R.U0[j].w;
+    const V3 kg = (K.dw(d * p.one_over_h) / (d + kEps) * vj) * dr;  // grad W_ij V_j
+    const float r[3] = {-dr.x, -dr.y, -dr.z}, g[3] = {kg.x, kg.y, kg.z};
+#pragma unroll
+    for (int x = 0; x < DIM; ++x)
+#pragma unroll
+      for (int y = 0; y < DIM; ++y) M[x * DIM + y] += r[x] * g[y];
+  });
+#pragma unroll
+  for (int k = 0; k < DIM * DIM; ++k) M[k] = lane_sum<TPP>(M[k]);
+  if (h != 0) return;
+  float L[9];
+  if (DIM == 2) {
+    const float det = M[0] * M[3] - M[1] * M[2];
+    const float id = 1.0f / det;
+    L[0] = M[3] * id; L[1] = -M[1] * id; L[2] = -M[2] * id; L[3] = M[0] * id;
+  } else {
+    const float c00 = M[4] * M[8] - M[5] * M[7], c01 = M[5] * M[6] - M[3] * M[8],
+                c02 = M[3] * M[7] - M[4] * M[6];
+    const float det = M[0] * c00 + M[1] * c01 + M[2] * c02;
+    const float id = 1.0f / det;
+    L[0] = c00 * id;
+    L[1] = (M[2] * M[7] - M[1] * M[8]) * id;
+    L[2] = (M[1] * M[5] - M[2] * M[4]) * id;
+    L[3] = c01 * id;
+    L[4] = (M[0] * M[8] - M[2] * M[6]) * id;
+    L[5] = (M[2] * M[3] - M[0] * M[5]) * id;
+    L[6] = c02 * id;
+    L[7] = (M[1] * M[6] - M[0] * M[7]) * id;
+    L[8] = (M[0] * M[4] - M[1] * M[3]) * id;
+  }
+  store_mat<DIM>(a, s, L);
+}
+
+// Density-gradient terms of solver.py:60-66:
+//   grad_i = L_i sum_j (rho_j - rho_i) grad W_ij V_j
+//   grad_j = sum_j (rho_i - rho_j) (-L_j) (-grad W_ij V_i)
+template <int DIM, int KIND>
+__global__ __launch_bounds__(kB) void k_delta_grad(SweepArgs a) {
+  constexpr int TPP = tpp_of(DIM);
+  const int tid = logical_block(a.remap) * blockDim.x + threadIdx.x;
+  const int s = tid / TPP, h = tid % TPP;
+  if (s >= a.n) return;
+  const sph_solver_params& p = a.prm;
+  const KernelEval<KIND, DIM> K{p.kernel, p.sigma, p.one_over_h};
+  const float4 Pi = a.R.P[s];
+  const float rhoi = a.R.U0[s].w;
+  const float vi = Pi.w / rhoi;
+  V3 si = {0.f, 0.f, 0.f}, gj = {0.f, 0.f, 0.f};
+  for_each_pair(a, s, h, [&](const int j) {
+    const float4 Pj = a.R.P[j];
+    const V3 dr = min_image<DIM>(Pi, Pj, a.geo);
+    const float d2 = dist2<DIM>(dr.x, dr.y, dr.z);
+    if (!in_cutoff<DIM>(a, d2, Pi, Pj)) return;
+    const float d = sqrtf(d2);
+    const float rhoj = a.R.U0[j].w;
+    const V3 kg = (K.dw(d * p.one_over_h) / (d + kEps)) * dr;
+    si = si + ((rhoj - rhoi) * (Pj.w / rhoj)) * kg;
+    gj = gj + ((rhoi - rhoj) * vi) * mat_vec<DIM>(a, j, kg);  // (-L_j)(-grad W V_i)
+  });
+  si = lane_sum<TPP>(si);
+  gj = lane_sum<TPP>(gj);
+  if (h != 0) return;
+  a.R.DG0[s] = f4(mat_vec<DIM>(a, s, si), 0.0f);
+  a.R.DG1[s] = f4(gj, 0.0f);
+}
+
+// ------------------------------------------------------------- renormalisation
+// solver.py:167-173 after a continuity update: rho = sum m_j W / min(sum (m/rho)_j W, 1)
+// over all particles, then the EOS; writes the U3/Q3 records.
+template <int DIM, int KIND>
+__global__ __launch_bounds__(kB) void k_renorm(SweepArgs a) {
+  constexpr int TPP = tpp_of(DIM);
+  const int tid = logical_block(a.remap) * blockDim.x + threadIdx.x;
+  const int s = tid / TPP, h = tid % TPP;
+  if (s >= a.n) return;
+  const sph_solver_params& p = a.prm;
+  const KernelEval<KIND, DIM> K{p.kernel, p.sigma, p.one_over_h};
+  const Eos eos = make_eos(p);
+  const float4 Pi = a.R.P[s];
+  float nom = 0.f, den = 0.f;
+  for_each_pair(a, s, h, [&](const int j) {
+    const float4 Pj = a.R.P[j];
+    const V3 dr = min_image<DIM>(Pi, Pj, a.geo);
+    const float d2 = dist2<DIM>(dr.x, dr.y, dr.z);
+    if (!in_cutoff<DIM>(a, d2, Pi, Pj)) return;
+    const float w = K.w(sqrtf(d2) * p.one_over_h);
+    nom += Pj.w * w;
+    den += (Pj.w / a.R.U1[j].w) * w;
+  });
+  nom = lane_sum<TPP>(nom);
+  den = lane_sum<TPP>(den);
+  if (h != 0) return;
+  const float rho = nom / (den > 1.0f ? 1.0f : den);
+  const float4 Ui = a.R.U1[s], Qi = a.R.Q1[s];
+  a.R.U3[s] = make_float4(Ui.x, Ui.y, Ui.z, rho);
+  a.R.Q3[s] = make_float4(eos.p_fn(rho), Qi.y, Qi.z, Qi.w);
+}
+
 // ------------------------------------------------------------- walls
 // MODE 0: generalized wall BC of the SPH solver (solver.py:431-530)
 // MODE 1: heat BC of the Riemann solver (solver.py:554-565)
@@ -43,7 +184,7 @@ __global__ __launch_bounds__(kB) void k_walls(SweepArgs a) {
   const sph_solver_params& p = a.prm;
   const KernelEval<KIND, DIM> K{p.kernel, p.sigma, p.one_over_h};
   const Eos eos = make_eos(p);
-  const float4 Pi = a.R.P[s], Ui = a.R.U1[s], Vi = a.R.V0[s], Qi = a.R.Q1[s];
+  const float4 Pi = a.R.P[s], Ui = a.Ud[s], Vi = a.R.V0[s], Qi = a.Qd[s];
   const int tag_i = (int)Qi.z;
   const int i = a.perm[s];
   V3 u = v3(Ui), v = v3(Vi);
@@ -52,7 +193,7 @@ __global__ __launch_bounds__(kB) void k_walls(SweepArgs a) {
     float ws = 0.f, sp = 0.f, sT = 0.f;
     V3 su = {0.f, 0.f, 0.f}, sv = {0.f, 0.f, 0.f}, srr = {0.f, 0.f, 0.f};
     for_each_pair(a, s, h, [&](const int j) {
-      const float4 Qj = a.R.Q1[j];
+      const float4 Qj = a.Qd[j];
       if ((int)Qj.z != kFluid) return;
       const float4 Pj = a.R.P[j];
       const V3 dr = min_image<DIM>(Pi, Pj, a.geo);
@@ -62,7 +203,7 @@ __global__ __launch_bounds__(kB) void k_walls(SweepArgs a) {
       const float w = K.w(d * p.one_over_h);
       ws += w;
       if (MODE == 0) {
-        const float4 Uj = a.R.U1[j], Vj = a.R.V0[j];
+        const float4 Uj = a.Ud[j], Vj = a.R.V0[j];
         su = su + w * v3(Uj);
         sv = sv + w * v3(Vj);
         sp += w * Qj.x;
@@ -132,6 +273,7 @@ __global__ __launch_bounds__(kB) void k_force(SweepArgs a) {
   const float kappa_i = Vi.w;
   const float Cp_i = heat ? a.R.NW[s].w : 1.0f;
   const float av_pref = p.artificial_alpha * p.av_h * 10.0f;  // alpha * h_ab * c_ab
+  const float delta_pref = p.diff_alpha * p.support * p.c_ref * p.rho_ref;  // alpha h c rho_0
   const float av_h2 = 0.01f * p.av_h * p.av_h;
   V3 gi = g_ext_of<DIM>(a, i);
   V3 acc = {0.f, 0.f, 0.f}, accv = {0.f, 0.f, 0.f};
@@ -152,7 +294,7 @@ __global__ __launch_bounds__(kB) void k_force(SweepArgs a) {
     const float Vj = mj / rhoj;
     const float cpre = (Vi2 + Vj * Vj) * inv_mi * gw * inv_d;
     const float eta_ij = 2.0f * etai * etaj / (etai + etaj + kEps);
-    if (SOLVER == 0) {
+    if (SOLVER == 0 || SOLVER == 2) {
       const float pij = (rhoj * pi_ + rhoi * pj) / (rhoi + rhoj);
       V3 t = (-pij) * dr + eta_ij * (ui - uj);
       if (TVF) {
@@ -161,6 +303,11 @@ __global__ __launch_bounds__(kB) void k_force(SweepArgs a) {
         t = t + Ar;
       }
       acc = acc + cpre * t;
+      if (SOLVER == 2 && (int)Qj.z == kFluid) {
+        // delta-SPH velocity diffusion (solver.py:300-311)
+        const float pij_d = dot<DIM>(uj - ui, (-1.0f) * dr) * inv_d * inv_d;
+        acc = acc + (Vj * pij_d * gw * inv_d * delta_pref / rhoi) * dr;
+      }
     } else {
       const int tag_j = (int)Qj.z;
       const bool wall_j = is_wall(tag_j);
@@ -305,15 +452,25 @@ int launch_sweeps(sph_ws* ws, SweepArgs& a, hipStream_t st) {
   const int n = a.n;
   dim3 grid(xcd_grid(cdivl((int64_t)n * tpp_of(DIM), kB), a.remap));
   const auto& p = a.prm;
-  // density
-  if (!p.density_evolution)
+  // density (solver.py:750-796)
+  if (!p.density_evolution) {
     SPH_LAUNCH(ws, "density", st, (k_density<DIM, KIND, 0><<<grid, kB, 0, st>>>(a)));
-  else if (p.solver == 0)
+  } else if (p.solver == 0) {
     SPH_LAUNCH(ws, "density", st, (k_density<DIM, KIND, 1><<<grid, kB, 0, st>>>(a)));
-  else
+  } else if (p.solver == 2) {
+    SPH_LAUNCH(ws, "delta_L", st, (k_delta_L<DIM, KIND><<<grid, kB, 0, st>>>(a)));
+    SPH_LAUNCH(ws, "delta_grad", st, (k_delta_grad<DIM, KIND><<<grid, kB, 0, st>>>(a)));
+    SPH_LAUNCH(ws, "density", st, (k_density<DIM, KIND, 3><<<grid, kB, 0, st>>>(a)));
+  } else {
     SPH_LAUNCH(ws, "density", st, (k_density<DIM, KIND, 2><<<grid, kB, 0, st>>>(a)));
-  // walls
-  const bool gwbc = p.bc_trick && p.solver == 0;
+  }
+  a.Ud = a.R.U1; a.Qd = a.R.Q1;
+  if (p.density_evolution && p.rho_renorm) {  // solver.py:755, 770, 792
+    SPH_LAUNCH(ws, "renorm", st, (k_renorm<DIM, KIND><<<grid, kB, 0, st>>>(a)));
+    a.Ud = a.R.U3; a.Qd = a.R.Q3;
+  }
+  // walls (solver.py:806-828)
+  const bool gwbc = p.bc_trick && (p.solver == 0 || p.solver == 2);
   const bool heat_bc = p.bc_trick && p.solver == 1 && p.heat_conduction;
   if (gwbc) {
     SPH_LAUNCH(ws, "walls", st, (k_walls<DIM, KIND, 0><<<grid, kB, 0, st>>>(a)));
@@ -323,13 +480,18 @@ int launch_sweeps(sph_ws* ws, SweepArgs& a, hipStream_t st) {
   if (gwbc || heat_bc) {
     a.Uf = a.R.U2; a.Vf = a.R.V2; a.Qf = a.R.Q2;
   } else {
-    a.Uf = a.R.U1; a.Vf = a.R.V0; a.Qf = a.R.Q1;
+    a.Uf = a.Ud; a.Vf = a.R.V0; a.Qf = a.Qd;
   }
   if (p.solver == 0) {
     if (p.tvf_stress)
       SPH_LAUNCH(ws, "force", st, (k_force<DIM, KIND, 0, true><<<grid, kB, 0, st>>>(a)));
     else
       SPH_LAUNCH(ws, "force", st, (k_force<DIM, KIND, 0, false><<<grid, kB, 0, st>>>(a)));
+  } else if (p.solver == 2) {
+    if (p.tvf_stress)
+      SPH_LAUNCH(ws, "force", st, (k_force<DIM, KIND, 2, true><<<grid, kB, 0, st>>>(a)));
+    else
+      SPH_LAUNCH(ws, "force", st, (k_force<DIM, KIND, 2, false><<<grid, kB, 0, st>>>(a)));
   } else {
     SPH_LAUNCH(ws, "force", st, (k_force<DIM, KIND, 1, false><<<grid, kB, 0, st>>>(a)));
   }

@@ -89,6 +89,10 @@ struct sph_ws {
   float4 *Q0 = nullptr, *Q1 = nullptr, *Q2 = nullptr;  // p, eta, tag, T
   float4* NW = nullptr;  // wall normal
   float4* UT = nullptr;  // Riemann wall velocity u_tilde
+  float4 *U3 = nullptr, *Q3 = nullptr;  // after density renormalisation
+  // delta-SPH: renormalisation matrix L (row-major, 4 or 9 floats) and the
+  // two density-gradient terms
+  float4 *DL0 = nullptr, *DL1 = nullptr, *DL2 = nullptr, *DG0 = nullptr, *DG1 = nullptr;
   float* drho = nullptr; // sorted drhodt
   // adjoint sweeps: sorted cotangent of dudt, then the per-particle partials
   float4 *LA = nullptr, *AB0 = nullptr, *AB1 = nullptr;

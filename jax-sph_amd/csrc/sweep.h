@@ -108,6 +108,8 @@ __device__ __forceinline__ float4 f4(V3 v, float w) { return make_float4(v.x, v.
 struct Recs {
   float4 *P, *U0, *V0, *Q0, *NW, *U1, *Q1, *U2, *V2, *Q2, *UT;
   float* drho;
+  float4 *U3, *Q3;                    // renormalised density stage
+  float4 *DL0, *DL1, *DL2, *DG0, *DG1;  // delta-SPH matrices and gradient terms
 };
 
 struct SweepArgs {
@@ -122,6 +124,9 @@ struct SweepArgs {
   int n, npad, lane_width;
   // which record set the force sweep reads
   const float4 *Uf, *Vf, *Qf;
+  // which records hold the density stage's (u, rho) and (p, eta, tag, T):
+  // U1/Q1, or U3/Q3 after renormalisation
+  const float4 *Ud, *Qd;
   int heat_in_density;  // apply T += dt*dTdt in the density epilogue
   sph_bc_spec bc;
   // Pair filter of skin lists: squared distances below cut_lo are inside the
@@ -225,6 +230,8 @@ __device__ __forceinline__ V3 g_ext_of(const SweepArgs& a, int i) {
 // MODE 0: summation (solver.py:176-178, 794-796)
 // MODE 1: SPH continuity (solver.py:24-30, 750-756)
 // MODE 2: Riemann continuity (solver.py:108-164, 772-793)
+// MODE 3: delta-SPH continuity (solver.py:33-105, 757-769) from the gradient
+//         terms of k_delta_grad
 // plus the Riemann wall velocity u_tilde (solver.py:547-552) for wall targets,
 // then the EOS (solver.py:801).
 // Returns the pairs of s inside the cutoff (summed over its TPP lanes).
@@ -239,10 +246,11 @@ __device__ __forceinline__ int density_of(const SweepArgs& a, const int s, const
   const bool fluid_i = tag_i == kFluid;
   const bool want_ut = p.solver == 1 && p.bc_trick && !p.free_slip && is_wall(tag_i);
   const V3 ui = v3(Ui);
-  float wsum = 0.0f, acc = 0.0f, utw = 0.0f;
+  float wsum = 0.0f, acc = 0.0f, utw = 0.0f, diff = 0.0f;
   V3 ut = {0.f, 0.f, 0.f};
   V3 gi = {0.f, 0.f, 0.f};
   if (MODE == 2) gi = g_ext_of<DIM>(a, a.perm[s]);
+  const V3 rgi = MODE == 3 ? v3(a.R.DG0[s]) : V3{0.f, 0.f, 0.f};
   int npair = 0;
   for_each_pair(a, s, h, [&](const int j) {
     const float4 Pj = a.R.P[j];
@@ -262,6 +270,16 @@ __device__ __forceinline__ int density_of(const SweepArgs& a, const int s, const
       if (MODE == 1) {
         const float vj = Pj.w / Uj.w;
         acc += vj * dot<DIM>(ui - v3(Uj), kg);
+      } else if (MODE == 3) {
+        const float vj = Pj.w / Uj.w;
+        acc += vj * dot<DIM>(ui - v3(Uj), kg);
+        if ((int)a.R.Q0[j].z == kFluid) {
+          // psi_ij = 2 (rho_j - rho_i) (-r_ij) / (d + eps)^2 - grad_i - grad_j
+          const float de = d + kEps;
+          const V3 psi = ((2.0f * (Uj.w - Ui.w)) / (de * de)) * ((-1.0f) * dr) - rgi -
+                         v3(a.R.DG1[j]);
+          diff += dot<DIM>(psi, kg) * vj;
+        }
       } else {
         const float4 Qj = a.R.Q0[j];
         const bool wall_j = is_wall((int)Qj.z);
@@ -304,6 +322,7 @@ __device__ __forceinline__ int density_of(const SweepArgs& a, const int s, const
   } else {
     acc = lane_sum<TPP>(acc);
   }
+  if (MODE == 3) diff = lane_sum<TPP>(diff);
   if (p.solver == 1 && p.bc_trick) {
     utw = lane_sum<TPP>(utw);
     ut = lane_sum<TPP>(ut);
@@ -317,6 +336,9 @@ __device__ __forceinline__ int density_of(const SweepArgs& a, const int s, const
     drhodt = a.in.drhodt ? a.in.drhodt[i] : 0.0f;
   } else if (MODE == 1) {
     drhodt = Ui.w * acc;
+    rho = Ui.w + p.dt * drhodt;
+  } else if (MODE == 3) {
+    drhodt = Ui.w * acc + p.c_ref * p.diff_delta * p.support * diff;
     rho = Ui.w + p.dt * drhodt;
   } else {
     drhodt = acc * (fluid_i ? 1.0f : 0.0f);
@@ -402,6 +424,7 @@ inline int check_state(const sph_solver_params* prm, const sph_state_in* in, con
     return SPH_ERR_ARG;
   }
   if (prm->kernel < 0 || prm->kernel > 6) { set_error("unknown kernel"); return SPH_ERR_ARG; }
+  if (prm->solver < 0 || prm->solver > 2) { set_error("unknown solver"); return SPH_ERR_ARG; }
   return SPH_OK;
 }
 
@@ -411,8 +434,9 @@ inline SweepArgs make_args(sph_ws* ws, int bank, const sph_solver_params* prm, c
   a.prm = *prm;
   a.in = *in;
   a.out = *out;
-  a.R = Recs{ws->P, ws->U0, ws->V0, ws->Q0, ws->NW, ws->U1, ws->Q1, ws->U2, ws->V2, ws->Q2,
-             ws->UT, ws->drho};
+  a.R = Recs{ws->P,  ws->U0, ws->V0, ws->Q0,  ws->NW,  ws->U1,  ws->Q1,  ws->U2,  ws->V2,
+             ws->Q2, ws->UT, ws->drho, ws->U3, ws->Q3, ws->DL0, ws->DL1, ws->DL2, ws->DG0,
+             ws->DG1};
   const Bank& B = bank == kSkinBank ? ws->sk.b : ws->bank[bank];
   a.perm = B.perm;
   a.ell = B.ell;
@@ -431,6 +455,7 @@ inline SweepArgs make_args(sph_ws* ws, int bank, const sph_solver_params* prm, c
   a.part = ws->part_sweep;
   a.remap = ws->xcd_remap;
   a.Uf = a.R.U1; a.Vf = a.R.V0; a.Qf = a.R.Q1;
+  a.Ud = a.R.U1; a.Qd = a.R.Q1;
   a.heat_in_density = prm->heat_conduction && !prm->bc_trick;
   a.bc.kind = 0;
   a.bc.u_lid[0] = a.bc.u_lid[1] = a.bc.u_lid[2] = 0.0f;

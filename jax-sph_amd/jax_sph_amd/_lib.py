@@ -44,7 +44,9 @@ class SolverParams(Structure):
         ("p_ref", c_float), ("rho_ref", c_float), ("p_bg", c_float), ("gamma", c_float),
         ("inv_gamma", c_float), ("riemann_k", c_float),
         ("box", c_float * 3), ("g_const", c_float * 3),
-        ("tvf_stress", c_int32), ("pad_", c_int32),
+        ("tvf_stress", c_int32), ("rho_renorm", c_int32),
+        ("diff_delta", c_float), ("diff_alpha", c_float), ("support", c_float),
+        ("pad_", c_int32),
     ]
 
 

@@ -2,14 +2,16 @@
 
 ``WCSPH(...).forward_wrapper()`` returns ``forward(state, neighbors)`` with the
 reference's 16-key state dict.  The whole RHS runs in the HIP library
-(``sph_forward``): a density sweep (summation / SPH continuity / Riemann
-continuity, plus the Riemann wall velocity), the wall sweep (generalized wall
-BC or the Riemann heat BC), and one force sweep (standard or Riemann pressure
-and viscosity, transport-velocity background and stress terms, artificial
-viscosity, heat conduction) over the pairs of the neighbour list.
+(``sph_forward``) as pair sweeps over the neighbour list:
 
-Not provided: the DELTA solver and density renormalisation (no configuration on
-this path uses them); they raise ``NotImplementedError``.
+* a density sweep: summation, SPH / Riemann / delta-SPH continuity, plus the
+  Riemann wall velocity.  Delta-SPH first needs two sweeps for its
+  renormalisation matrices and density gradients;
+* optional density renormalisation;
+* the wall sweep: generalized wall BC, or the Riemann heat BC;
+* one force sweep: standard, Riemann or delta-SPH pressure and viscosity,
+  transport-velocity background and stress terms, artificial viscosity and
+  heat conduction.
 """
 from __future__ import annotations
 
@@ -61,10 +63,8 @@ class WCSPH:
                  is_bc_trick: bool = False, is_rho_evol: bool = False,
                  artificial_alpha: float = 0.0, is_free_slip: bool = False,
                  is_rho_renorm: bool = False, is_heat_conduction: bool = False):
-        if solver not in ("SPH", "RIE"):
-            raise NotImplementedError(f"solver {solver!r} is not provided (SPH and RIE are)")
-        if is_rho_renorm:
-            raise NotImplementedError("density renormalisation is not provided")
+        if solver not in ("SPH", "RIE", "DELTA"):
+            raise ValueError(f"unknown solver {solver!r} (SPH, RIE or DELTA)")
         if kernel not in KERNELS:
             raise ValueError(f"unknown kernel {kernel!r}")
         if not isinstance(displacement_fn, PeriodicDisplacement):
@@ -86,6 +86,7 @@ class WCSPH:
         self.is_heat_conduction = is_heat_conduction
         self.dx = dx
         self.dim = dim
+        self.support = h_fac * dx          # solver.py:688, 693
         self._kernel_fn = KERNELS[kernel](h=h_fac * dx, dim=dim)
 
     # ---------------------------------------------------------------- params
@@ -101,7 +102,11 @@ class WCSPH:
         k = self._kernel_fn
         p.dim = self.dim
         p.kernel = k.kind
-        p.solver = 0 if self.solver == "SPH" else 1
+        p.solver = {"SPH": 0, "RIE": 1, "DELTA": 2}[self.solver]
+        p.rho_renorm = 1 if self.is_rho_renorm else 0
+        p.diff_delta = float(_f(self.diff_delta))
+        p.diff_alpha = float(_f(self.diff_alpha))
+        p.support = float(_f(self.support))
         p.density_evolution = 1 if self.is_rho_evol else 0
         p.bc_trick = 1 if self.is_bc_trick else 0
         p.free_slip = 1 if self.is_free_slip else 0

@@ -240,14 +240,19 @@ def _isin_wall(tag):
 
 # ------------------------------------------------------------------------- WCSPH
 class WCSPH:
-    """``solver.py:613-951`` restated (SPH and RIE solvers; DELTA not restated)."""
+    """``solver.py:613-951`` restated (SPH, RIE and DELTA solvers, density
+    renormalisation)."""
 
     def __init__(self, box, eos, g_ext_fn: Callable, dx, dim, dt, c_ref, eta_limiter=3,
                  solver="SPH", kernel="QSK", h_fac=1.0, is_bc_trick=False,
                  is_rho_evol=False, artificial_alpha=0.0, is_free_slip=False,
-                 is_heat_conduction=False, dtype=np.float32):
-        if solver not in ("SPH", "RIE"):
-            raise NotImplementedError("oracle restates the SPH and RIE solvers only")
+                 is_heat_conduction=False, dtype=np.float32, diff_delta=0.02, diff_alpha=0.1,
+                 is_rho_renorm=False):
+        if solver not in ("SPH", "RIE", "DELTA"):
+            raise NotImplementedError(f"unknown solver {solver!r}")
+        self.diff_delta, self.diff_alpha = diff_delta, diff_alpha      # solver.py:626-627
+        self.is_rho_renorm = is_rho_renorm
+        self.support = h_fac * dx                                      # solver.py:688, 693
         self.t = np.dtype(dtype).type
         self.dtype = np.dtype(dtype)
         self.box = np.asarray(box, dtype=self.dtype).reshape(-1)
@@ -309,11 +314,20 @@ class WCSPH:
             temp = v_j * ((u[i_s] - u[j_s]) * grad_w_dist).sum(axis=1)
             drhodt = rho * _seg(temp, i_s, N)
             rho = rho + dt * drhodt
+            if self.is_rho_renorm:
+                rho = self._rho_renorm(rho, mass, i_s, j_s, w_dist, N)
+        elif self.is_rho_evol and self.solver == "DELTA":
+            rho, drhodt = self._rho_evol_delta(rho, mass, dr, dist, u, i_s, j_s, dt, N,
+                                               fluid_mask[j_s])
+            if self.is_rho_renorm:
+                rho = self._rho_renorm(rho, mass, i_s, j_s, w_dist, N)
         elif self.is_rho_evol and self.solver == "RIE":
             temp = self._rho_evol_riemann(e_s, rho, mass, u, p, dr, gw, wall_j, nw,
                                           g_ext, i_s, j_s)
             drhodt = _seg(temp, i_s, N) * fluid_mask
             rho = rho + dt * drhodt
+            if self.is_rho_renorm:
+                rho = self._rho_renorm(rho, mass, i_s, j_s, w_dist, N)
         else:
             rho_ = mass * _seg(w_dist, i_s, N)
             rho = np.where(fluid_mask != 0, rho_, rho)
@@ -321,7 +335,7 @@ class WCSPH:
         p = self.eos.p_fn(rho)                                           # solver.py:801
         p_bg_tvf = self.eos.p_fn(np.zeros_like(p))
 
-        if self.is_bc_trick and self.solver == "SPH":
+        if self.is_bc_trick and self.solver in ("SPH", "DELTA"):           # solver.py:806
             p, rho, u, v, T = self._gwbc(T, rho, tag, u, v, p, g_ext, i_s, j_s, w_dist,
                                          dr, nw, N)
             mask = None
@@ -346,13 +360,25 @@ class WCSPH:
         wv = ((m_i / rho_i) ** 2 + (m_j / rho_j) ** 2) / m_i
         c = wv * gw / (dist + eps)
 
-        if self.solver == "SPH":                                         # solver.py:221-256
+        if self.solver in ("SPH", "DELTA"):                              # solver.py:221-256
             eta_ij = t(2) * eta[i_s] * eta[j_s] / (eta[i_s] + eta[j_s] + eps)
             p_ij = (rho_j * p[i_s] + rho_i * p[j_s]) / (rho_i + rho_j)
             Ar_i = (rho_i[:, None] * u[i_s]) * ((v[i_s] - u[i_s]) * dr).sum(axis=1)[:, None]
             Ar_j = (rho_j[:, None] * u[j_s]) * ((v[j_s] - u[j_s]) * dr).sum(axis=1)[:, None]
             A_r = (Ar_i + Ar_j) / t(2)
             a = c[:, None] * (-p_ij[:, None] * dr + A_r + eta_ij[:, None] * (u[i_s] - u[j_s]))
+            if self.solver == "DELTA":                                   # solver.py:290-311
+                kg = gw[:, None] * (dr / (dist[:, None] + eps))
+                V_j = m_j / rho_j
+                pi_ij = ((u[j_s] - u[i_s]) * -dr).sum(axis=1) / (dist + eps) ** 2
+                ad = (V_j * pi_ij)[:, None] * kg
+                ad = ad * t(self.diff_alpha)
+                ad = ad * t(self.support)
+                ad = ad * t(self.c_ref)
+                ad = ad * t(self.eos.rho_ref)
+                ad = ad / rho_i[:, None]
+                ad = ad * fluid_mask[j_s][:, None]
+                a = a + ad
         else:
             a = self._acc_riemann(e_s, dr, dist, gw, rho, mass, u, p, eta, wall_j, mask,
                                   nw, g_ext, u_tilde, i_s, j_s)
@@ -378,6 +404,58 @@ class WCSPH:
             "dudt": dudt + g_ext, "dvdt": dvdt, "rho": rho, "p": p, "mass": mass,
             "eta": eta, "dTdt": dTdt, "T": T, "kappa": kappa, "Cp": Cp, "nw": nw,
         }
+
+    # solver.py:167-173 (all particles, no fluid mask)
+    def _rho_renorm(self, rho, mass, i_s, j_s, w_dist, N):
+        t = self.t
+        nominator = _seg(mass[j_s] * w_dist, i_s, N)
+        den = _seg((mass / rho)[j_s] * w_dist, i_s, N)
+        den = np.where(den > 1, t(1), den)
+        return nominator / den
+
+    # solver.py:33-105, literally: L_matj is the inverse of the j_s-grouped sum
+    # of outer(r_ij, grad W_ij V_i), and rho_grad_term_j is segment-summed over
+    # i_s, then read at j_s.
+    def _rho_evol_delta(self, rho, mass, r_ij, d_ij, u, i_s, j_s, dt, N, fluidmask_j):
+        t = self.t
+        eps = t(EPS)
+        kg, V_i, V_j, L_i, L_j, rgi, rgj = self.delta_gradients(rho, mass, r_ij, d_ij, i_s,
+                                                                 j_s, N)
+        rho_term = t(2) * (rho[j_s] - rho[i_s])[:, None] * (-r_ij) / (
+            (d_ij + eps) ** 2)[:, None]
+        psi = rho_term - rgi[i_s] - rgj[j_s]
+        diff_term = _seg((psi * kg).sum(axis=1) * V_j * fluidmask_j, i_s, N)
+        cont = ((u[i_s] - u[j_s]) * kg).sum(axis=1) * V_j
+        coef = t(self.c_ref * self.diff_delta * self.support)
+        drhodt = rho * _seg(cont, i_s, N) + coef * diff_term
+        return rho + t(dt) * drhodt, drhodt
+
+    def delta_gradients(self, rho, mass, r_ij, d_ij, i_s, j_s, N):
+        """Renormalisation matrices and density-gradient terms of solver.py:42-69."""
+        t = self.t
+        eps = t(EPS)
+        dim = r_ij.shape[1]
+        e_ab = r_ij / (d_ij[:, None] + eps)
+        kg = self.kernel.grad_w(d_ij)[:, None] * e_ab
+        V_i = mass[i_s] / rho[i_s]
+        V_j = mass[j_s] / rho[j_s]
+
+        def seg_mat(vals, seg):
+            out = np.zeros((N, dim, dim), self.dtype)
+            np.add.at(out, seg, vals)
+            return out
+
+        temp = (-r_ij)[:, :, None] * (kg * V_j[:, None])[:, None, :]   # L_fn(r_ij, kg, V_j)
+        L_i = np.linalg.inv(seg_mat(temp, i_s)).astype(self.dtype)
+        temp = (r_ij)[:, :, None] * (kg * V_i[:, None])[:, None, :]    # L_fn(-r_ij, kg, V_i)
+        L_j = np.linalg.inv(seg_mat(temp, j_s)).astype(self.dtype)
+        gi = (rho[j_s] - rho[i_s])[:, None] * np.einsum("pab,pb->pa", L_i[i_s],
+                                                          kg * V_j[:, None])
+        rgi = _seg(gi, i_s, N)
+        gj = (rho[i_s] - rho[j_s])[:, None] * np.einsum("pab,pb->pa", L_j[j_s],
+                                                          -kg * V_i[:, None])
+        rgj = _seg(gj, i_s, N)
+        return kg, V_i, V_j, L_i, L_j, rgi, rgj
 
     # solver.py:108-164
     def _rho_evol_riemann(self, e_s, rho, mass, u, p, dr, gw, wall_j, nw, g_ext, i_s, j_s):

@@ -37,7 +37,8 @@ def oracle_solver(s: "js.solver.WCSPH", box):
     return so.WCSPH(box, oracle_eos(s.eos), oracle_g(s.g_ext_fn), s.dx, s.dim, s.dt, s.c_ref,
                     s.eta_limiter, s.solver, s._kernel_fn.name, s._kernel_fn.h / s.dx,
                     s.is_bc_trick, s.is_rho_evol, s.artificial_alpha, s.is_free_slip,
-                    s.is_heat_conduction, np.float32)
+                    s.is_heat_conduction, np.float32, diff_delta=s.diff_delta,
+                    diff_alpha=s.diff_alpha, is_rho_renorm=s.is_rho_renorm)
 
 
 def oracle_bc(bc: "cs.CaseBC", dim):
@@ -122,4 +123,5 @@ def oracle_solver64(s: "js.solver.WCSPH", box):
     return so.WCSPH(box, eos, lambda r: g32(r.astype(np.float32)).astype(np.float64), s.dx,
                     s.dim, s.dt, s.c_ref, s.eta_limiter, s.solver, s._kernel_fn.name,
                     s._kernel_fn.h / s.dx, s.is_bc_trick, s.is_rho_evol, s.artificial_alpha,
-                    s.is_free_slip, s.is_heat_conduction, np.float64)
+                    s.is_free_slip, s.is_heat_conduction, np.float64, diff_delta=s.diff_delta,
+                    diff_alpha=s.diff_alpha, is_rho_renorm=s.is_rho_renorm)

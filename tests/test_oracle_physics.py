@@ -427,3 +427,93 @@ def test_tgv2d_kinetic_energy_decay():
     # of the physical decay (about +12% in rate here); the viscous operator itself is
     # pinned sharply by test_shear_wave_viscous_term
     assert abs(rate / (16 * math.pi ** 2 / 100.0) - 1) < 0.15, rate
+
+
+# ------------------------------------------------------ DELTA-SPH and renormalisation
+def _delta_case(rho_fn, jitter=0.2, n=30):
+    r, box, dx = _periodic_case(n, 2, jitter=jitter)
+    st = H.base_state(r, dx=dx, dtype=np.float64)
+    st["rho"] = rho_fn(r)
+    st["mass"] = np.full(len(r), dx * dx)
+    sol = _solver(box, dx, 2, solver="DELTA", is_rho_evol=True)
+    idx = H.pairs(r, box, sol.kernel.cutoff, np.float64)
+    keep = idx[0] < len(r)
+    i_s, j_s = idx[0][keep].astype(np.int64), idx[1][keep].astype(np.int64)
+    dr = nlo.periodic_displacement(sol.box, r[i_s] - r[j_s])
+    d = nlo.distance(dr)
+    interior = np.all((r > 0.2) & (r < 0.8), axis=1)   # no neighbour across a box face
+    return sol, st, idx, i_s, j_s, dr, d, interior
+
+
+def test_delta_renormalised_gradient_is_exact_for_linear_density():
+    """solver.py:42-63: with L = inv(sum (r_j - r_i) x grad W V_j), the term
+    sum (rho_j - rho_i) L grad W V_j reproduces grad rho exactly for a linear
+    density on any particle distribution (the matrix is symmetric)."""
+    b = np.array([0.3, -0.2])
+    sol, st, idx, i_s, j_s, dr, d, inner = _delta_case(lambda r: 1.0 + r @ b)
+    N = len(st["r"])
+    kg, V_i, V_j, L_i, L_j, rgi, rgj = sol.delta_gradients(st["rho"], st["mass"], dr, d, i_s,
+                                                           j_s, N)
+    np.testing.assert_allclose(rgi[inner], np.broadcast_to(b, rgi[inner].shape), rtol=0,
+                               atol=1e-10)
+    # the j_s-grouped matrix is the negated i_s-grouped one, so its inverse is -L_i
+    np.testing.assert_allclose(L_j, -L_i, rtol=1e-9, atol=1e-9)
+
+
+def test_delta_j_gradient_term_has_the_references_sign():
+    """rho_grad_term_j (solver.py:64-66) uses the neighbour's L_matj = -L_i
+    and the centre's volume: for a weak linear density it is -grad rho (not
+    +grad rho as in Marrone et al.), which a sign slip would flip."""
+    b = np.array([1e-3, 2e-3])
+    sol, st, idx, i_s, j_s, dr, d, inner = _delta_case(lambda r: 1.0 + r @ b, jitter=0.0)
+    N = len(st["r"])
+    *_, rgi, rgj = sol.delta_gradients(st["rho"], st["mass"], dr, d, i_s, j_s, N)
+    inner2 = np.all((st["r"] > 0.3) & (st["r"] < 0.7), axis=1)
+    np.testing.assert_allclose(rgj[inner2], -np.broadcast_to(b, rgj[inner2].shape),
+                               rtol=0, atol=2e-2 * np.abs(b).max())
+
+
+def test_delta_reduces_to_standard_sph_for_uniform_fields():
+    """Uniform density: the diffusive part of the DELTA continuity vanishes
+    (rho_j - rho_i = 0), so drho/dt equals the standard SPH continuity; uniform
+    velocity: the DELTA acceleration term vanishes (u_j - u_i = 0)."""
+    r, box, dx = _periodic_case(24, 2, jitter=0.15)
+    rng = np.random.default_rng(3)
+    st = H.base_state(r, dx=dx, dtype=np.float64)
+    st["u"] = rng.normal(size=r.shape) * 0.1
+    st["v"] = st["u"].copy()
+    st["p"] = rng.normal(size=len(r))
+    sph = _solver(box, dx, 2, is_rho_evol=True)
+    dlt = _solver(box, dx, 2, solver="DELTA", is_rho_evol=True)
+    idx = H.pairs(r, box, sph.kernel.cutoff, np.float64)
+    a, b = sph.forward(st, idx), dlt.forward(st, idx)
+    np.testing.assert_allclose(b["drhodt"], a["drhodt"], rtol=1e-10, atol=1e-12)
+    np.testing.assert_allclose(b["rho"], a["rho"], rtol=1e-12)
+    st["u"] = np.broadcast_to([0.3, -0.1], r.shape).copy()
+    st["v"] = st["u"].copy()
+    a, b = sph.forward(st, idx), dlt.forward(st, idx)
+    np.testing.assert_allclose(b["dudt"], a["dudt"], rtol=1e-10, atol=1e-12)
+
+
+def test_density_renormalisation_is_a_projection():
+    """solver.py:167-173: rho = sum m W / min(sum (m/rho) W, 1).  For a uniform
+    density c it returns c while the Shepard weight sum (m/c) W <= 1, and the
+    plain sum m W once the weight is clamped at 1."""
+    r, box, dx = _periodic_case(20, 2, jitter=0.1)
+    c = 1.3
+    st = H.base_state(r, dx=dx, dtype=np.float64)
+    st["rho"] = np.full(len(r), c)
+    sol = _solver(box, dx, 2, is_rho_evol=True, is_rho_renorm=True)
+    idx = H.pairs(r, box, sol.kernel.cutoff, np.float64)
+    keep = idx[0] < len(r)
+    i_s, j_s = idx[0][keep].astype(np.int64), idx[1][keep].astype(np.int64)
+    w = sol.kernel.w(nlo.distance(nlo.periodic_displacement(sol.box, r[i_s] - r[j_s])))
+    sw = np.bincount(i_s, weights=w, minlength=len(r))
+    for scale in (0.9, 1.1):
+        m = scale * c / sw.max() if scale < 1 else scale * c / sw.min()
+        mass = np.full(len(r), m)
+        out = sol._rho_renorm(st["rho"], mass, i_s, j_s, w, len(r))
+        if scale < 1:
+            np.testing.assert_allclose(out, c, rtol=1e-12)
+        else:
+            np.testing.assert_allclose(out, m * sw, rtol=1e-12)

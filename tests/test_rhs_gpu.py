@@ -73,6 +73,19 @@ CASES = [
     ("pf", {"case": {"dx": 0.05}}),
     ("tgv", {"case": {"dim": 2, "dx": 0.02}, "solver": {"name": "RIE",
                                                        "density_evolution": True}}),
+    # delta-SPH (solver.py:33-105, 259-313) and density renormalisation (:167-173)
+    ("tgv", {"case": {"dim": 2, "dx": 0.02}, "solver": {"name": "DELTA"}}),
+    ("tgv", {"case": {"dim": 2, "dx": 0.02}, "solver": {"name": "DELTA",
+                                                       "density_evolution": True}}),
+    ("tgv", {"case": {"dim": 3, "dx": 2 * np.pi / 16}, "solver": {"name": "DELTA",
+                                                                 "density_evolution": True,
+                                                                 "tvf": 1.0}}),
+    ("ldc", {"case": {"dx": 0.04}, "solver": {"name": "DELTA", "density_evolution": True}}),
+    ("tgv", {"case": {"dim": 2, "dx": 0.02}, "solver": {"density_evolution": True,
+                                                       "density_renormalize": True}}),
+    ("ldc", {"case": {"dx": 0.04}, "solver": {"name": "DELTA", "density_evolution": True,
+                                              "density_renormalize": True}}),
+    ("db", {"case": {"dx": 0.04}, "solver": {"name": "RIE", "density_renormalize": True}}),
 ]
 
 

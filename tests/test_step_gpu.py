@@ -63,6 +63,8 @@ STATE = ("r", "u", "v", "rho")
     ("tgv", {"case": {"dim": 3, "dx": 2 * np.pi / 16}, "solver": {"tvf": 1.0}}),
     ("ldc", {"case": {"dx": 0.04}}),
     ("db", {"case": {"dx": 0.04}, "solver": {"name": "RIE"}, "kernel": {"name": "WC2K"}}),
+    ("ldc", {"case": {"dx": 0.04}, "solver": {"name": "DELTA", "density_evolution": True,
+                                              "density_renormalize": True}}),
 ])
 def test_fused_steps_match_oracle(case, ov, skin):
     cfg, state, nb, adv, R, *_ = _make(case, ov, skin=skin)
