@@ -517,3 +517,53 @@ def test_density_renormalisation_is_a_projection():
             np.testing.assert_allclose(out, c, rtol=1e-12)
         else:
             np.testing.assert_allclose(out, m * sw, rtol=1e-12)
+
+
+def test_delta_density_diffusion_continuum_limit():
+    """solver.py:67-100 on a smooth density rho = 1 + e sin(2 pi x), u = 0.
+
+    The Morris-type part 2 (rho_j - rho_i) r_ji / d^2 . grad W V_j tends to
+    s lap(rho), with s = m / (rho dx^2) the lattice volume ratio.  The i-side
+    renormalised gradient sums to s-free zero against grad W, and the j-side
+    term, which the reference builds from the j_s-grouped matrix (= -L_i), is
+    -grad rho_j, whose divergence sum adds another s lap(rho).  So the DELTA
+    continuity exceeds the plain one by c_ref delta support 2 s lap(rho).  A
+    dropped factor, a different coefficient, a flipped sign of either part or
+    Marrone's sign convention (which makes the two parts cancel) all fail."""
+    n, e, k, delta, s = 50, 1e-3, 2 * np.pi, 0.1, 0.8
+    r, box, dx = _periodic_case(n, 2)
+    st = H.base_state(r, dx=dx, mass=s * dx * dx, dtype=np.float64)
+    st["rho"] = 1 + e * np.sin(k * r[:, 0])
+    sph = _solver(box, dx, 2, is_rho_evol=True)
+    dlt = _solver(box, dx, 2, solver="DELTA", is_rho_evol=True, diff_delta=delta)
+    idx = H.pairs(r, box, sph.kernel.cutoff, np.float64)
+    diff = dlt.forward(st, idx)["drhodt"] - sph.forward(st, idx)["drhodt"]
+    expect = 10.0 * delta * dx * 2 * s * (-k * k * e * np.sin(k * r[:, 0]))
+    # O((k h)^2) smoothing at h = dx, n = 50: about 1%
+    np.testing.assert_allclose(diff, expect, rtol=0, atol=0.02 * np.abs(expect).max())
+
+
+@pytest.mark.parametrize("wave", ["shear", "compression"])
+def test_delta_acceleration_continuum_limit(wave):
+    """solver.py:296-309: sum V_j ((u_j - u_i).r_ji / d^2) grad W tends to
+    s (lap u + 2 grad div u) / (2 (D + 2)) for a smooth u (isotropic fourth
+    moment of grad W; D = 2), s = m / (rho dx^2).  Times alpha support c_ref
+    rho_ref / rho_i: shear u = (e sin 2 pi y, 0) gives lap u; compression
+    u = (e sin 2 pi x, 0) gives 3 lap u.  h = 2 dx samples the kernel finely
+    enough for the lattice moments to be isotropic to about 1%."""
+    n, e, k, alpha, hf, rho = 60, 1e-3, 2 * np.pi, 0.05, 2.0, 1.25
+    r, box, dx = _periodic_case(n, 2)
+    st = H.base_state(r, rho=rho, dx=dx, dtype=np.float64)
+    ax, fac = (1, 1.0) if wave == "shear" else (0, 3.0)
+    st["u"][:, 0] = e * np.sin(k * r[:, ax])
+    st["v"] = st["u"].copy()
+    sph = _solver(box, dx, 2, is_rho_evol=True, h_fac=hf)
+    dlt = _solver(box, dx, 2, solver="DELTA", is_rho_evol=True, diff_alpha=alpha, h_fac=hf)
+    idx = H.pairs(r, box, sph.kernel.cutoff, np.float64)
+    diff = dlt.forward(st, idx)["dudt"] - sph.forward(st, idx)["dudt"]
+    s = 1.0 / rho
+    lap = -k * k * e * np.sin(k * r[:, ax])
+    expect = alpha * hf * dx * 10.0 * 1.0 / rho * s * fac * lap / 8
+    np.testing.assert_allclose(diff[:, 0], expect, rtol=0, atol=0.03 * np.abs(expect).max())
+    # the uniform pressure's lattice sums cancel to rounding only
+    assert np.abs(diff[:, 1]).max() < 1e-6 * np.abs(expect).max()
