@@ -43,6 +43,11 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the 2D TGV forward + adjoint leg")
+    ap.add_argument("--no-ensemble", action="store_true",
+                    help="skip the 64-member batched ensemble leg")
+    ap.add_argument("--ensemble-steps", type=int, default=1000)
+    ap.add_argument("--ensemble-only", action="store_true",
+                    help="run only the ensemble leg and print its JSON")
     ap.add_argument("--cpu-sample-side", type=int, default=20)
     ap.add_argument("--skin", default="auto",
                     help="Verlet skin of the fused step's pair lists ('auto' = 0.1 cutoff, "
@@ -152,6 +157,91 @@ def fwd_adjoint_leg(steps=100, every=10, seed=0):
     return out
 
 
+ENSEMBLE_DX = (0.025, 0.02, 0.0125, 0.01, 0.005)
+
+
+def ensemble_configs(members=64, seed=0):
+    """The batched-ensemble workload: the first half 2D Taylor-Green vortices
+    (periodic unit square, standard SPH), the second half lid-driven cavities
+    (lid u = 1, Re = 100, 3 wall layers, generalised wall BC); each dx drawn
+    from ENSEMBLE_DX with a seeded generator; dt from each member's CFL
+    condition (1,600 to 42,436 particles per member)."""
+    from jax_sph_amd import case_setup as cs
+    dxs = np.random.default_rng(seed).choice(ENSEMBLE_DX, size=members)
+    cfgs = []
+    for k, dx in enumerate(dxs):
+        if k < members // 2:
+            cfgs.append(cs.load_config("tgv", {"case": {"dim": 2, "dx": float(dx)}}))
+        else:
+            cfgs.append(cs.load_config("ldc", {"case": {"dx": float(dx)},
+                                               "solver": {"dt": None}}))
+    return cfgs, [float(x) for x in dxs]
+
+
+def ensemble_leg(steps=1000, members=64, seed=0, graph_steps=20):
+    """64 uneven members in one launch per step phase (graph-replayed, layout
+    autotuned) against the same 64 runs one after the other, each on its own
+    fused step, eager and graph-replayed."""
+    import torch
+    from jax_sph_amd.ensemble import Ensemble, setup_member
+    from jax_sph_amd.graph import StepGraph
+
+    reps = max(1, steps // graph_steps)
+    steps = reps * graph_steps
+    cfgs, dxs = ensemble_configs(members, seed)
+    t0 = time.perf_counter()
+    ens = Ensemble.from_configs(cfgs)
+    setup_s = time.perf_counter() - t0
+    info = ens.info()
+    tune = ens.autotune(steps=graph_steps, reps=2)
+    ens.capture(graph_steps)
+    ens.replay(1)  # warm-up; the timed replays continue the run
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ens.replay(reps)
+    torch.cuda.synchronize()
+    batched = time.perf_counter() - t0
+    overflow_batched = ens.overflowed()
+    del ens
+    seq_eager = seq_graph = 0.0
+    overflow_seq = 0
+    for cfg in ensemble_configs(members, seed)[0]:
+        adv, st, nb, dt = setup_member(cfg)
+        st, nb = adv(dt, st, nb)  # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            st, nb = adv(dt, st, nb)
+        torch.cuda.synchronize()
+        seq_eager += time.perf_counter() - t0
+        overflow_seq |= int(nb.error.code)
+        g = StepGraph(adv, st, nb, dt, steps_per_replay=graph_steps)
+        g.replay(1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        g.replay(reps)
+        torch.cuda.synchronize()
+        seq_graph += time.perf_counter() - t0
+        overflow_seq |= int(g.did_buffer_overflow)
+        del g, st, nb, adv
+    return {
+        "workload": f"{members} 2D members: {members // 2} Taylor-Green vortices + "
+                    f"{members - members // 2} lid-driven cavities (Re 100, 3 wall layers), "
+                    f"dx drawn from {list(ENSEMBLE_DX)} (seed {seed}), CFL dt, QSK h=dx",
+        "members": info["members"], "particles": info["particles"],
+        "sweep_groups": info["groups"], "steps": steps, "setup_s": setup_s,
+        "batched_ms_per_step": batched / steps * 1e3,
+        "sequential_eager_ms_per_step": seq_eager / steps * 1e3,
+        "sequential_graph_ms_per_step": seq_graph / steps * 1e3,
+        "speedup_vs_sequential_eager": seq_eager / batched,
+        "speedup_vs_sequential_graph": seq_graph / batched,
+        "member_steps_per_s_batched": members * steps / batched,
+        "particle_steps_per_s_batched": info["particles"] * steps / batched,
+        "autotune": tune, "overflowed_members": overflow_batched,
+        "sequential_error_code": overflow_seq, "dx": dxs,
+    }
+
+
 def cpu_baseline(n_side, seconds=15.0):
     """Oracle si_euler on a 3D TGV lattice of n_side^3 particles (same physics)."""
     sys.path.insert(0, ROOT)
@@ -209,6 +299,9 @@ def main():
     from jax_sph_amd import case_setup as cs
     from jax_sph_amd import integrator, partition
 
+    if a.ensemble_only:
+        print(json.dumps({"ensemble": ensemble_leg(steps=a.ensemble_steps)}), flush=True)
+        return
     n_side = a.n_side
     cfg = cs.load_config("tgv", {"case": {"dim": 3, "dx": 2 * np.pi / n_side,
                                           "viscosity": 1 / 1600},
@@ -297,6 +390,7 @@ def main():
     }
     if not a.no_adjoint:
         out["fwd_adjoint"] = fwd_adjoint_leg()
+    out["ensemble"] = None if a.no_ensemble else ensemble_leg(steps=a.ensemble_steps)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(a.cpu_sample_side)
     print(json.dumps(out), flush=True)

@@ -127,7 +127,9 @@ int sph_ws_destroy(sph_ws *ws);
 int64_t sph_ws_bytes(const sph_ws *ws);
 /* Launch-shape knobs of a workspace (no counterpart in the reference, whose
  * launches XLA decides).  "xcd_remap" (default 1): the pair build and the
- * sweeps hand each of the 8 XCDs a contiguous range of cell-sorted particles. */
+ * sweeps hand each of the 8 XCDs a contiguous range of cell-sorted particles.
+ * "skin_rebuild" (1): the next fused step rebuilds the skin lists, as after
+ * sph_skin_enable (synchronous; outside graph capture). */
 int sph_ws_tune(sph_ws *ws, const char *key, int64_t value);
 
 /* ---- neighbour search: jax_sph/partition.py:492-571 + jax_md/partition.py:723-1026 */
@@ -218,6 +220,43 @@ int sph_step_vjp(sph_ws *ws, int bank, const sph_solver_params *prm, float dt,
 int sph_integrate(int32_t dim, int64_t n, const float *r, const float *u, const float *dudt,
                   const float *dvdt, float *r_out, float *u_out, float *v_out, float dt,
                   float tvf_coef, const float *box, hipStream_t stream);
+
+/* ---- ensembles ------------------------------------------------------------ */
+/* Many independent simulations advanced by one launch per phase of the fused
+ * step.  The reference has no counterpart: it runs one simulation per process
+ * (simulate.py:19-136), so an ensemble there is a host loop over sph_step-like
+ * steps, one member after the other.  Each member is exactly one sph_step call
+ * on bank 2 (its own workspace with skin lists enabled, its own parameters,
+ * state buffers and error block); a member's results are bitwise those of that
+ * call.  All members have the same dimension and distinct workspaces. */
+typedef struct sph_ensemble sph_ensemble;
+typedef struct {
+  sph_ws *ws;
+  const sph_solver_params *prm;
+  float dt;
+  float tvf_coef;
+  const sph_bc_spec *bc; /* nullable: no case hook */
+  const sph_state_in *in;
+  float *r_out;
+  sph_state_out *out;
+  const int64_t *prev_info; /* nullable */
+  int64_t max_occupancy;
+  int64_t *info;
+} sph_ensemble_member;
+/* Copies the member descriptions (pointers are kept, not the data) and builds
+ * the device argument tables and block maps.  Allocates. */
+int sph_ensemble_create(const sph_ensemble_member *members, int32_t n, sph_ensemble **out);
+/* One fused step of every member on `stream`; allocates nothing and makes no
+ * host decisions, so a HIP graph can capture it. */
+int sph_ensemble_step(sph_ensemble *e, hipStream_t stream);
+/* "order": 0 given, 1 largest member first (default), 2 blocks interleaved;
+ * "xcd": 1 (default) XCD-contiguous workgroup layout.  Rebuilds the block
+ * maps (allocates): tune before capturing steps in a graph. */
+int sph_ensemble_tune(sph_ensemble *e, const char *key, int64_t value);
+/* members, groups of equal sweep plans, workgroups of one lane-parallel launch */
+int sph_ensemble_info(const sph_ensemble *e, int64_t *members, int64_t *groups,
+                      int64_t *lane_blocks);
+int sph_ensemble_destroy(sph_ensemble *e);
 
 /* ---- profiling hooks ---------------------------------------------------- */
 /* When on, the library records a HIP event pair around every kernel it

@@ -11,6 +11,7 @@
 
 #include <cmath>
 
+#include "ens.h"
 #include "sph_device.h"
 #include "sph_internal.h"
 
@@ -91,11 +92,13 @@ __device__ __forceinline__ int wave_aggregated_inc(int32_t* __restrict__ counter
 }
 
 template <int DIM>
-__global__ void k_hash_count(const float* __restrict__ r, int n, Grid g,
-                             int32_t* __restrict__ cell_count, int32_t* __restrict__ pcell,
-                             int32_t* __restrict__ pslot, const int32_t* __restrict__ gate) {
+__device__ __forceinline__ void hash_count_body(const float* __restrict__ r, int n, const Grid& g,
+                                                int32_t* __restrict__ cell_count,
+                                                int32_t* __restrict__ pcell,
+                                                int32_t* __restrict__ pslot,
+                                                const int32_t* __restrict__ gate, int blk) {
   if (gate && *gate == 0) return;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = blk * blockDim.x + threadIdx.x;
   if (i >= n) return;
   int h = 0;
   if (g.use_cells) {
@@ -108,6 +111,13 @@ __global__ void k_hash_count(const float* __restrict__ r, int n, Grid g,
     pcell[i] = h;
     pslot[i] = slot;
   }
+}
+
+template <int DIM>
+__global__ void k_hash_count(const float* __restrict__ r, int n, Grid g,
+                             int32_t* __restrict__ cell_count, int32_t* __restrict__ pcell,
+                             int32_t* __restrict__ pslot, const int32_t* __restrict__ gate) {
+  hash_count_body<DIM>(r, n, g, cell_count, pcell, pslot, gate, blockIdx.x);
 }
 
 // si_euler kick + transport velocity + drift + periodic wrap (integrator.py:26-30)
@@ -152,15 +162,14 @@ __global__ void k_integrate_hash(const float* __restrict__ r, const float* __res
 // rebuild test: a particle that moved more than the trigger distance since
 // the last build (minimum image) raises *flag.
 template <int DIM>
-__global__ void k_integrate_skin(const float* __restrict__ r, const float* __restrict__ u,
-                                 const float* __restrict__ dudt, const float* __restrict__ dvdt,
-                                 float* __restrict__ r_out, float* __restrict__ u_out,
-                                 float* __restrict__ v_out, int n, float dt, float ctvf, Grid g,
-                                 float bx, float by, float bz, int32_t* __restrict__ cell_count,
-                                 const float4* __restrict__ posb, float trigger_sq,
-                                 int32_t* __restrict__ flag) {
+__device__ __forceinline__ void integrate_skin_body(
+    const float* __restrict__ r, const float* __restrict__ u, const float* __restrict__ dudt,
+    const float* __restrict__ dvdt, float* __restrict__ r_out, float* __restrict__ u_out,
+    float* __restrict__ v_out, int n, float dt, float ctvf, const Grid& g, float bx, float by,
+    float bz, int32_t* __restrict__ cell_count, const float4* __restrict__ posb, float trigger_sq,
+    int32_t* __restrict__ flag, int blk) {
 #pragma clang fp contract(off)
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = blk * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float side[3] = {bx, by, bz};
   float x[3] = {0.f, 0.f, 0.f};
@@ -190,15 +199,26 @@ __global__ void k_integrate_skin(const float* __restrict__ r, const float* __res
   if (m2 > trigger_sq) *flag = 1;  // same value from every writer
 }
 
+template <int DIM>
+__global__ void k_integrate_skin(const float* __restrict__ r, const float* __restrict__ u,
+                                 const float* __restrict__ dudt, const float* __restrict__ dvdt,
+                                 float* __restrict__ r_out, float* __restrict__ u_out,
+                                 float* __restrict__ v_out, int n, float dt, float ctvf, Grid g,
+                                 float bx, float by, float bz, int32_t* __restrict__ cell_count,
+                                 const float4* __restrict__ posb, float trigger_sq,
+                                 int32_t* __restrict__ flag) {
+  integrate_skin_body<DIM>(r, u, dudt, dvdt, r_out, u_out, v_out, n, dt, ctvf, g, bx, by, bz,
+                           cell_count, posb, trigger_sq, flag, blockIdx.x);
+}
+
 // Largest reference-grid cell occupancy of this step (one workgroup): sets
 // info[2] and the cell-capacity bit, resets the other info entries (keeping
 // the sticky bits of prev[0]) and clears the histogram for the next step.
 constexpr int kCellsThreads = 1024;
-__global__ __launch_bounds__(kCellsThreads) void k_cells_max(int32_t* __restrict__ count,
-                                                             int64_t ncell,
-                                                             int64_t* __restrict__ info,
-                                                             const int64_t* __restrict__ prev,
-                                                             int64_t capacity) {
+__device__ __forceinline__ void cells_max_body(int32_t* __restrict__ count, int64_t ncell,
+                                               int64_t* __restrict__ info,
+                                               const int64_t* __restrict__ prev,
+                                               int64_t capacity) {
   __shared__ int wm[kCellsThreads / 64];
   int m = 0;
   for (int64_t i = threadIdx.x; i < ncell; i += kCellsThreads) {
@@ -224,6 +244,14 @@ __global__ __launch_bounds__(kCellsThreads) void k_cells_max(int32_t* __restrict
   }
 }
 
+__global__ __launch_bounds__(kCellsThreads) void k_cells_max(int32_t* __restrict__ count,
+                                                             int64_t ncell,
+                                                             int64_t* __restrict__ info,
+                                                             const int64_t* __restrict__ prev,
+                                                             int64_t capacity) {
+  cells_max_body(count, ncell, info, prev, capacity);
+}
+
 __global__ void k_max_reduce(const int32_t* __restrict__ a, int64_t n, int64_t* __restrict__ out) {
   int m = 0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -243,14 +271,10 @@ constexpr int kScanThreads = 1024;
 constexpr int kScanItems = 8;
 
 template <typename TIn, typename TOut>
-__global__ __launch_bounds__(kScanThreads) void k_scan_single(const TIn* in, int64_t n,
-                                                              TOut* __restrict__ out,
-                                                              int64_t* __restrict__ max_out,
-                                                              int64_t* __restrict__ info,
-                                                              int64_t capacity,
-                                                              TIn* clear = nullptr,
-                                                              const int32_t* __restrict__ gate =
-                                                                  nullptr) {
+__device__ __forceinline__ void scan_single_body(const TIn* in, int64_t n, TOut* __restrict__ out,
+                                                 int64_t* __restrict__ max_out,
+                                                 int64_t* __restrict__ info, int64_t capacity,
+                                                 TIn* clear, const int32_t* __restrict__ gate) {
   // `clear` (the input array itself, or null) is zeroed as it is read;
   // `gate` (or null) skips the whole scan when *gate == 0.
   __shared__ TOut wave_tot[kScanThreads / 64];
@@ -322,25 +346,45 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_single(const TIn* in, int
   }
 }
 
-__global__ void k_scatter(const int32_t* __restrict__ pcell, const int32_t* __restrict__ pslot,
-                          const int32_t* __restrict__ cell_start, int32_t* __restrict__ unsorted,
-                          int n, const int32_t* __restrict__ gate = nullptr) {
+template <typename TIn, typename TOut>
+__global__ __launch_bounds__(kScanThreads) void k_scan_single(const TIn* in, int64_t n,
+                                                              TOut* __restrict__ out,
+                                                              int64_t* __restrict__ max_out,
+                                                              int64_t* __restrict__ info,
+                                                              int64_t capacity,
+                                                              TIn* clear = nullptr,
+                                                              const int32_t* __restrict__ gate =
+                                                                  nullptr) {
+  scan_single_body<TIn, TOut>(in, n, out, max_out, info, capacity, clear, gate);
+}
+
+__device__ __forceinline__ void scatter_body(const int32_t* __restrict__ pcell,
+                                             const int32_t* __restrict__ pslot,
+                                             const int32_t* __restrict__ cell_start,
+                                             int32_t* __restrict__ unsorted, int n,
+                                             const int32_t* __restrict__ gate, int blk) {
   if (gate && *gate == 0) return;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = blk * blockDim.x + threadIdx.x;
   if (i >= n) return;
   unsorted[cell_start[pcell[i]] + pslot[i]] = i;
 }
 
+__global__ void k_scatter(const int32_t* __restrict__ pcell, const int32_t* __restrict__ pslot,
+                          const int32_t* __restrict__ cell_start, int32_t* __restrict__ unsorted,
+                          int n, const int32_t* __restrict__ gate = nullptr) {
+  scatter_body(pcell, pslot, cell_start, unsorted, n, gate, blockIdx.x);
+}
+
 // One wavefront per cell: rank-sort the ids that arrived in the cell so that a
 // cell lists its particles in ascending id, like the reference's stable argsort.
-__global__ __launch_bounds__(256) void k_cell_sort(const int32_t* __restrict__ cell_start,
-                                                   const int32_t* __restrict__ unsorted,
-                                                   int32_t* __restrict__ perm,
-                                                   int32_t* __restrict__ iperm, int64_t ncell,
-                                                   const int32_t* __restrict__ gate = nullptr) {
+__device__ __forceinline__ void cell_sort_body(const int32_t* __restrict__ cell_start,
+                                               const int32_t* __restrict__ unsorted,
+                                               int32_t* __restrict__ perm,
+                                               int32_t* __restrict__ iperm, int64_t ncell,
+                                               const int32_t* __restrict__ gate, int blk) {
   if (gate && *gate == 0) return;
   const int lane = threadIdx.x & 63;
-  const int64_t cell = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t cell = ((int64_t)blk * blockDim.x + threadIdx.x) >> 6;
   if (cell >= ncell) return;
   const int b = cell_start[cell], e = cell_start[cell + 1], n = e - b;
   if (n <= 64) {
@@ -363,6 +407,14 @@ __global__ __launch_bounds__(256) void k_cell_sort(const int32_t* __restrict__ c
       iperm[id] = b + rank;
     }
   }
+}
+
+__global__ __launch_bounds__(256) void k_cell_sort(const int32_t* __restrict__ cell_start,
+                                                   const int32_t* __restrict__ unsorted,
+                                                   int32_t* __restrict__ perm,
+                                                   int32_t* __restrict__ iperm, int64_t ncell,
+                                                   const int32_t* __restrict__ gate = nullptr) {
+  cell_sort_body(cell_start, unsorted, perm, iperm, ncell, gate, blockIdx.x);
 }
 
 template <int DIM>
@@ -456,11 +508,13 @@ __device__ __forceinline__ void test_range(const BuildArgs& a, int s, int h, con
 // TPP lanes per sorted particle (the sender): each visits the 3^d cells around
 // the particle's own cell as 3^(d-1) contiguous x-strips, keeps its share of
 // the receivers with d^2 < cutoff^2 and stores them in its own sub-list.
+// blk: the workgroup's block within this list's grid; it also indexes the
+// workgroup's partials.
 template <int DIM, int TPP, bool ORDERED>
-__global__ __launch_bounds__(kBlock) void k_nl_build(BuildArgs a) {
+__device__ __forceinline__ void nl_build_body(const BuildArgs& a, const int blk) {
   __shared__ int32_t stage_buf[kStage][kBlock];
-  if (a.gate && *a.gate == 0) return;  // uniform over the grid
-  const int tid = logical_block(a.remap) * blockDim.x + threadIdx.x;
+  if (a.gate && *a.gate == 0) return;  // uniform over the list's grid
+  const int tid = blk * blockDim.x + threadIdx.x;
   const int s = tid / TPP, h = tid % TPP;
   int count = 0;
   if (s < a.n) {
@@ -536,9 +590,14 @@ __global__ __launch_bounds__(kBlock) void k_nl_build(BuildArgs a) {
     unsigned long long t = 0;
     int mm = 0;
     for (int w = 0; w < kBlock / 64; ++w) { t += tot[w]; mm = mx[w] > mm ? mx[w] : mm; }
-    a.part[2 * blockIdx.x] = (int64_t)t;
-    a.part[2 * blockIdx.x + 1] = mm;
+    a.part[2 * blk] = (int64_t)t;
+    a.part[2 * blk + 1] = mm;
   }
+}
+
+template <int DIM, int TPP, bool ORDERED>
+__global__ __launch_bounds__(kBlock) void k_nl_build(BuildArgs a) {
+  nl_build_body<DIM, TPP, ORDERED>(a, logical_block(a.remap));
 }
 
 __global__ void k_finalize(int64_t* __restrict__ info, int64_t max_occ) {
@@ -551,10 +610,12 @@ __global__ void k_finalize(int64_t* __restrict__ info, int64_t max_occ) {
 // *gate != 0; with clear_flag, zeroes *clear_flag at the end (the skin step's
 // rebuild request is consumed).
 constexpr int kReduceThreads = 1024;
-__global__ __launch_bounds__(kReduceThreads) void k_reduce_parts(
-    const int64_t* __restrict__ part, int nblocks, int64_t* __restrict__ tot_out,
-    int64_t* __restrict__ max_out, int64_t* __restrict__ info, int64_t max_occ,
-    const int32_t* __restrict__ gate, int32_t* __restrict__ clear_flag) {
+__device__ __forceinline__ void reduce_parts_body(const int64_t* __restrict__ part, int nblocks,
+                                                  int64_t* __restrict__ tot_out,
+                                                  int64_t* __restrict__ max_out,
+                                                  int64_t* __restrict__ info, int64_t max_occ,
+                                                  const int32_t* __restrict__ gate,
+                                                  int32_t* __restrict__ clear_flag) {
   __shared__ int64_t ws_t[kReduceThreads / 64], ws_m[kReduceThreads / 64];
   const bool run = !gate || *gate != 0;  // uniform
   if (run) {
@@ -583,6 +644,13 @@ __global__ __launch_bounds__(kReduceThreads) void k_reduce_parts(
     }
   }
   if (clear_flag && threadIdx.x == 0) *clear_flag = 0;
+}
+
+__global__ __launch_bounds__(kReduceThreads) void k_reduce_parts(
+    const int64_t* __restrict__ part, int nblocks, int64_t* __restrict__ tot_out,
+    int64_t* __restrict__ max_out, int64_t* __restrict__ info, int64_t max_occ,
+    const int32_t* __restrict__ gate, int32_t* __restrict__ clear_flag) {
+  reduce_parts_body(part, nblocks, tot_out, max_out, info, max_occ, gate, clear_flag);
 }
 
 template <int DIM>
@@ -908,7 +976,7 @@ int skin_begin(sph_ws* ws, const float* r, const float* u, const float* dudt, co
 // here (the sweeps count the pairs inside the cutoff); info[7] receives the
 // largest list of one particle, and an overflowing sub-list raises
 // SPH_NEIGHBOR_LIST_OVERFLOW so the caller re-allocates.
-int skin_build(sph_ws* ws, int64_t* info, hipStream_t st) {
+static BuildArgs skin_build_args(sph_ws* ws, int64_t* info) {
   const auto& c = ws->cfg;
   auto& S = ws->sk;
   BuildArgs a;
@@ -934,6 +1002,13 @@ int skin_build(sph_ws* ws, int64_t* info, hipStream_t st) {
   a.trunc_bits = SPH_NEIGHBOR_LIST_OVERFLOW;
   a.gate = S.flag;
   a.remap = ws->xcd_remap;
+  return a;
+}
+
+int skin_build(sph_ws* ws, int64_t* info, hipStream_t st) {
+  const auto& c = ws->cfg;
+  auto& S = ws->sk;
+  const BuildArgs a = skin_build_args(ws, info);
   constexpr int T2 = tpp_of(2), T3 = tpp_of(3);
   const dim3 grid(xcd_grid(cdiv((int64_t)c.n * (c.dim == 2 ? T2 : T3), kBlock), a.remap));
   if (c.dim == 2)
@@ -968,6 +1043,227 @@ int pair_offsets(sph_ws* ws, int bank, hipStream_t st) {
   SPH_LAUNCH(ws, "scan_pairs", st,
              (k_scan_single<int64_t, int64_t><<<1, kScanThreads, 0, st>>>(
                  ws->cnt64, n, ws->off64, nullptr, nullptr, -1)));
+  return SPH_OK;
+}
+
+// ------------------------------------------------------------ ensembles
+// Batched forms of the skin-step phases above (skin_begin, skin_build,
+// skin_finish): one launch serves every member.  Per-member arguments are
+// those the single-simulation launches pass; the bodies are shared.
+struct EnsInteg {
+  const float *r, *u, *dudt, *dvdt;
+  float *r_out, *u_out, *v_out;
+  int n;
+  float dt, ctvf;
+  Grid g;
+  float box[3];
+  int32_t* cell_count;
+  const float4* posb;
+  float trigger_sq;
+  int32_t* flag;
+};
+
+struct EnsCells {
+  int32_t* count;
+  int64_t ncell;
+  int64_t* info;
+  const int64_t* prev;
+  int64_t capacity;
+};
+
+struct EnsBin {  // skin-grid binning, all gated on the member's rebuild flag
+  const float* r;
+  int n;
+  Grid g;
+  int32_t *cell_count, *pcell, *pslot, *flag, *cell_start;
+  int64_t ncell;
+  int32_t *unsorted, *perm, *iperm;
+};
+
+struct EnsReduce {
+  const int64_t* part;
+  int nblocks;
+  int64_t *tot_out, *max_out, *info;
+  int64_t max_occ;
+  const int32_t* gate;
+  int32_t* clear_flag;
+};
+
+struct EnsNL {
+  int n = 0;
+  EnsInteg* integ = nullptr;
+  EnsCells* cells = nullptr;
+  EnsBin* bin = nullptr;
+  BuildArgs* build = nullptr;
+  EnsReduce* red_build = nullptr;
+  EnsReduce* red_sweep = nullptr;
+};
+
+template <int DIM>
+__global__ void k_integrate_skin_ens(const EnsInteg* __restrict__ t,
+                                     const EnsBlock* __restrict__ map) {
+  const EnsBlock m = map[blockIdx.x];
+  if (m.sim < 0) return;
+  const EnsInteg& a = t[m.sim];
+  integrate_skin_body<DIM>(a.r, a.u, a.dudt, a.dvdt, a.r_out, a.u_out, a.v_out, a.n, a.dt,
+                           a.ctvf, a.g, a.box[0], a.box[1], a.box[2], a.cell_count, a.posb,
+                           a.trigger_sq, a.flag, m.blk);
+}
+
+// one workgroup per member
+__global__ __launch_bounds__(kCellsThreads) void k_cells_max_ens(const EnsCells* __restrict__ t) {
+  const EnsCells& a = t[blockIdx.x];
+  cells_max_body(a.count, a.ncell, a.info, a.prev, a.capacity);
+}
+
+template <int DIM>
+__global__ void k_hash_count_ens(const EnsBin* __restrict__ t, const EnsBlock* __restrict__ map) {
+  const EnsBlock m = map[blockIdx.x];
+  if (m.sim < 0) return;
+  const EnsBin& a = t[m.sim];
+  hash_count_body<DIM>(a.r, a.n, a.g, a.cell_count, a.pcell, a.pslot, a.flag, m.blk);
+}
+
+// one workgroup per member
+__global__ __launch_bounds__(kScanThreads) void k_scan_ens(const EnsBin* __restrict__ t) {
+  const EnsBin& a = t[blockIdx.x];
+  scan_single_body<int32_t, int32_t>(a.cell_count, a.ncell, a.cell_start, nullptr, nullptr, -1,
+                                     a.cell_count, a.flag);
+}
+
+__global__ void k_scatter_ens(const EnsBin* __restrict__ t, const EnsBlock* __restrict__ map) {
+  const EnsBlock m = map[blockIdx.x];
+  if (m.sim < 0) return;
+  const EnsBin& a = t[m.sim];
+  scatter_body(a.pcell, a.pslot, a.cell_start, a.unsorted, a.n, a.flag, m.blk);
+}
+
+__global__ __launch_bounds__(256) void k_cell_sort_ens(const EnsBin* __restrict__ t,
+                                                       const EnsBlock* __restrict__ map) {
+  const EnsBlock m = map[blockIdx.x];
+  if (m.sim < 0) return;
+  const EnsBin& a = t[m.sim];
+  cell_sort_body(a.cell_start, a.unsorted, a.perm, a.iperm, a.ncell, a.flag, m.blk);
+}
+
+template <int DIM, int TPP>
+__global__ __launch_bounds__(kBlock) void k_nl_build_ens(const BuildArgs* __restrict__ t,
+                                                         const EnsBlock* __restrict__ map) {
+  const EnsBlock m = map[blockIdx.x];
+  if (m.sim < 0) return;
+  nl_build_body<DIM, TPP, false>(t[m.sim], m.blk);
+}
+
+// one workgroup per member
+__global__ __launch_bounds__(kReduceThreads) void k_reduce_parts_ens(
+    const EnsReduce* __restrict__ t) {
+  const EnsReduce& a = t[blockIdx.x];
+  reduce_parts_body(a.part, a.nblocks, a.tot_out, a.max_out, a.info, a.max_occ, a.gate,
+                    a.clear_flag);
+}
+
+template <typename T>
+static int upload(const std::vector<T>& h, T** d) {
+  hipError_t e = hipMalloc((void**)d, sizeof(T) * h.size());
+  if (e != hipSuccess) return hip_fail(e, "ensemble tables");
+  e = hipMemcpy(*d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_fail(e, "ensemble tables");
+  return SPH_OK;
+}
+
+int ens_nl_create(const std::vector<EnsMember>& members, EnsNL** out) {
+  const int n = (int)members.size();
+  std::vector<EnsInteg> integ(n);
+  std::vector<EnsCells> cells(n);
+  std::vector<EnsBin> bin(n);
+  std::vector<BuildArgs> build(n);
+  std::vector<EnsReduce> rb(n), rs(n);
+  for (int k = 0; k < n; ++k) {
+    const EnsMember& m = members[k];
+    sph_ws* ws = m.ws;
+    const auto& c = ws->cfg;
+    auto& S = ws->sk;
+    const int64_t lane_blocks = cdiv((int64_t)c.n * ws->tpp, kBlock);
+    integ[k] = EnsInteg{m.in.r, m.in.u, m.in.dudt, m.in.dvdt, m.r_out, m.out.u, m.out.v, c.n,
+                        m.dt, m.ctvf, make_grid(c), {m.prm.box[0], m.prm.box[1], m.prm.box[2]},
+                        ws->cell_count, S.posb, S.trigger_sq, S.flag};
+    cells[k] = EnsCells{ws->cell_count, ws->ncell, m.info, m.prev_info,
+                        (int64_t)c.cell_capacity};
+    bin[k] = EnsBin{m.r_out, c.n, skin_grid(ws), S.cell_count, ws->pcell, ws->pslot, S.flag,
+                    S.b.cell_start, S.ncell, ws->unsorted, S.b.perm, S.b.iperm};
+    build[k] = skin_build_args(ws, m.info);
+    rb[k] = EnsReduce{ws->part_build, (int)lane_blocks, nullptr, &m.info[7], m.info, -1, S.flag,
+                      nullptr};
+    rs[k] = EnsReduce{ws->part_sweep, (int)lane_blocks, &m.info[1], &m.info[3], m.info,
+                      m.max_occupancy, nullptr, S.flag};
+  }
+  EnsNL* t = new EnsNL;
+  t->n = n;
+  int rc = upload(integ, &t->integ);
+  if (!rc) rc = upload(cells, &t->cells);
+  if (!rc) rc = upload(bin, &t->bin);
+  if (!rc) rc = upload(build, &t->build);
+  if (!rc) rc = upload(rb, &t->red_build);
+  if (!rc) rc = upload(rs, &t->red_sweep);
+  if (rc) {
+    ens_nl_destroy(t);
+    return rc;
+  }
+  *out = t;
+  return SPH_OK;
+}
+
+void ens_nl_destroy(EnsNL* t) {
+  if (!t) return;
+  (void)hipFree(t->integ);
+  (void)hipFree(t->cells);
+  (void)hipFree(t->bin);
+  (void)hipFree(t->build);
+  (void)hipFree(t->red_build);
+  (void)hipFree(t->red_sweep);
+  delete t;
+}
+
+int ens_skin_begin(EnsNL* t, int dim, const EnsMap& part, const EnsMap& cells, sph_ws* prof,
+                   hipStream_t st) {
+  if (dim == 2)
+    SPH_LAUNCH(prof, "ens_integrate_skin", st,
+               k_integrate_skin_ens<2><<<part.blocks, kBlock, 0, st>>>(t->integ, part.d));
+  else
+    SPH_LAUNCH(prof, "ens_integrate_skin", st,
+               k_integrate_skin_ens<3><<<part.blocks, kBlock, 0, st>>>(t->integ, part.d));
+  SPH_LAUNCH(prof, "ens_cells_max", st,
+             k_cells_max_ens<<<t->n, kCellsThreads, 0, st>>>(t->cells));
+  if (dim == 2)
+    SPH_LAUNCH(prof, "ens_skin_hash", st,
+               k_hash_count_ens<2><<<part.blocks, kBlock, 0, st>>>(t->bin, part.d));
+  else
+    SPH_LAUNCH(prof, "ens_skin_hash", st,
+               k_hash_count_ens<3><<<part.blocks, kBlock, 0, st>>>(t->bin, part.d));
+  SPH_LAUNCH(prof, "ens_skin_scan", st, k_scan_ens<<<t->n, kScanThreads, 0, st>>>(t->bin));
+  SPH_LAUNCH(prof, "ens_skin_scatter", st,
+             k_scatter_ens<<<part.blocks, kBlock, 0, st>>>(t->bin, part.d));
+  SPH_LAUNCH(prof, "ens_skin_sort", st,
+             k_cell_sort_ens<<<cells.blocks, 256, 0, st>>>(t->bin, cells.d));
+  return SPH_OK;
+}
+
+int ens_skin_build(EnsNL* t, int dim, const EnsMap& lanes, sph_ws* prof, hipStream_t st) {
+  constexpr int T2 = tpp_of(2), T3 = tpp_of(3);
+  if (dim == 2)
+    SPH_LAUNCH(prof, "ens_skin_build", st,
+               (k_nl_build_ens<2, T2><<<lanes.blocks, kBlock, 0, st>>>(t->build, lanes.d)));
+  else
+    SPH_LAUNCH(prof, "ens_skin_build", st,
+               (k_nl_build_ens<3, T3><<<lanes.blocks, kBlock, 0, st>>>(t->build, lanes.d)));
+  SPH_LAUNCH(prof, "ens_reduce_skin", st,
+             k_reduce_parts_ens<<<t->n, kReduceThreads, 0, st>>>(t->red_build));
+  return SPH_OK;
+}
+
+int ens_skin_finish(EnsNL* t, sph_ws* prof, hipStream_t st) {
+  SPH_LAUNCH(prof, "ens_reduce_pairs", st,
+             k_reduce_parts_ens<<<t->n, kReduceThreads, 0, st>>>(t->red_sweep));
   return SPH_OK;
 }
 

@@ -11,6 +11,7 @@
 #include <cmath>
 #include <string>
 
+#include "ens.h"
 #include "sph_device.h"
 #include "sph_internal.h"
 #include "sweep.h"
@@ -57,9 +58,9 @@ __device__ __forceinline__ V3 mat_vec(const SweepArgs& a, int s, V3 x) {
 }
 
 template <int DIM, int KIND>
-__global__ __launch_bounds__(kB) void k_delta_L(SweepArgs a) {
+__device__ __forceinline__ void delta_L_run(const SweepArgs& a, const int blk) {
   constexpr int TPP = tpp_of(DIM);
-  const int tid = logical_block(a.remap) * blockDim.x + threadIdx.x;
+  const int tid = blk * blockDim.x + threadIdx.x;
   const int s = tid / TPP, h = tid % TPP;
   if (s >= a.n) return;
   const sph_solver_params& p = a.prm;
@@ -105,14 +106,15 @@ __global__ __launch_bounds__(kB) void k_delta_L(SweepArgs a) {
   }
   store_mat<DIM>(a, s, L);
 }
+SPH_SWEEP_KERNELS(k_delta_L, delta_L_run, (int DIM, int KIND), (DIM, KIND))
 
 // Density-gradient terms of solver.py:60-66:
 //   grad_i = L_i sum_j (rho_j - rho_i) grad W_ij V_j
 //   grad_j = sum_j (rho_i - rho_j) (-L_j) (-grad W_ij V_i)
 template <int DIM, int KIND>
-__global__ __launch_bounds__(kB) void k_delta_grad(SweepArgs a) {
+__device__ __forceinline__ void delta_grad_run(const SweepArgs& a, const int blk) {
   constexpr int TPP = tpp_of(DIM);
-  const int tid = logical_block(a.remap) * blockDim.x + threadIdx.x;
+  const int tid = blk * blockDim.x + threadIdx.x;
   const int s = tid / TPP, h = tid % TPP;
   if (s >= a.n) return;
   const sph_solver_params& p = a.prm;
@@ -138,14 +140,15 @@ __global__ __launch_bounds__(kB) void k_delta_grad(SweepArgs a) {
   a.R.DG0[s] = f4(mat_vec<DIM>(a, s, si), 0.0f);
   a.R.DG1[s] = f4(gj, 0.0f);
 }
+SPH_SWEEP_KERNELS(k_delta_grad, delta_grad_run, (int DIM, int KIND), (DIM, KIND))
 
 // ------------------------------------------------------------- renormalisation
 // solver.py:167-173 after a continuity update: rho = sum m_j W / min(sum (m/rho)_j W, 1)
 // over all particles, then the EOS; writes the U3/Q3 records.
 template <int DIM, int KIND>
-__global__ __launch_bounds__(kB) void k_renorm(SweepArgs a) {
+__device__ __forceinline__ void renorm_run(const SweepArgs& a, const int blk) {
   constexpr int TPP = tpp_of(DIM);
-  const int tid = logical_block(a.remap) * blockDim.x + threadIdx.x;
+  const int tid = blk * blockDim.x + threadIdx.x;
   const int s = tid / TPP, h = tid % TPP;
   if (s >= a.n) return;
   const sph_solver_params& p = a.prm;
@@ -170,15 +173,16 @@ __global__ __launch_bounds__(kB) void k_renorm(SweepArgs a) {
   a.R.U3[s] = make_float4(Ui.x, Ui.y, Ui.z, rho);
   a.R.Q3[s] = make_float4(eos.p_fn(rho), Qi.y, Qi.z, Qi.w);
 }
+SPH_SWEEP_KERNELS(k_renorm, renorm_run, (int DIM, int KIND), (DIM, KIND))
 
 // ------------------------------------------------------------- walls
 // MODE 0: generalized wall BC of the SPH solver (solver.py:431-530)
 // MODE 1: heat BC of the Riemann solver (solver.py:554-565)
 // followed by T += dt * dTdt when heat conduction is on (solver.py:834).
 template <int DIM, int KIND, int MODE>
-__global__ __launch_bounds__(kB) void k_walls(SweepArgs a) {
+__device__ __forceinline__ void walls_run(const SweepArgs& a, const int blk) {
   constexpr int TPP = tpp_of(DIM);
-  const int tid = logical_block(a.remap) * blockDim.x + threadIdx.x;
+  const int tid = blk * blockDim.x + threadIdx.x;
   const int s = tid / TPP, h = tid % TPP;
   if (s >= a.n) return;  // whole lane groups leave together
   const sph_solver_params& p = a.prm;
@@ -241,6 +245,7 @@ __global__ __launch_bounds__(kB) void k_walls(SweepArgs a) {
   a.R.V2[s] = f4(v, Vi.w);
   a.R.Q2[s] = make_float4(pr, Qi.y, Qi.z, T);
 }
+SPH_SWEEP_KERNELS(k_walls, walls_run, (int DIM, int KIND, int MODE), (DIM, KIND, MODE))
 
 // ------------------------------------------------------------- forces
 // SOLVER 0: acceleration_standard_fn (solver.py:221-256) + transport velocity
@@ -249,9 +254,9 @@ __global__ __launch_bounds__(kB) void k_walls(SweepArgs a) {
 // Plus artificial viscosity (solver.py:404-428), heat (solver.py:591-610),
 // dudt + g_ext (solver.py:936) and the case boundary hook (integrator.py:54).
 template <int DIM, int KIND, int SOLVER, bool TVF>
-__global__ __launch_bounds__(kB) void k_force(SweepArgs a) {
+__device__ __forceinline__ void force_run(const SweepArgs& a, const int blk) {
   constexpr int TPP = tpp_of(DIM);
-  const int tid = logical_block(a.remap) * blockDim.x + threadIdx.x;
+  const int tid = blk * blockDim.x + threadIdx.x;
   const int s = tid / TPP, h = tid % TPP;
   if (s >= a.n) return;  // whole lane groups leave together
   const sph_solver_params& p = a.prm;
@@ -405,6 +410,8 @@ __global__ __launch_bounds__(kB) void k_force(SweepArgs a) {
   if (o.T) o.T[i] = Ti;
   if (o.dTdt) o.dTdt[i] = heat ? dT : (a.in.dTdt ? a.in.dTdt[i] : 0.0f);
 }
+SPH_SWEEP_KERNELS(k_force, force_run, (int DIM, int KIND, int SOLVER, bool TVF),
+                  (DIM, KIND, SOLVER, TVF))
 
 // ------------------------------------------------------------- integrate
 // si_euler kick + transport velocity + drift + periodic wrap (integrator.py:26-30)
@@ -444,69 +451,129 @@ __global__ void k_kernel_eval(int kind, float ih, float sg, const float* __restr
 
 using namespace sph;
 
-namespace {
+namespace sph {
 
+// Calls f(name, kernel, ensemble kernel) for each sweep of the plan, in order;
+// stops at the first non-zero return.
+template <int DIM, int KIND, typename F>
+int for_plan(const SweepPlan& pl, F&& f) {
+  int rc = 0;
+  switch (pl.density) {
+    case 0: rc = f("density", k_density<DIM, KIND, 0>, k_density_ens<DIM, KIND, 0>); break;
+    case 1: rc = f("density", k_density<DIM, KIND, 1>, k_density_ens<DIM, KIND, 1>); break;
+    case 3:
+      if ((rc = f("delta_L", k_delta_L<DIM, KIND>, k_delta_L_ens<DIM, KIND>))) return rc;
+      if ((rc = f("delta_grad", k_delta_grad<DIM, KIND>, k_delta_grad_ens<DIM, KIND>))) return rc;
+      rc = f("density", k_density<DIM, KIND, 3>, k_density_ens<DIM, KIND, 3>);
+      break;
+    default: rc = f("density", k_density<DIM, KIND, 2>, k_density_ens<DIM, KIND, 2>); break;
+  }
+  if (rc) return rc;
+  if (pl.renorm && (rc = f("renorm", k_renorm<DIM, KIND>, k_renorm_ens<DIM, KIND>))) return rc;
+  if (pl.walls == 0) rc = f("walls", k_walls<DIM, KIND, 0>, k_walls_ens<DIM, KIND, 0>);
+  if (pl.walls == 1) rc = f("walls", k_walls<DIM, KIND, 1>, k_walls_ens<DIM, KIND, 1>);
+  if (rc) return rc;
+  if (pl.force == 0)
+    return pl.tvf ? f("force", k_force<DIM, KIND, 0, true>, k_force_ens<DIM, KIND, 0, true>)
+                  : f("force", k_force<DIM, KIND, 0, false>, k_force_ens<DIM, KIND, 0, false>);
+  if (pl.force == 2)
+    return pl.tvf ? f("force", k_force<DIM, KIND, 2, true>, k_force_ens<DIM, KIND, 2, true>)
+                  : f("force", k_force<DIM, KIND, 2, false>, k_force_ens<DIM, KIND, 2, false>);
+  return f("force", k_force<DIM, KIND, 1, false>, k_force_ens<DIM, KIND, 1, false>);
+}
 
-template <int DIM, int KIND>
-int launch_sweeps(sph_ws* ws, SweepArgs& a, hipStream_t st) {
-  const int n = a.n;
-  dim3 grid(xcd_grid(cdivl((int64_t)n * tpp_of(DIM), kB), a.remap));
-  const auto& p = a.prm;
-  // density (solver.py:750-796)
-  if (!p.density_evolution) {
-    SPH_LAUNCH(ws, "density", st, (k_density<DIM, KIND, 0><<<grid, kB, 0, st>>>(a)));
-  } else if (p.solver == 0) {
-    SPH_LAUNCH(ws, "density", st, (k_density<DIM, KIND, 1><<<grid, kB, 0, st>>>(a)));
-  } else if (p.solver == 2) {
-    SPH_LAUNCH(ws, "delta_L", st, (k_delta_L<DIM, KIND><<<grid, kB, 0, st>>>(a)));
-    SPH_LAUNCH(ws, "delta_grad", st, (k_delta_grad<DIM, KIND><<<grid, kB, 0, st>>>(a)));
-    SPH_LAUNCH(ws, "density", st, (k_density<DIM, KIND, 3><<<grid, kB, 0, st>>>(a)));
-  } else {
-    SPH_LAUNCH(ws, "density", st, (k_density<DIM, KIND, 2><<<grid, kB, 0, st>>>(a)));
+template <int DIM, typename F>
+int for_plan_kind(const SweepPlan& pl, F&& f) {
+  switch (pl.kind) {
+    case SPH_KERNEL_QSK: return for_plan<DIM, SPH_KERNEL_QSK>(pl, f);
+    case SPH_KERNEL_WC2K: return for_plan<DIM, SPH_KERNEL_WC2K>(pl, f);
+    default: return for_plan<DIM, -1>(pl, f);
   }
-  a.Ud = a.R.U1; a.Qd = a.R.Q1;
-  if (p.density_evolution && p.rho_renorm) {  // solver.py:755, 770, 792
-    SPH_LAUNCH(ws, "renorm", st, (k_renorm<DIM, KIND><<<grid, kB, 0, st>>>(a)));
-    a.Ud = a.R.U3; a.Qd = a.R.Q3;
+}
+
+// ------------------------------------------------------------ ensembles
+// Per-member gather and sweep arguments, exactly those sph_step passes on
+// bank 2 (the gather reads the step's new state; the sweeps count pairs).
+struct EnsRHS {
+  int n = 0;
+  GatherArgs* gather = nullptr;
+  SweepArgs* sweep = nullptr;
+};
+
+int ens_rhs_create(const std::vector<EnsMember>& members, EnsRHS** out) {
+  const int n = (int)members.size();
+  std::vector<GatherArgs> g(n);
+  std::vector<SweepArgs> s(n);
+  for (int k = 0; k < n; ++k) {
+    const EnsMember& m = members[k];
+    sph_ws* ws = m.ws;
+    SweepArgs a = make_args(ws, kSkinBank, &m.prm, &m.in, const_cast<sph_state_out*>(&m.out));
+    a.bc = m.bc;
+    g[k] = GatherArgs{a, m.r_out, m.out.u, m.out.v, ws->sk.b.pos, ws->sk.flag, ws->sk.posb};
+    a.in.r = m.r_out;
+    a.in.u = m.out.u;
+    a.in.v = m.out.v;
+    a.info = m.info;
+    select_records(a, plan_of(m.prm));
+    s[k] = a;
   }
-  // walls (solver.py:806-828)
-  const bool gwbc = p.bc_trick && (p.solver == 0 || p.solver == 2);
-  const bool heat_bc = p.bc_trick && p.solver == 1 && p.heat_conduction;
-  if (gwbc) {
-    SPH_LAUNCH(ws, "walls", st, (k_walls<DIM, KIND, 0><<<grid, kB, 0, st>>>(a)));
-  } else if (heat_bc) {
-    SPH_LAUNCH(ws, "walls", st, (k_walls<DIM, KIND, 1><<<grid, kB, 0, st>>>(a)));
+  EnsRHS* t = new EnsRHS;
+  t->n = n;
+  hipError_t e = hipMalloc((void**)&t->gather, sizeof(GatherArgs) * n);
+  if (e == hipSuccess) e = hipMalloc((void**)&t->sweep, sizeof(SweepArgs) * n);
+  if (e == hipSuccess)
+    e = hipMemcpy(t->gather, g.data(), sizeof(GatherArgs) * n, hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(t->sweep, s.data(), sizeof(SweepArgs) * n, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    ens_rhs_destroy(t);
+    return hip_fail(e, "ensemble tables");
   }
-  if (gwbc || heat_bc) {
-    a.Uf = a.R.U2; a.Vf = a.R.V2; a.Qf = a.R.Q2;
-  } else {
-    a.Uf = a.Ud; a.Vf = a.R.V0; a.Qf = a.Qd;
-  }
-  if (p.solver == 0) {
-    if (p.tvf_stress)
-      SPH_LAUNCH(ws, "force", st, (k_force<DIM, KIND, 0, true><<<grid, kB, 0, st>>>(a)));
-    else
-      SPH_LAUNCH(ws, "force", st, (k_force<DIM, KIND, 0, false><<<grid, kB, 0, st>>>(a)));
-  } else if (p.solver == 2) {
-    if (p.tvf_stress)
-      SPH_LAUNCH(ws, "force", st, (k_force<DIM, KIND, 2, true><<<grid, kB, 0, st>>>(a)));
-    else
-      SPH_LAUNCH(ws, "force", st, (k_force<DIM, KIND, 2, false><<<grid, kB, 0, st>>>(a)));
-  } else {
-    SPH_LAUNCH(ws, "force", st, (k_force<DIM, KIND, 1, false><<<grid, kB, 0, st>>>(a)));
-  }
+  *out = t;
   return SPH_OK;
 }
 
-template <int DIM>
-int launch_kind(sph_ws* ws, SweepArgs& a, hipStream_t st) {
-  switch (a.prm.kernel) {
-    case SPH_KERNEL_QSK: return launch_sweeps<DIM, SPH_KERNEL_QSK>(ws, a, st);
-    case SPH_KERNEL_WC2K: return launch_sweeps<DIM, SPH_KERNEL_WC2K>(ws, a, st);
-    default: return launch_sweeps<DIM, -1>(ws, a, st);
-  }
+void ens_rhs_destroy(EnsRHS* t) {
+  if (!t) return;
+  (void)hipFree(t->gather);
+  (void)hipFree(t->sweep);
+  delete t;
 }
 
+int ens_gather(EnsRHS* t, int dim, const EnsMap& part, sph_ws* prof, hipStream_t st) {
+  if (dim == 2)
+    SPH_LAUNCH(prof, "ens_gather_state", st,
+               k_gather_state_ens<2><<<part.blocks, kB, 0, st>>>(t->gather, part.d));
+  else
+    SPH_LAUNCH(prof, "ens_gather_state", st,
+               k_gather_state_ens<3><<<part.blocks, kB, 0, st>>>(t->gather, part.d));
+  return SPH_OK;
+}
+
+int ens_sweeps(EnsRHS* t, int dim, const SweepPlan& plan, const EnsMap& lanes, sph_ws* prof,
+               hipStream_t st) {
+  auto f = [&](const char* name, SweepKernel, EnsSweepKernel k) -> int {
+    SPH_LAUNCH(prof, name, st, (k<<<lanes.blocks, kB, 0, st>>>(t->sweep, lanes.d)));
+    return 0;
+  };
+  return dim == 2 ? for_plan_kind<2>(plan, f) : for_plan_kind<3>(plan, f);
+}
+
+}  // namespace sph
+
+namespace {
+
+// The sweeps of one list (density ... force), one launch each.
+template <int DIM>
+int launch_kind(sph_ws* ws, SweepArgs& a, hipStream_t st) {
+  const SweepPlan pl = plan_of(a.prm);
+  select_records(a, pl);
+  const dim3 grid(xcd_grid(cdivl((int64_t)a.n * tpp_of(DIM), kB), a.remap));
+  return for_plan_kind<DIM>(pl, [&](const char* name, SweepKernel k, EnsSweepKernel) -> int {
+    SPH_LAUNCH(ws, name, st, (k<<<grid, kB, 0, st>>>(a)));
+    return 0;
+  });
+}
 
 }  // namespace
 

@@ -271,6 +271,12 @@ __host__ __device__ inline unsigned xcd_grid(long long blocks, int remap) {
   return (unsigned)(remap ? (blocks + kXcds - 1) / kXcds * kXcds : blocks);
 }
 
+// Batched launches over the members of an ensemble: workgroup b of the launch
+// serves block `blk` of member `sim` (sim < 0 pads the grid; it exits at once).
+struct EnsBlock {
+  int32_t sim, blk;
+};
+
 // tags (utils.py:22-32)
 constexpr int kFluid = 0, kSolidWall = 1, kMovingWall = 2, kDirichletWall = 3;
 __device__ __forceinline__ bool is_wall(int t) { return t >= 1 && t <= 3; }

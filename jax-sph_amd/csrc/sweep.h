@@ -198,10 +198,13 @@ __device__ __forceinline__ V3 lane_sum(V3 v) {
 // (nullable) receives the sorted positions of the pair build; with a gate it
 // is written only when *gate != 0, together with posb (original order).
 template <int DIM>
-__global__ void k_gather_state(SweepArgs a, const float* __restrict__ r, const float* __restrict__ u,
-                               const float* __restrict__ v, float4* __restrict__ nl_pos,
-                               const int32_t* __restrict__ gate, float4* __restrict__ posb) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void gather_state_run(const SweepArgs& a, const float* __restrict__ r,
+                                                 const float* __restrict__ u,
+                                                 const float* __restrict__ v,
+                                                 float4* __restrict__ nl_pos,
+                                                 const int32_t* __restrict__ gate,
+                                                 float4* __restrict__ posb, const int blk) {
+  int s = blk * blockDim.x + threadIdx.x;
   if (s >= a.n) return;
   const int i = a.perm[s];
   const sph_state_in& in = a.in;
@@ -218,6 +221,49 @@ __global__ void k_gather_state(SweepArgs a, const float* __restrict__ r, const f
     V3 n = in.nw ? load_vec<DIM>(in.nw, i) : V3{0.f, 0.f, 0.f};
     a.R.NW[s] = f4(n, in.Cp ? in.Cp[i] : 0.0f);
   }
+}
+
+template <int DIM>
+__global__ void k_gather_state(SweepArgs a, const float* __restrict__ r, const float* __restrict__ u,
+                               const float* __restrict__ v, float4* __restrict__ nl_pos,
+                               const int32_t* __restrict__ gate, float4* __restrict__ posb) {
+  gather_state_run<DIM>(a, r, u, v, nl_pos, gate, posb, blockIdx.x);
+}
+
+// The single-list kernel NAME and its ensemble form NAME_ens around a sweep
+// body RUN<TARGS>(args, blk).  TPARAMS and TARGS are parenthesised template
+// parameter and argument lists.
+#define SPH_UNPAREN(...) __VA_ARGS__
+#define SPH_SWEEP_KERNELS(NAME, RUN, TPARAMS, TARGS)                                      \
+  template <SPH_UNPAREN TPARAMS>                                                          \
+  __global__ __launch_bounds__(kB) void NAME(SweepArgs a) {                               \
+    RUN<SPH_UNPAREN TARGS>(a, logical_block(a.remap));                                    \
+  }                                                                                       \
+  template <SPH_UNPAREN TPARAMS>                                                          \
+  __global__ __launch_bounds__(kB) void NAME##_ens(const SweepArgs* __restrict__ t,       \
+                                                   const EnsBlock* __restrict__ map) {    \
+    const EnsBlock m = map[blockIdx.x];                                                   \
+    if (m.sim < 0) return;                                                                \
+    RUN<SPH_UNPAREN TARGS>(t[m.sim], m.blk);                                              \
+  }
+
+// Arguments of one ensemble member's gather: the sweep arguments plus the
+// step's new state and, for a list rebuild, the build positions.
+struct GatherArgs {
+  SweepArgs a;
+  const float *r, *u, *v;
+  float4* nl_pos;
+  const int32_t* gate;
+  float4* posb;
+};
+
+template <int DIM>
+__global__ void k_gather_state_ens(const GatherArgs* __restrict__ t,
+                                   const EnsBlock* __restrict__ map) {
+  const EnsBlock m = map[blockIdx.x];
+  if (m.sim < 0) return;
+  const GatherArgs& g = t[m.sim];
+  gather_state_run<DIM>(g.a, g.r, g.u, g.v, g.nl_pos, g.gate, g.posb, m.blk);
 }
 
 template <int DIM>
@@ -359,10 +405,12 @@ __device__ __forceinline__ int density_of(const SweepArgs& a, const int s, const
   return npair;
 }
 
+// blk: the workgroup's block within this list's grid; it also indexes the
+// workgroup's partials.
 template <int DIM, int KIND, int MODE>
-__global__ __launch_bounds__(kB) void k_density(SweepArgs a) {
+__device__ __forceinline__ void density_run(const SweepArgs& a, const int blk) {
   constexpr int TPP = tpp_of(DIM);
-  const int tid = logical_block(a.remap) * blockDim.x + threadIdx.x;
+  const int tid = blk * blockDim.x + threadIdx.x;
   const int s = tid / TPP, h = tid % TPP;
   int npair = 0;
   if (s < a.n) npair = density_of<DIM, KIND, MODE>(a, s, h);  // lane groups stay together
@@ -387,11 +435,58 @@ __global__ __launch_bounds__(kB) void k_density(SweepArgs a) {
     unsigned long long t = 0;
     int m = 0;
     for (int w = 0; w < kB / 64; ++w) { t += btot[w]; m = bmax[w] > m ? bmax[w] : m; }
-    a.part[2 * blockIdx.x] = (int64_t)t;  // reduced by one workgroup afterwards
-    a.part[2 * blockIdx.x + 1] = m;
+    a.part[2 * blk] = (int64_t)t;  // reduced by one workgroup afterwards
+    a.part[2 * blk + 1] = m;
   }
 }
 
+SPH_SWEEP_KERNELS(k_density, density_run, (int DIM, int KIND, int MODE), (DIM, KIND, MODE))
+
+
+// ------------------------------------------------------------- sweep plans
+// The sweeps of one forward evaluation as fixed by the solver parameters
+// (solver.py:750-928): density variant, renormalisation, wall treatment,
+// force variant, and the kernel template the sweeps are compiled for.
+struct SweepPlan {
+  int density;  // k_density MODE; MODE 3 runs k_delta_L and k_delta_grad first
+  bool renorm;  // k_renorm after a continuity update
+  int walls;    // -1: none, else k_walls MODE
+  int force;    // k_force SOLVER
+  bool tvf;     // k_force TVF
+  int kind;     // SPH_KERNEL_QSK, SPH_KERNEL_WC2K, or -1 (runtime switch)
+  bool operator==(const SweepPlan& o) const {
+    return density == o.density && renorm == o.renorm && walls == o.walls &&
+           force == o.force && tvf == o.tvf && kind == o.kind;
+  }
+};
+
+inline SweepPlan plan_of(const sph_solver_params& p) {
+  SweepPlan pl;
+  pl.density = !p.density_evolution ? 0 : p.solver == 0 ? 1 : p.solver == 2 ? 3 : 2;
+  pl.renorm = p.density_evolution && p.rho_renorm;  // solver.py:755, 770, 792
+  const bool gwbc = p.bc_trick && (p.solver == 0 || p.solver == 2);        // solver.py:806
+  const bool heat_bc = p.bc_trick && p.solver == 1 && p.heat_conduction;  // solver.py:822
+  pl.walls = gwbc ? 0 : heat_bc ? 1 : -1;
+  pl.force = p.solver;
+  pl.tvf = p.solver != 1 && p.tvf_stress;
+  pl.kind = (p.kernel == SPH_KERNEL_QSK || p.kernel == SPH_KERNEL_WC2K) ? p.kernel : -1;
+  return pl;
+}
+
+// The records the walls sweep (Ud, Qd) and the force sweep (Uf, Vf, Qf) read;
+// the other sweeps read fixed records.
+inline void select_records(SweepArgs& a, const SweepPlan& pl) {
+  a.Ud = pl.renorm ? a.R.U3 : a.R.U1;
+  a.Qd = pl.renorm ? a.R.Q3 : a.R.Q1;
+  if (pl.walls >= 0) {
+    a.Uf = a.R.U2; a.Vf = a.R.V2; a.Qf = a.R.Q2;
+  } else {
+    a.Uf = a.Ud; a.Vf = a.R.V0; a.Qf = a.Qd;
+  }
+}
+
+using SweepKernel = void (*)(SweepArgs);
+using EnsSweepKernel = void (*)(const SweepArgs*, const EnsBlock*);
 
 // ------------------------------------------------------------- host set-up
 inline Geo make_geo(const float* box) {

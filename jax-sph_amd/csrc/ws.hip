@@ -263,6 +263,14 @@ extern "C" int sph_ws_tune(sph_ws* ws, const char* key, int64_t value) {
     ws->xcd_remap = value ? 1 : 0;
     return SPH_OK;
   }
+  if (std::strcmp(key, "skin_rebuild") == 0) {
+    // the next fused step rebuilds the skin lists, as the first step after
+    // sph_skin_enable does (synchronous; not for use under graph capture)
+    if (!ws->sk.on) { set_error("sph_ws_tune: skin lists are not enabled"); return SPH_ERR_ARG; }
+    const int32_t one = value ? 1 : 0;
+    hipError_t e = hipMemcpy(ws->sk.flag, &one, sizeof(one), hipMemcpyHostToDevice);
+    return e == hipSuccess ? SPH_OK : hip_fail(e, "sph_ws_tune");
+  }
   set_error(std::string("sph_ws_tune: unknown key ") + key);
   return SPH_ERR_ARG;
 }

@@ -75,6 +75,14 @@ class CotangentOut(Structure):
     _fields_ = [("r", c_void_p), ("u", c_void_p), ("a", c_void_p)]
 
 
+class EnsembleMember(Structure):
+    """One member of sph_ensemble_create: the arguments of its sph_step on bank 2."""
+    _fields_ = [("ws", c_void_p), ("prm", POINTER(SolverParams)), ("dt", c_float),
+                ("tvf_coef", c_float), ("bc", POINTER(BCSpec)), ("in_", POINTER(StateIn)),
+                ("r_out", c_void_p), ("out", POINTER(StateOut)), ("prev_info", c_void_p),
+                ("max_occupancy", c_int64), ("info", c_void_p)]
+
+
 class SphError(RuntimeError):
     """A libsph call returned an error code."""
 
@@ -119,6 +127,11 @@ def lib():
                                   POINTER(c_float), vp]),
         "sph_step_vjp": (c_int, [vp, c_int, POINTER(SolverParams), c_float, POINTER(StateIn),
                                  POINTER(CotangentIn), POINTER(CotangentOut), c_int32, vp]),
+        "sph_ensemble_create": (c_int, [POINTER(EnsembleMember), c_int32, POINTER(c_void_p)]),
+        "sph_ensemble_step": (c_int, [vp, vp]),
+        "sph_ensemble_tune": (c_int, [vp, c_char_p, c_int64]),
+        "sph_ensemble_info": (c_int, [vp, i64p, i64p, i64p]),
+        "sph_ensemble_destroy": (c_int, [vp]),
         "sph_prof_enable": (c_int, [vp, c_int]),
         "sph_prof_reset": (c_int, [vp]),
         "sph_prof_read": (c_int, [vp, c_char_p, c_int, POINTER(c_double), i64p, c_int]),

@@ -123,4 +123,7 @@ def si_euler(tvf: float, model: Callable, shift_fn: Callable, bc_fn: Callable,
         return state, neighbors
 
     advance._sph_fusable = _fusable
+    # what an ensemble needs to issue this step itself (see ensemble.py)
+    advance._sph_solver = solver
+    advance._sph_tvf = tvf
     return advance
