@@ -47,6 +47,9 @@ def test_ctypes_struct_layout_matches_c():
              sizeof(sph_state_in), sizeof(sph_state_out), sizeof(sph_bc_spec));
       printf("%zu %zu %zu\n", offsetof(sph_nl_config, ell_width),
              offsetof(sph_solver_params, tvf_stress), offsetof(sph_state_in, tag));
+      printf("%zu %zu %zu %zu\n", sizeof(sph_ensemble_member),
+             offsetof(sph_ensemble_member, r_out), offsetof(sph_ensemble_member, max_occupancy),
+             offsetof(sph_ensemble_member, info));
       return 0;
     }
     '''
@@ -64,8 +67,11 @@ def test_ctypes_struct_layout_matches_c():
     assert sizes[:5] == [ctypes.sizeof(_lib.NLConfig), ctypes.sizeof(_lib.SolverParams),
                          ctypes.sizeof(_lib.StateIn), ctypes.sizeof(_lib.StateOut),
                          ctypes.sizeof(_lib.BCSpec)]
-    assert sizes[5:] == [_lib.NLConfig.ell_width.offset, _lib.SolverParams.tvf_stress.offset,
-                         _lib.StateIn.tag.offset]
+    assert sizes[5:8] == [_lib.NLConfig.ell_width.offset, _lib.SolverParams.tvf_stress.offset,
+                          _lib.StateIn.tag.offset]
+    M = _lib.EnsembleMember
+    assert sizes[8:] == [ctypes.sizeof(M), M.r_out.offset, M.max_occupancy.offset,
+                         M.info.offset]
 
 
 def test_product_package_never_imports_the_oracle():
