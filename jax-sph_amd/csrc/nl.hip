@@ -448,25 +448,32 @@ struct BuildArgs {
   int remap;              // XCD-contiguous block order (grid a multiple of 8)
 };
 
-// Lane h (of TPP) tests candidates lo + h, lo + h + TPP, ... < hi of one cell
-// range whose image is shifted by `img` (0 or +-box per axis).  Hits are
-// collected 32 at a time in a bit mask so the divergent stores happen once per
-// hit, in candidate order, into the lane's own sub-list.
-// The first kStage hits of each lane are staged in LDS (one column per
-// thread) and written out row by row at the end, so every pair-buffer row is
-// written by whole wavefronts at once (coalesced, no partially written lines
-// left for the L2 to evict); later hits go straight to the buffer.
+// Lane h (of TPP) tests candidates base + h, base + h + TPP, ... of one cell
+// range [lo, hi) whose image is shifted by `img`; the TPP lanes of a particle
+// step through the range together, 32 candidates each per chunk, and collect
+// their hits in a bit mask.  Hits are numbered over the particle's lanes (lane
+// 0's hits of the chunk first, then lane 1's, ...) and hit k goes to sub-list
+// k mod TPP at position k / TPP: the particle's sub-lists differ in length by
+// at most one, so the lanes that later sweep them finish together.
+// The first kStage entries of each sub-list are staged in LDS (in the column
+// of the thread that owns the sub-list) and written out row by row at the end,
+// so every pair-buffer row is written by whole wavefronts at once; later
+// entries go straight to the buffer.
 constexpr int kStage = 32;
 
 template <int DIM, int TPP, bool ORDERED>
 __device__ __forceinline__ void test_range(const BuildArgs& a, int s, int h, const float4 ps,
                                            int sid, int lo, int hi, const float* img,
-                                           int32_t* __restrict__ out, int32_t* stage,
+                                           int32_t* __restrict__ out0, int32_t* stage0,
                                            int& count) {
+  // out0: entry 0 of the particle's sub-list 0; stage0: the stage column of
+  // the particle's lane 0 (sub-list q is column stage0 + q)
   const float sx = ps.x - img[0], sy = ps.y - img[1], sz = ps.z - img[2];
   constexpr int G = 8;  // candidates whose loads are issued together
-  for (int c0 = lo + h; c0 < hi; c0 += 32 * TPP) {
-    const int nb = (hi - c0 + TPP - 1) / TPP < 32 ? (hi - c0 + TPP - 1) / TPP : 32;
+  for (int base = lo; base < hi; base += 32 * TPP) {  // same trip count on all TPP lanes
+    const int c0 = base + h;
+    int nb = c0 < hi ? (hi - c0 + TPP - 1) / TPP : 0;
+    nb = nb < 32 ? nb : 32;
     unsigned mask = 0u;
     for (int t0 = 0; t0 < nb; t0 += G) {
       float4 pg[G];
@@ -494,14 +501,26 @@ __device__ __forceinline__ void test_range(const BuildArgs& a, int s, int h, con
         mask |= hit ? (1u << t) : 0u;
       }
     }
+    // number this chunk's hits over the particle's lanes
+    const int nh = __popc(mask);
+    int incl = nh;
+#pragma unroll
+    for (int o = 1; o < TPP; o <<= 1) {
+      const int y = __shfl_up(incl, o, TPP);
+      if (h >= o) incl += y;
+    }
+    const int total = __shfl(incl, TPP - 1, TPP);
+    int k = count + incl - nh;
     while (mask) {
       const int t = __ffs(mask) - 1;
       mask &= mask - 1u;
       const int32_t b = c0 + t * TPP;
-      if (count < kStage) stage[count * kBlock] = b;
-      else if (count < a.lane_width) out[(int64_t)count * a.npad] = b;
-      ++count;
+      const int q = k % TPP, pos = k / TPP;
+      if (pos < kStage) stage0[pos * kBlock + q] = b;
+      else if (pos < a.lane_width) out0[((int64_t)q * a.lane_width + pos) * a.npad] = b;
+      ++k;
     }
+    count += total;
   }
 }
 
@@ -516,12 +535,12 @@ __device__ __forceinline__ void nl_build_body(const BuildArgs& a, const int blk)
   if (a.gate && *a.gate == 0) return;  // uniform over the list's grid
   const int tid = blk * blockDim.x + threadIdx.x;
   const int s = tid / TPP, h = tid % TPP;
-  int count = 0;
+  int count = 0;  // hits of the particle (the same in its TPP lanes)
   if (s < a.n) {
     const float4 ps = a.pos[s];
     const int sid = ORDERED ? a.perm[s] : 0;
-    int32_t* out = a.ell + ((int64_t)h * a.lane_width) * a.npad + s;
-    int32_t* stage = &stage_buf[0][threadIdx.x];
+    int32_t* out = a.ell + s;                           // sub-list 0 of s
+    int32_t* stage = &stage_buf[0][threadIdx.x - h];    // stage column of lane 0 of s
     float img[3] = {0.0f, 0.0f, 0.0f};
     if (!a.g.use_cells) {
       test_range<DIM, TPP, ORDERED>(a, s, h, ps, sid, 0, a.n, img, out, stage, count);
@@ -559,24 +578,29 @@ __device__ __forceinline__ void nl_build_body(const BuildArgs& a, const int blk)
         }
       }
     }
-    int staged = count < kStage ? count : kStage;
+  }
+  // the particle's other lanes wrote into this thread's stage column
+  __syncthreads();
+  // this lane's sub-list: hits h, h + TPP, ... of the particle
+  const int share = s < a.n ? (count + TPP - 1 - h) / TPP : 0;
+  if (s < a.n) {
+    int32_t* out = a.ell + ((int64_t)h * a.lane_width) * a.npad + s;
+    const int32_t* stage = &stage_buf[0][threadIdx.x];
+    int staged = share < kStage ? share : kStage;
     staged = staged < a.lane_width ? staged : a.lane_width;  // never past the sub-list
     for (int k = 0; k < staged; ++k) out[(int64_t)k * a.npad] = stage[k * kBlock];
-    a.cnt[h * a.npad + s] = count < a.lane_width ? count : a.lane_width;
-    if (count > a.lane_width) {
+    a.cnt[h * a.npad + s] = share < a.lane_width ? share : a.lane_width;
+    if (h == 0 && count > TPP * a.lane_width) {  // some sub-list of s was truncated
       atomicAdd((unsigned long long*)&a.info[5], 1ull);
       if (a.trunc_bits) atomicOr((unsigned long long*)&a.info[0], (unsigned long long)a.trunc_bits);
     }
   }
-  // pairs of the particle (sum over its TPP lanes), then block reductions:
-  // total pairs and the largest per-particle count
-  int tot_p = count;
-#pragma unroll
-  for (int o = 1; o < TPP; o <<= 1) tot_p += __shfl_xor(tot_p, o, 64);
+  // block reductions: total pairs (sum of the lanes' shares) and the largest
+  // per-particle count
   __shared__ unsigned long long tot[kBlock / 64];
   __shared__ int mx[kBlock / 64];
-  unsigned long long v = (unsigned long long)count;
-  int m = tot_p;
+  unsigned long long v = (unsigned long long)share;
+  int m = count;
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
     v += __shfl_down(v, o, 64);

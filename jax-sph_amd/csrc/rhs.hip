@@ -72,9 +72,9 @@ __device__ __forceinline__ void delta_L_run(const SweepArgs& a, const int blk) {
     const V3 dr = min_image<DIM>(Pi, Pj, a.geo);
     const float d2 = dist2<DIM>(dr.x, dr.y, dr.z);
     if (!in_cutoff<DIM>(a, d2, Pi, Pj)) return;
-    const float d = sqrtf(d2);
-    const float vj = Pj.w / a.R.U0[j].w;
-    const V3 kg = (K.dw(d * p.one_over_h) / (d + kEps) * vj) * dr;  // grad W_ij V_j
+    const float d = sqrt_fast(d2);
+    const float vj = Pj.w * rcp_fast(a.R.U0[j].w);
+    const V3 kg = (K.dw(d * p.one_over_h) * rcp_fast(d + kEps) * vj) * dr;  // grad W_ij V_j
     const float r[3] = {-dr.x, -dr.y, -dr.z}, g[3] = {kg.x, kg.y, kg.z};
 #pragma unroll
     for (int x = 0; x < DIM; ++x)
@@ -128,10 +128,10 @@ __device__ __forceinline__ void delta_grad_run(const SweepArgs& a, const int blk
     const V3 dr = min_image<DIM>(Pi, Pj, a.geo);
     const float d2 = dist2<DIM>(dr.x, dr.y, dr.z);
     if (!in_cutoff<DIM>(a, d2, Pi, Pj)) return;
-    const float d = sqrtf(d2);
+    const float d = sqrt_fast(d2);
     const float rhoj = a.R.U0[j].w;
-    const V3 kg = (K.dw(d * p.one_over_h) / (d + kEps)) * dr;
-    si = si + ((rhoj - rhoi) * (Pj.w / rhoj)) * kg;
+    const V3 kg = (K.dw(d * p.one_over_h) * rcp_fast(d + kEps)) * dr;
+    si = si + ((rhoj - rhoi) * (Pj.w * rcp_fast(rhoj))) * kg;
     gj = gj + ((rhoi - rhoj) * vi) * mat_vec<DIM>(a, j, kg);  // (-L_j)(-grad W V_i)
   });
   si = lane_sum<TPP>(si);
@@ -161,9 +161,9 @@ __device__ __forceinline__ void renorm_run(const SweepArgs& a, const int blk) {
     const V3 dr = min_image<DIM>(Pi, Pj, a.geo);
     const float d2 = dist2<DIM>(dr.x, dr.y, dr.z);
     if (!in_cutoff<DIM>(a, d2, Pi, Pj)) return;
-    const float w = K.w(sqrtf(d2) * p.one_over_h);
+    const float w = K.w(sqrt_fast(d2) * p.one_over_h);
     nom += Pj.w * w;
-    den += (Pj.w / a.R.U1[j].w) * w;
+    den += (Pj.w * rcp_fast(a.R.U1[j].w)) * w;
   });
   nom = lane_sum<TPP>(nom);
   den = lane_sum<TPP>(den);
@@ -203,7 +203,7 @@ __device__ __forceinline__ void walls_run(const SweepArgs& a, const int blk) {
       const V3 dr = min_image<DIM>(Pi, Pj, a.geo);
       const float d2 = dist2<DIM>(dr.x, dr.y, dr.z);
       if (!in_cutoff<DIM>(a, d2, Pi, Pj)) return;
-      const float d = sqrtf(d2);
+      const float d = sqrt_fast(d2);
       const float w = K.w(d * p.one_over_h);
       ws += w;
       if (MODE == 0) {
@@ -267,6 +267,7 @@ __device__ __forceinline__ void force_run(const SweepArgs& a, const int blk) {
   const int tag_i = (int)Qi.z;
   const float mi = Pi.w, rhoi = Ui.w, pi_ = Qi.x, etai = Qi.y, Ti = Qi.w;
   const float inv_mi = 1.0f / mi;
+  const float inv_rhoi = 1.0f / rhoi;
   const float Vi2 = (mi / rhoi) * (mi / rhoi);
   const V3 ui = v3(Ui), vi = v3(Vi);
   const V3 Ai = rhoi * ui;          // rho_i u_i
@@ -289,18 +290,18 @@ __device__ __forceinline__ void force_run(const SweepArgs& a, const int blk) {
     const float d2 = dist2<DIM>(dr.x, dr.y, dr.z);
     if (d2 == 0.0f) return;  // self pair / coincident: every term is exactly 0
     if (!in_cutoff<DIM>(a, d2, Pi, Pj)) return;
-    const float d = sqrtf(d2);
+    const float d = sqrt_fast(d2);
     const float q = d * p.one_over_h;
     const float gw = K.dw(q);
     const float4 Uj = a.Uf[j], Qj = a.Qf[j];
     const float mj = Pj.w, rhoj = Uj.w, pj = Qj.x, etaj = Qj.y;
     const V3 uj = v3(Uj);
-    const float inv_d = 1.0f / (d + kEps);
-    const float Vj = mj / rhoj;
+    const float inv_d = rcp_fast(d + kEps);
+    const float Vj = mj * rcp_fast(rhoj);
     const float cpre = (Vi2 + Vj * Vj) * inv_mi * gw * inv_d;
-    const float eta_ij = 2.0f * etai * etaj / (etai + etaj + kEps);
+    const float eta_ij = 2.0f * etai * etaj * rcp_fast(etai + etaj + kEps);
     if (SOLVER == 0 || SOLVER == 2) {
-      const float pij = (rhoj * pi_ + rhoi * pj) / (rhoi + rhoj);
+      const float pij = (rhoj * pi_ + rhoi * pj) * rcp_fast(rhoi + rhoj);
       V3 t = (-pij) * dr + eta_ij * (ui - uj);
       if (TVF) {
         const V3 vj = v3(a.Vf[j]);
@@ -311,7 +312,7 @@ __device__ __forceinline__ void force_run(const SweepArgs& a, const int blk) {
       if (SOLVER == 2 && (int)Qj.z == kFluid) {
         // delta-SPH velocity diffusion (solver.py:300-311)
         const float pij_d = dot<DIM>(uj - ui, (-1.0f) * dr) * inv_d * inv_d;
-        acc = acc + (Vj * pij_d * gw * inv_d * delta_pref / rhoi) * dr;
+        acc = acc + (Vj * pij_d * gw * inv_d * delta_pref * inv_rhoi) * dr;
       }
     } else {
       const int tag_j = (int)Qj.z;
@@ -346,7 +347,7 @@ __device__ __forceinline__ void force_run(const SweepArgs& a, const int blk) {
         beta = fminf(p.eta_limiter * fmaxf(uL - uR, 0.0f), p.c_ref);
       }
       const float Pstar = Pavg + 0.5f * rhoavg * (uL - uR) * beta;
-      const float irr = 1.0f / (rhoi * rhoj);
+      const float irr = rcp_fast(rhoi * rhoj);
       const V3 eq9 = (-2.0f * mj * (Pstar * irr)) * kg;
       float mask = 1.0f;
       if (p.bc_trick && p.free_slip) mask = fluid_i ? 1.0f : 0.0f;
@@ -360,14 +361,14 @@ __device__ __forceinline__ void force_run(const SweepArgs& a, const int blk) {
         const float rho_ab = (rhoi + rhoj) * 0.5f;
         const float num = mj * av_pref * dot<DIM>(ui - uj, dr);
         const float den = rho_ab * (d2 + av_h2);
-        acc = acc + (num * gw * inv_d / den) * dr;
+        acc = acc + (num * gw * inv_d * rcp_fast(den)) * dr;
       }
     }
     if (heat) {
       const float kj = a.Vf[j].w;
-      const float keff = (kappa_i * kj) / (kappa_i + kj);
-      const float F = dot<DIM>(dr, (gw * inv_d) * dr) / (d2 + kEps);
-      dT += 4.0f * mj * keff * (Ti - Qj.w) * F / (Cp_i * rhoi * rhoj);
+      const float keff = (kappa_i * kj) * rcp_fast(kappa_i + kj);
+      const float F = dot<DIM>(dr, (gw * inv_d) * dr) * rcp_fast(d2 + kEps);
+      dT += 4.0f * mj * keff * (Ti - Qj.w) * F * rcp_fast(Cp_i * rhoi * rhoj);
     }
   });
   acc = lane_sum<TPP>(acc);

@@ -68,6 +68,13 @@ __device__ __forceinline__ float disp_fast(float a, float b, float side, float i
   return (a - k * side) - b;
 }
 
+// Hardware reciprocal and square root (v_rcp_f32, v_sqrt_f32, about 1 ulp) for
+// the pair sums, whose results are held to the reference at float32 rounding
+// noise; the compiler's IEEE forms cost 5-10 instructions each.  The
+// neighbour predicate never divides or takes roots, so pair sets stay exact.
+__device__ __forceinline__ float rcp_fast(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float sqrt_fast(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 // x^2 + y^2 (+ z^2) with explicit fused multiply-adds (same bits everywhere).
 template <int DIM>
 __device__ __forceinline__ float dist2(float dx, float dy, float dz) {

@@ -147,29 +147,19 @@ __device__ __forceinline__ bool in_cutoff(const SweepArgs& a, float d2, float4 P
   return metric_sq_exact<DIM>(Pi, Pj, a.geo.box, a.geo.half) < a.cut_sq;
 }
 
-// Visits the pairs of sorted particle s.  Pair indices are fetched in batches
-// of 8 independent (lane-coalesced) loads so their latency is paid once per
-// batch; the body is unrolled by two so gathers of one pair overlap the
-// arithmetic of the other.
+// Visits the pairs of sorted particle s.  The next pair index (a
+// lane-coalesced load) is requested before the current pair's body runs, so
+// its latency hides behind the body; indices live in scalars, not an array a
+// rolled loop would have to index dynamically.
 template <typename F>
 __device__ __forceinline__ void for_each_pair(const SweepArgs& a, int s, int h, F&& f) {
-  constexpr int B = 8;
   const int c = a.cnt[h * a.npad + s];
   const int32_t* lst = a.ell + ((int64_t)h * a.lane_width) * a.npad + s;
-  int jb[B];
-#pragma unroll
-  for (int t = 0; t < B; ++t) jb[t] = (t < c) ? lst[(int64_t)t * a.npad] : -1;
-  for (int k0 = 0; k0 < c; k0 += B) {
-    // the next batch is requested before this one is consumed
-    int jn[B];
-#pragma unroll
-    for (int t = 0; t < B; ++t)
-      jn[t] = (k0 + B + t < c) ? lst[(int64_t)(k0 + B + t) * a.npad] : -1;
-#pragma unroll 2
-    for (int t = 0; t < B; ++t)
-      if (jb[t] >= 0) f(jb[t]);
-#pragma unroll
-    for (int t = 0; t < B; ++t) jb[t] = jn[t];
+  int jn = c > 0 ? lst[0] : 0;
+  for (int k = 0; k < c; ++k) {
+    const int j = jn;
+    if (k + 1 < c) jn = lst[(int64_t)(k + 1) * a.npad];
+    f(j);
   }
 }
 
@@ -304,25 +294,25 @@ __device__ __forceinline__ int density_of(const SweepArgs& a, const int s, const
     const float d2 = dist2<DIM>(dr.x, dr.y, dr.z);
     if (!in_cutoff<DIM>(a, d2, Pi, Pj)) return;
     ++npair;
-    const float d = sqrtf(d2);
+    const float d = sqrt_fast(d2);
     const float q = d * p.one_over_h;
     if (MODE == 0) {
       wsum += K.w(q);
     } else {
       const float4 Uj = a.R.U0[j];
       const float gw = K.dw(q);
-      const V3 e = (1.0f / (d + kEps)) * dr;
+      const V3 e = rcp_fast(d + kEps) * dr;
       const V3 kg = gw * e;
       if (MODE == 1) {
-        const float vj = Pj.w / Uj.w;
+        const float vj = Pj.w * rcp_fast(Uj.w);
         acc += vj * dot<DIM>(ui - v3(Uj), kg);
       } else if (MODE == 3) {
-        const float vj = Pj.w / Uj.w;
+        const float vj = Pj.w * rcp_fast(Uj.w);
         acc += vj * dot<DIM>(ui - v3(Uj), kg);
         if ((int)a.R.Q0[j].z == kFluid) {
           // psi_ij = 2 (rho_j - rho_i) (-r_ij) / (d + eps)^2 - grad_i - grad_j
           const float de = d + kEps;
-          const V3 psi = ((2.0f * (Uj.w - Ui.w)) / (de * de)) * ((-1.0f) * dr) - rgi -
+          const V3 psi = ((2.0f * (Uj.w - Ui.w)) * rcp_fast(de * de)) * ((-1.0f) * dr) - rgi -
                          v3(a.R.DG1[j]);
           diff += dot<DIM>(psi, kg) * vj;
         }
@@ -349,9 +339,9 @@ __device__ __forceinline__ int density_of(const SweepArgs& a, const int s, const
         const float Uavg = (uL + uR) * 0.5f;
         const V3 vavg = 0.5f * (ui + uj);
         const float rhoavg = (rhoL + rhoR) * 0.5f;
-        const float Ustar = Uavg + 0.5f * (pL - pR) / (rhoavg * p.c_ref);
+        const float Ustar = Uavg + 0.5f * (pL - pR) * rcp_fast(rhoavg * p.c_ref);
         const V3 vstar = Ustar * me + (vavg - Uavg * me);
-        acc += 2.0f * Ui.w * Pj.w / Uj.w * dot<DIM>(ui - vstar, kg);
+        acc += 2.0f * Ui.w * Pj.w * rcp_fast(Uj.w) * dot<DIM>(ui - vstar, kg);
       }
     }
     if (want_ut) {
