@@ -253,7 +253,9 @@ SPH_SWEEP_KERNELS(k_walls, walls_run, (int DIM, int KIND, int MODE), (DIM, KIND,
 // SOLVER 1: acceleration_fn_riemann (solver.py:316-401; its A term is zero).
 // Plus artificial viscosity (solver.py:404-428), heat (solver.py:591-610),
 // dudt + g_ext (solver.py:936) and the case boundary hook (integrator.py:54).
-template <int DIM, int KIND, int SOLVER, bool TVF>
+// EXTRAS: artificial viscosity and heat conduction are compiled in (otherwise
+// the sweep cannot issue the extra per-pair loads they need).
+template <int DIM, int KIND, int SOLVER, bool TVF, bool EXTRAS>
 __device__ __forceinline__ void force_run(const SweepArgs& a, const int blk) {
   constexpr int TPP = tpp_of(DIM);
   const int tid = blk * blockDim.x + threadIdx.x;
@@ -273,9 +275,9 @@ __device__ __forceinline__ void force_run(const SweepArgs& a, const int blk) {
   const V3 Ai = rhoi * ui;          // rho_i u_i
   const V3 wi = vi - ui;            // v_i - u_i
   const float pbg = eos.p_fn(0.0f);  // background pressure (solver.py:802)
-  const bool av = p.artificial_alpha != 0.0f;
+  const bool av = EXTRAS && p.artificial_alpha != 0.0f;
   const bool fluid_i = tag_i == kFluid;
-  const bool heat = p.heat_conduction != 0;
+  const bool heat = EXTRAS && p.heat_conduction != 0;
   const float kappa_i = Vi.w;
   const float Cp_i = heat ? a.R.NW[s].w : 1.0f;
   const float av_pref = p.artificial_alpha * p.av_h * 10.0f;  // alpha * h_ab * c_ab
@@ -411,8 +413,8 @@ __device__ __forceinline__ void force_run(const SweepArgs& a, const int blk) {
   if (o.T) o.T[i] = Ti;
   if (o.dTdt) o.dTdt[i] = heat ? dT : (a.in.dTdt ? a.in.dTdt[i] : 0.0f);
 }
-SPH_SWEEP_KERNELS(k_force, force_run, (int DIM, int KIND, int SOLVER, bool TVF),
-                  (DIM, KIND, SOLVER, TVF))
+SPH_SWEEP_KERNELS(k_force, force_run, (int DIM, int KIND, int SOLVER, bool TVF, bool EXTRAS),
+                  (DIM, KIND, SOLVER, TVF, EXTRAS))
 
 // ------------------------------------------------------------- integrate
 // si_euler kick + transport velocity + drift + periodic wrap (integrator.py:26-30)
@@ -454,6 +456,19 @@ using namespace sph;
 
 namespace sph {
 
+template <int DIM, int KIND, bool X, typename F>
+int for_force(const SweepPlan& pl, F&& f) {
+  if (pl.force == 0)
+    return pl.tvf ? f("force", k_force<DIM, KIND, 0, true, X>, k_force_ens<DIM, KIND, 0, true, X>)
+                  : f("force", k_force<DIM, KIND, 0, false, X>,
+                      k_force_ens<DIM, KIND, 0, false, X>);
+  if (pl.force == 2)
+    return pl.tvf ? f("force", k_force<DIM, KIND, 2, true, X>, k_force_ens<DIM, KIND, 2, true, X>)
+                  : f("force", k_force<DIM, KIND, 2, false, X>,
+                      k_force_ens<DIM, KIND, 2, false, X>);
+  return f("force", k_force<DIM, KIND, 1, false, X>, k_force_ens<DIM, KIND, 1, false, X>);
+}
+
 // Calls f(name, kernel, ensemble kernel) for each sweep of the plan, in order;
 // stops at the first non-zero return.
 template <int DIM, int KIND, typename F>
@@ -474,13 +489,7 @@ int for_plan(const SweepPlan& pl, F&& f) {
   if (pl.walls == 0) rc = f("walls", k_walls<DIM, KIND, 0>, k_walls_ens<DIM, KIND, 0>);
   if (pl.walls == 1) rc = f("walls", k_walls<DIM, KIND, 1>, k_walls_ens<DIM, KIND, 1>);
   if (rc) return rc;
-  if (pl.force == 0)
-    return pl.tvf ? f("force", k_force<DIM, KIND, 0, true>, k_force_ens<DIM, KIND, 0, true>)
-                  : f("force", k_force<DIM, KIND, 0, false>, k_force_ens<DIM, KIND, 0, false>);
-  if (pl.force == 2)
-    return pl.tvf ? f("force", k_force<DIM, KIND, 2, true>, k_force_ens<DIM, KIND, 2, true>)
-                  : f("force", k_force<DIM, KIND, 2, false>, k_force_ens<DIM, KIND, 2, false>);
-  return f("force", k_force<DIM, KIND, 1, false>, k_force_ens<DIM, KIND, 1, false>);
+  return pl.extras ? for_force<DIM, KIND, true>(pl, f) : for_force<DIM, KIND, false>(pl, f);
 }
 
 template <int DIM, typename F>

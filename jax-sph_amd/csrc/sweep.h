@@ -443,10 +443,11 @@ struct SweepPlan {
   int walls;    // -1: none, else k_walls MODE
   int force;    // k_force SOLVER
   bool tvf;     // k_force TVF
+  bool extras;  // k_force EXTRAS: artificial viscosity or heat conduction on
   int kind;     // SPH_KERNEL_QSK, SPH_KERNEL_WC2K, or -1 (runtime switch)
   bool operator==(const SweepPlan& o) const {
     return density == o.density && renorm == o.renorm && walls == o.walls &&
-           force == o.force && tvf == o.tvf && kind == o.kind;
+           force == o.force && tvf == o.tvf && extras == o.extras && kind == o.kind;
   }
 };
 
@@ -459,6 +460,7 @@ inline SweepPlan plan_of(const sph_solver_params& p) {
   pl.walls = gwbc ? 0 : heat_bc ? 1 : -1;
   pl.force = p.solver;
   pl.tvf = p.solver != 1 && p.tvf_stress;
+  pl.extras = p.artificial_alpha != 0.0f || p.heat_conduction != 0;
   pl.kind = (p.kernel == SPH_KERNEL_QSK || p.kernel == SPH_KERNEL_WC2K) ? p.kernel : -1;
   return pl;
 }
