@@ -100,12 +100,11 @@ extern "C" int sph_ws_create(const sph_nl_config* cfg, sph_ws** out) {
   ws->cfg = *cfg;
   ws->ncell = ncell;
   ws->npad = (int32_t)npad;
-  // Lane h of a particle scans every tpp-th candidate of each cell range, so
-  // it keeps at most ceil(len / tpp) of each of the 3^dim ranges.
+  // The pair build deals a particle's hits round-robin over its tpp
+  // sub-lists, so ceil(width / tpp) entries per sub-list hold `width` pairs.
   ws->tpp = tpp_of(cfg->dim);
   {
-    const int64_t nr = cfg->use_cells ? (cfg->dim == 3 ? 27 : 9) : 1;
-    int64_t lw = (cfg->ell_width + ws->tpp - 1) / ws->tpp + nr;
+    int64_t lw = (cfg->ell_width + ws->tpp - 1) / ws->tpp;
     const int64_t lw_n = (N + ws->tpp - 1) / ws->tpp + 1;
     ws->lane_width = (int32_t)(lw < lw_n ? lw : lw_n);
   }
@@ -180,8 +179,7 @@ extern "C" int sph_skin_enable(sph_ws* ws, float skin, int32_t width) {
     ncell *= k;
   }
   const int64_t N = c.n, npad = ws->npad;
-  const int64_t nr = c.dim == 3 ? 27 : 9;
-  int64_t lw = ((int64_t)width + ws->tpp - 1) / ws->tpp + nr;
+  int64_t lw = ((int64_t)width + ws->tpp - 1) / ws->tpp;  // hits dealt round-robin
   const int64_t lw_n = (N + ws->tpp - 1) / ws->tpp + 1;
   lw = lw < lw_n ? lw : lw_n;
   if ((double)lw * ws->tpp * (double)npad > 4.0e10) {
