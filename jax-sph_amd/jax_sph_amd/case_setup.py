@@ -174,8 +174,15 @@ class SimulationSetup:
         self.case = cfg.case
         self.special = cfg.case.special
         self.name = cfg.case.source.split(".")[0]
-        if cfg.case.mode != "sim" or cfg.case.r0_type != "cartesian":
-            raise NotImplementedError("relaxation runs and relaxed inputs are not provided")
+        if cfg.case.mode not in ("sim", "rlx"):
+            raise ValueError(f"case.mode must be 'sim' or 'rlx', not {cfg.case.mode!r}")
+        if cfg.case.r0_type != "cartesian":
+            # r0_type="relaxed" looks up data_relaxed/<name>.h5 (case_setup.py:277-306);
+            # here a relaxed state is passed as state0 (or an .npz state0_path) instead
+            raise NotImplementedError("r0_type 'relaxed' reads h5 files; pass state0 instead")
+        # relaxation (case_setup.py:356-404): the case's own geometry at rest,
+        # no body force, every wall at rest
+        self.relax = cfg.case.mode == "rlx"
 
     # ---------------------------------------------------------------- geometry
     def _box_size(self, dim, n_walls):
@@ -255,6 +262,8 @@ class SimulationSetup:
     def _velocity(self, r):
         dim = r.shape[1]
         v = np.zeros_like(r)
+        if self.relax:                                                  # case_setup.py:384-388
+            return v.astype(np.float32)
         if self.name == "tgv":
             f = np.float32
             if dim == 2:                                                # tgv.py:37-43
@@ -270,6 +279,8 @@ class SimulationSetup:
 
     def g_ext_fn(self, dim):
         name, g = self.name, self.case.g_ext_magnitude
+        if self.relax:                                                  # case_setup.py:390-391
+            return GravityField(np.zeros(dim, np.float32))
         if name == "db":
             return GravityField([0.0, -g] + ([0.0] if dim == 3 else []))
         if name == "pf":
@@ -285,6 +296,8 @@ class SimulationSetup:
         return GravityField(np.zeros(dim, np.float32))
 
     def bc_fn(self, dim):
+        if self.relax:                                                  # case_setup.py:393-402
+            return CaseBC(3)
         if self.name == "ldc":
             return CaseBC(1, [self.special.u_x_lid, 0.0, 0.0])
         if self.name == "cf":
@@ -296,9 +309,12 @@ class SimulationSetup:
         return CaseBC(0)
 
     # ----------------------------------------------------------------- main
-    def initialize(self, device=None):
+    def initialize(self, device=None, state0=None):
         """case_setup.py:44-231.  ``device`` (default: the HIP device) places
-        the state tensors; ``"cpu"`` gives the same values on the host."""
+        the state tensors; ``"cpu"`` gives the same values on the host.
+        ``state0`` (a state dict, e.g. a relaxation run's final state) or the
+        ``.npz`` file ``cfg.case.state0_path`` overwrites the keys
+        ``cfg.case.state0_keys`` of the fluid particles (case_setup.py:184-195)."""
         cfg = self.cfg
         dx, dim = cfg.case.dx, cfg.case.dim
         rho_ref, viscosity, u_ref = cfg.case.rho_ref, cfg.case.viscosity, cfg.case.u_ref
@@ -310,11 +326,16 @@ class SimulationSetup:
         mass_ref = h ** dim * rho_ref
         dt_conv = cfl * h / (c_ref + u_ref)                             # case_setup.py:94-110
         dt_visc = cfl * h ** 2 * rho_ref / (viscosity + EPS)
-        dt_body = cfl * (h / (cfg.case.g_ext_magnitude + EPS)) ** 0.5
+        g_mag = 0.0 if self.relax else cfg.case.g_ext_magnitude       # case_setup.py:375
+        dt_body = cfl * (h / (g_mag + EPS)) ** 0.5
         dt = float(np.amin([dt_conv, dt_visc, dt_body]))
         if cfg.solver.dt is not None:
             dt = cfg.solver.dt
-        sequence_length = int(cfg.solver.t_end / dt)
+        if self.relax:                                                  # case_setup.py:112-115
+            sequence_length = 5000
+            cfg.solver.t_end = dt * sequence_length
+        else:
+            sequence_length = int(cfg.solver.t_end / dt)
         eos = (RIEMANNEoS(rho_ref, p_bg, u_ref) if cfg.solver.name == "RIE"
                else TaitEoS(p_ref, rho_ref, p_bg, cfg.eos.gamma))
         n_walls = cfg.solver.n_walls
@@ -327,6 +348,24 @@ class SimulationSetup:
                                      cfg.case.r0_noise_factor * dx).numpy()
             side = np.asarray(box, np.float32)
             r = np.mod(r + noise, side).astype(np.float32)
+        if state0 is None and cfg.case.state0_path is not None:
+            from .io_state import read_state
+            state0 = read_state(cfg.case.state0_path)
+        if state0 is not None:                                          # case_setup.py:184-195
+            for k in cfg.case.state0_keys:
+                if k != "r":
+                    raise NotImplementedError("state0 overwrites positions ('r') only")
+                src = state0["r"]
+                src = (src.detach().cpu().numpy() if isinstance(src, torch.Tensor)
+                       else np.asarray(src)).astype(np.float32)
+                src_tag = state0["tag"]
+                src_tag = (src_tag.detach().cpu().numpy() if isinstance(src_tag, torch.Tensor)
+                           else np.asarray(src_tag))
+                mask, src_mask = tag == Tag.FLUID, src_tag == Tag.FLUID
+                if mask.sum() != src_mask.sum():
+                    raise ValueError("state0 has a different number of fluid particles")
+                r = r.copy()
+                r[mask] = src[src_mask]
         v = self._velocity(r)
         N = len(r)
         f32 = np.float32
@@ -360,6 +399,8 @@ class SimulationSetup:
         cfg.solver.dt, cfg.solver.sequence_length = dt, sequence_length
         cfg.case.num_particles_max = N
         cfg.case.pbc = [True, True, True]
+        # case_setup.py:201
+        cfg.case.bounds = np.array([np.zeros_like(box), box]).T.tolist()
         bc_fn = self.bc_fn(dim)
         state = bc_fn(state)
         return (cfg, box, state, self.g_ext_fn(dim), bc_fn, None, eos, gen, displacement_fn,

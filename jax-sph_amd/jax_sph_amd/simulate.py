@@ -27,13 +27,15 @@ from .utils import Logger
 
 
 def simulate(cfg: Cfg, write_fn: Optional[Callable[[int, Dict], None]] = None,
-             verbose: bool = True) -> Dict:
-    """Runs a case; returns the final state and run statistics."""
-    if cfg.case.mode == "rlx":
-        raise NotImplementedError("relaxation runs are not provided")
+             verbose: bool = True, state0: Optional[Dict] = None) -> Dict:
+    """Runs a case; returns the final state and run statistics.
+
+    ``cfg.case.mode == "rlx"`` runs the case's relaxation (simulate.py:30-32,
+    case_setup.py:356-404: 5000 steps at rest from noisy positions); its final
+    state is the ``state0`` (positions) of a later run."""
     Case = load_case(cfg.case.source or cfg.config)
     (cfg, box, state, g_ext_fn, bc_fn, nw_fn, eos, key, displacement_fn,
-     shift_fn) = Case(cfg).initialize()
+     shift_fn) = Case(cfg).initialize(state0=state0)
     solver = make_solver(cfg, displacement_fn, eos, g_ext_fn)
     forward = solver.forward_wrapper()
     n = state["r"].shape[0]
