@@ -166,6 +166,46 @@ def _compute_nws(r, tag, dx, n_walls, offset_vec, wall_part_fn):
     return nw
 
 
+class WallNormals:
+    """utils.py:197-275 (``compute_nws_jax_wrapper``): wall normals of the
+    current positions, for cases with moving walls whose solver reads normals
+    (recomputed every step, case_setup.py:208-218, integrator.py:32-33).
+
+    Each wall particle's normal is the unit periodic displacement to the
+    nearest point of one layer of 5x finer wall particles (the layer of the
+    initial normals).  The nearest point is found over the whole layer; the
+    reference searches a neighbour list of radius 1.01 sqrt(2) n_walls dx,
+    which always holds it.  Runs on the positions' device."""
+
+    def __init__(self, tag, layer, offset_vec, box):
+        self.walls = np.isin(np.asarray(tag), wall_tags)
+        self.layer = np.asarray(layer, np.float32)
+        self.offset = np.asarray(offset_vec, np.float32)
+        self.box = np.asarray(box, np.float32)
+
+    def __call__(self, r: torch.Tensor) -> torch.Tensor:
+        dev = r.device
+        walls = torch.as_tensor(self.walls, device=dev)
+        rw = r[walls] - torch.as_tensor(self.offset, device=dev)
+        layer = torch.as_tensor(self.layer, device=dev)
+        out = []
+        for a in range(0, rw.shape[0], 1024):       # bounded (walls x layer) blocks
+            d = space.periodic_displacement(self.box, layer[None, :, :] - rw[a:a + 1024, None])
+            dist = space.distance(d)
+            k = torch.argmin(dist, dim=1)
+            rows = torch.arange(k.shape[0], device=dev)
+            out.append(d[rows, k] / (dist[rows, k] + EPS)[:, None])
+        nw = torch.zeros_like(r)
+        nw[walls] = torch.cat(out).to(r.dtype)
+        return nw
+
+
+def _wall_layer(dx, n_walls, offset_vec, wall_part_fn):
+    """One layer of 5x finer wall particles, aligned like the walls (utils.py:183-185)."""
+    dx_fac = 5
+    return wall_part_fn(dx / dx_fac, 1) - offset_vec / n_walls / dx_fac
+
+
 class SimulationSetup:
     """case_setup.py:29-231 for one of tgv / ldc / db / pf / cf."""
 
@@ -381,6 +421,12 @@ class SimulationSetup:
                               lambda d, n: self._walls(d, n, dim))
         else:
             nw = np.zeros_like(r)
+        # case_setup.py:208-218: normals follow moving walls every step
+        nw_fn = None
+        if is_nw and np.any(tag == Tag.MOVING_WALL):
+            layer = _wall_layer(dx, n_walls, self.offset_vec(dim),
+                                lambda d, n: self._walls(d, n, dim))
+            nw_fn = WallNormals(tag, layer, self.offset_vec(dim), box)
         dev = _lib.device() if device is None else torch.device(device)
 
         def T_(x, dtype=torch.float32):
@@ -403,7 +449,7 @@ class SimulationSetup:
         cfg.case.bounds = np.array([np.zeros_like(box), box]).T.tolist()
         bc_fn = self.bc_fn(dim)
         state = bc_fn(state)
-        return (cfg, box, state, self.g_ext_fn(dim), bc_fn, None, eos, gen, displacement_fn,
+        return (cfg, box, state, self.g_ext_fn(dim), bc_fn, nw_fn, eos, gen, displacement_fn,
                 shift_fn)
 
 

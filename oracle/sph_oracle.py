@@ -548,7 +548,8 @@ class WCSPH:
 
 
 # --------------------------------------------------------------------- integrator
-def si_euler(tvf, model, box, bc_fn: Optional[Callable], nl_update, dtype=np.float32):
+def si_euler(tvf, model, box, bc_fn: Optional[Callable], nl_update, dtype=np.float32,
+             nw_fn: Optional[Callable] = None):
     """``integrator.py:8-58``: returns advance(dt, state, nbrs) -> (state, nbrs)."""
     t = np.dtype(dtype).type
     box = np.asarray(box, dtype=dtype).reshape(-1)
@@ -559,6 +560,8 @@ def si_euler(tvf, model, box, bc_fn: Optional[Callable], nl_update, dtype=np.flo
         state["u"] = state["u"] + t(1.0 * dtf) * state["dudt"]
         state["v"] = state["u"] + t(t(tvf * 0.5) * dtf) * state["dvdt"]
         state["r"] = periodic_shift(box, state["r"], t(1.0 * dtf) * state["v"])
+        if nw_fn is not None:                                   # integrator.py:32-33
+            state["nw"] = nw_fn(state["r"])
         nbrs = nl_update(state["r"], nbrs)
         state = model(state, nbrs.idx)
         if bc_fn is not None:
@@ -566,6 +569,23 @@ def si_euler(tvf, model, box, bc_fn: Optional[Callable], nl_update, dtype=np.flo
         return state, nbrs
 
     return advance
+
+
+def wall_normals(r, tag, layer, offset_vec, box, dtype=np.float32):
+    """``utils.py:197-275``: normal of each wall particle = unit periodic
+    displacement (layer - wall) to its nearest point of the fine wall layer
+    (brute force over the layer; first minimum on ties)."""
+    t = np.dtype(dtype).type
+    walls = np.isin(tag, (1, 2, 3))
+    rw = (r[walls] - np.asarray(offset_vec, dtype)).astype(dtype)
+    lay = np.asarray(layer, dtype)
+    d = periodic_displacement(np.asarray(box, dtype), lay[None, :, :] - rw[:, None, :])
+    dist = np.sqrt((d * d).sum(-1))
+    k = np.argmin(dist, axis=1)
+    rows = np.arange(len(k))
+    nw = np.zeros_like(r, dtype=dtype)
+    nw[walls] = d[rows, k] / (dist[rows, k] + t(np.finfo(float).eps))[:, None]
+    return nw
 
 
 def get_ekin(state, dx):

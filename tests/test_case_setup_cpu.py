@@ -106,3 +106,42 @@ def test_state0_overwrites_fluid_positions_and_round_trips():
     with pytest.raises(ValueError):
         cs.load_case("ldc")(cs.load_config("ldc", {"case": {"dx": 0.04}})).initialize(
             device="cpu", state0=short)
+
+
+def test_wall_normals_point_into_the_fluid():
+    """utils.py:169-194 at set-up: on each straight wall the normal is the
+    inward axis direction (bottom +y, lid -y, left +x, right -x)."""
+    cfg, box, state, g, bc, nw_fn, *_ = _init(
+        "ldc", {"case": {"dx": 0.04}, "solver": {"name": "RIE", "density_evolution": True}})
+    r, tag, nw = state["r"].numpy(), state["tag"].numpy(), state["nw"].numpy()
+    dx, lo, hi = 0.04, 3 * 0.04, 1 + 3 * 0.04
+    mid = lambda c: (c > lo + 0.2) & (c < hi - 0.2)  # noqa: E731
+    wall = tag != Tag.FLUID
+    for sel, want in [(wall & mid(r[:, 0]) & (r[:, 1] < lo), (0, 1)),
+                      (wall & mid(r[:, 0]) & (r[:, 1] > hi), (0, -1)),
+                      (wall & mid(r[:, 1]) & (r[:, 0] < lo), (1, 0)),
+                      (wall & mid(r[:, 1]) & (r[:, 0] > hi), (-1, 0))]:
+        assert sel.sum() > 0
+        np.testing.assert_allclose(nw[sel], np.broadcast_to(want, nw[sel].shape), atol=1e-5)
+    assert np.all(nw[~wall] == 0)
+
+
+def test_moving_wall_normals_follow_the_walls():
+    """case_setup.py:208-218: with moving walls and a solver that reads
+    normals, nw_fn recomputes them from the current positions
+    (utils.py:197-275); it reproduces the set-up normals, and after the lid
+    slides it matches the oracle's brute-force nearest-point restatement."""
+    from oracle import sph_oracle as so
+    cfg, box, state, g, bc, nw_fn, *_ = _init(
+        "ldc", {"case": {"dx": 0.04}, "solver": {"name": "RIE", "density_evolution": True}})
+    assert nw_fn is not None
+    assert _init("ldc", {"case": {"dx": 0.04}})[5] is None     # SPH: no normals read
+    assert _init("tgv", {"case": {"dim": 2}})[5] is None       # no moving walls
+    np.testing.assert_allclose(nw_fn(state["r"]).numpy(), state["nw"].numpy(), atol=1e-5)
+    r = state["r"].clone()
+    lid = state["tag"] == Tag.MOVING_WALL
+    r[lid, 0] = torch.remainder(r[lid, 0] + 0.37, float(box[0]))   # across the corners
+    got = nw_fn(r).numpy()
+    want = so.wall_normals(r.numpy(), state["tag"].numpy(), nw_fn.layer, nw_fn.offset, box)
+    np.testing.assert_allclose(got, want, atol=1e-6)
+    assert not np.allclose(got, state["nw"].numpy(), atol=1e-3)  # the normals did change
