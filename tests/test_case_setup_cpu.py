@@ -113,7 +113,9 @@ def test_wall_normals_point_into_the_fluid():
     inward axis direction (bottom +y, lid -y, left +x, right -x)."""
     cfg, box, state, g, bc, nw_fn, *_ = _init(
         "ldc", {"case": {"dx": 0.04}, "solver": {"name": "RIE", "density_evolution": True}})
+    from oracle import sph_oracle as so
     r, tag, nw = state["r"].numpy(), state["tag"].numpy(), state["nw"].numpy()
+    nw_o = so.wall_normals(r, tag, nw_fn.layer, nw_fn.offset, box)  # the oracle's restatement
     dx, lo, hi = 0.04, 3 * 0.04, 1 + 3 * 0.04
     mid = lambda c: (c > lo + 0.2) & (c < hi - 0.2)  # noqa: E731
     wall = tag != Tag.FLUID
@@ -122,8 +124,13 @@ def test_wall_normals_point_into_the_fluid():
                       (wall & mid(r[:, 1]) & (r[:, 0] < lo), (1, 0)),
                       (wall & mid(r[:, 1]) & (r[:, 0] > hi), (-1, 0))]:
         assert sel.sum() > 0
-        np.testing.assert_allclose(nw[sel], np.broadcast_to(want, nw[sel].shape), atol=1e-5)
-    assert np.all(nw[~wall] == 0)
+        for n in (nw, nw_o):
+            np.testing.assert_allclose(n[sel], np.broadcast_to(want, n[sel].shape), atol=1e-5)
+    assert np.all(nw[~wall] == 0) and np.all(nw_o[~wall] == 0)
+    # corner particles: unit vectors pointing diagonally into the fluid
+    corner = wall & (r[:, 0] < lo) & (r[:, 1] < lo)
+    np.testing.assert_allclose(np.linalg.norm(nw_o[corner], axis=1), 1.0, rtol=1e-5)
+    assert np.all(nw_o[corner] >= 0)
 
 
 def test_moving_wall_normals_follow_the_walls():
