@@ -170,8 +170,11 @@ __device__ __forceinline__ void renorm_run(const SweepArgs& a, const int blk) {
   if (h != 0) return;
   const float rho = nom / (den > 1.0f ? 1.0f : den);
   const float4 Ui = a.R.U1[s], Qi = a.R.Q1[s];
+  const float pr = eos.p_fn(rho);
+  const float4 Vi = a.R.V0[s];
   a.R.U3[s] = make_float4(Ui.x, Ui.y, Ui.z, rho);
-  a.R.Q3[s] = make_float4(eos.p_fn(rho), Qi.y, Qi.z, Qi.w);
+  a.R.Q3[s] = make_float4(pr, Qi.y, Qi.z, Qi.w);
+  a.R.W3[s] = make_float4(Vi.x, Vi.y, Vi.z, pr);
 }
 SPH_SWEEP_KERNELS(k_renorm, renorm_run, (int DIM, int KIND), (DIM, KIND))
 
@@ -244,6 +247,7 @@ __device__ __forceinline__ void walls_run(const SweepArgs& a, const int blk) {
   a.R.U2[s] = f4(u, rho);
   a.R.V2[s] = f4(v, Vi.w);
   a.R.Q2[s] = make_float4(pr, Qi.y, Qi.z, T);
+  a.R.W2[s] = f4(v, pr);
 }
 SPH_SWEEP_KERNELS(k_walls, walls_run, (int DIM, int KIND, int MODE), (DIM, KIND, MODE))
 
@@ -286,6 +290,11 @@ __device__ __forceinline__ void force_run(const SweepArgs& a, const int blk) {
   V3 gi = g_ext_of<DIM>(a, i);
   V3 acc = {0.f, 0.f, 0.f}, accv = {0.f, 0.f, 0.f};
   float dT = 0.0f;
+  // The standard solver without extras reads (p, eta) of j only: p (and v for
+  // the transport-velocity stress) from the (v, p) record, eta from Q unless
+  // every particle has the same viscosity (then eta_j = eta_i, the same value).
+  constexpr bool kLean = SOLVER == 0 && !EXTRAS;
+  const bool need_q = !kLean || *a.eta_varies != 0;
   for_each_pair(a, s, h, [&](const int j) {
     const float4 Pj = a.R.P[j];
     const V3 dr = min_image<DIM>(Pi, Pj, a.geo);
@@ -295,8 +304,12 @@ __device__ __forceinline__ void force_run(const SweepArgs& a, const int blk) {
     const float d = sqrt_fast(d2);
     const float q = d * p.one_over_h;
     const float gw = K.dw(q);
-    const float4 Uj = a.Uf[j], Qj = a.Qf[j];
-    const float mj = Pj.w, rhoj = Uj.w, pj = Qj.x, etaj = Qj.y;
+    const float4 Uj = a.Uf[j];
+    float4 Qj = make_float4(0.0f, etai, 0.0f, 0.0f);
+    if (need_q) Qj = a.Qf[j];
+    float4 Wj = make_float4(0.0f, 0.0f, 0.0f, Qj.x);  // (v, p) of j
+    if (SOLVER != 1 && (TVF || !need_q)) Wj = a.Wf[j];
+    const float mj = Pj.w, rhoj = Uj.w, pj = Wj.w, etaj = Qj.y;
     const V3 uj = v3(Uj);
     const float inv_d = rcp_fast(d + kEps);
     const float Vj = mj * rcp_fast(rhoj);
@@ -306,7 +319,7 @@ __device__ __forceinline__ void force_run(const SweepArgs& a, const int blk) {
       const float pij = (rhoj * pi_ + rhoi * pj) * rcp_fast(rhoi + rhoj);
       V3 t = (-pij) * dr + eta_ij * (ui - uj);
       if (TVF) {
-        const V3 vj = v3(a.Vf[j]);
+        const V3 vj = v3(Wj);
         const V3 Ar = 0.5f * (dot<DIM>(wi, dr) * Ai + dot<DIM>(vj - uj, dr) * (rhoj * uj));
         t = t + Ar;
       }

@@ -90,6 +90,12 @@ struct sph_ws {
   float4* NW = nullptr;  // wall normal
   float4* UT = nullptr;  // Riemann wall velocity u_tilde
   float4 *U3 = nullptr, *Q3 = nullptr;  // after density renormalisation
+  // (v, p) of each stage (density, walls, renormalisation): the force sweep's
+  // transport velocity and pressure in one record
+  float4 *W1 = nullptr, *W2 = nullptr, *W3 = nullptr;
+  // device flag, sticky: 1 once a gathered state had particles of different
+  // viscosity (until then the force sweep takes eta_j = eta_i)
+  int32_t* eta_varies = nullptr;
   // delta-SPH: renormalisation matrix L (row-major, 4 or 9 floats) and the
   // two density-gradient terms
   float4 *DL0 = nullptr, *DL1 = nullptr, *DL2 = nullptr, *DG0 = nullptr, *DG1 = nullptr;

@@ -110,6 +110,7 @@ struct Recs {
   float* drho;
   float4 *U3, *Q3;                    // renormalised density stage
   float4 *DL0, *DL1, *DL2, *DG0, *DG1;  // delta-SPH matrices and gradient terms
+  float4 *W1, *W2, *W3;                // (v, p) after density, walls, renormalisation
 };
 
 struct SweepArgs {
@@ -127,6 +128,8 @@ struct SweepArgs {
   // which records hold the density stage's (u, rho) and (p, eta, tag, T):
   // U1/Q1, or U3/Q3 after renormalisation
   const float4 *Ud, *Qd;
+  const float4* Wf;     // (v, p) records the force sweep reads
+  int32_t* eta_varies;  // sticky device flag: the gathered viscosities differ
   int heat_in_density;  // apply T += dt*dTdt in the density epilogue
   sph_bc_spec bc;
   // Pair filter of skin lists: squared distances below cut_lo are inside the
@@ -206,7 +209,9 @@ __device__ __forceinline__ void gather_state_run(const SweepArgs& a, const float
   }
   a.R.U0[s] = f4(load_vec<DIM>(u, i), in.rho[i]);
   a.R.V0[s] = f4(load_vec<DIM>(v, i), in.kappa ? in.kappa[i] : 0.0f);
-  a.R.Q0[s] = make_float4(in.p[i], in.eta[i], (float)in.tag[i], in.T ? in.T[i] : 0.0f);
+  const float eta = in.eta[i];
+  a.R.Q0[s] = make_float4(in.p[i], eta, (float)in.tag[i], in.T ? in.T[i] : 0.0f);
+  if (eta != in.eta[0]) *a.eta_varies = 1;  // same value from every writer
   if (in.nw || in.Cp) {
     V3 n = in.nw ? load_vec<DIM>(in.nw, i) : V3{0.f, 0.f, 0.f};
     a.R.NW[s] = f4(n, in.Cp ? in.Cp[i] : 0.0f);
@@ -385,6 +390,8 @@ __device__ __forceinline__ int density_of(const SweepArgs& a, const int s, const
   if (a.heat_in_density) T = T + p.dt * a.in.dTdt[i];
   a.R.U1[s] = make_float4(Ui.x, Ui.y, Ui.z, rho);
   a.R.Q1[s] = make_float4(pr, Qi.y, Qi.z, T);
+  const float4 Vi = a.R.V0[s];
+  a.R.W1[s] = make_float4(Vi.x, Vi.y, Vi.z, pr);
   a.R.drho[s] = drhodt;
   if (p.solver == 1 && p.bc_trick) {
     // u_tilde of every particle: Shepard average for no-slip walls, u otherwise
@@ -471,9 +478,9 @@ inline void select_records(SweepArgs& a, const SweepPlan& pl) {
   a.Ud = pl.renorm ? a.R.U3 : a.R.U1;
   a.Qd = pl.renorm ? a.R.Q3 : a.R.Q1;
   if (pl.walls >= 0) {
-    a.Uf = a.R.U2; a.Vf = a.R.V2; a.Qf = a.R.Q2;
+    a.Uf = a.R.U2; a.Vf = a.R.V2; a.Qf = a.R.Q2; a.Wf = a.R.W2;
   } else {
-    a.Uf = a.Ud; a.Vf = a.R.V0; a.Qf = a.Qd;
+    a.Uf = a.Ud; a.Vf = a.R.V0; a.Qf = a.Qd; a.Wf = pl.renorm ? a.R.W3 : a.R.W1;
   }
 }
 
@@ -523,7 +530,8 @@ inline SweepArgs make_args(sph_ws* ws, int bank, const sph_solver_params* prm, c
   a.out = *out;
   a.R = Recs{ws->P,  ws->U0, ws->V0, ws->Q0,  ws->NW,  ws->U1,  ws->Q1,  ws->U2,  ws->V2,
              ws->Q2, ws->UT, ws->drho, ws->U3, ws->Q3, ws->DL0, ws->DL1, ws->DL2, ws->DG0,
-             ws->DG1};
+             ws->DG1, ws->W1, ws->W2, ws->W3};
+  a.eta_varies = ws->eta_varies;
   const Bank& B = bank == kSkinBank ? ws->sk.b : ws->bank[bank];
   a.perm = B.perm;
   a.ell = B.ell;
