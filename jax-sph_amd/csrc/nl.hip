@@ -375,46 +375,84 @@ __global__ void k_scatter(const int32_t* __restrict__ pcell, const int32_t* __re
   scatter_body(pcell, pslot, cell_start, unsorted, n, gate, blockIdx.x);
 }
 
-// One wavefront per cell: rank-sort the ids that arrived in the cell so that a
-// cell lists its particles in ascending id, like the reference's stable argsort.
+// Morton code of the particle's 4^DIM sub-cell inside its cell (2 bits per
+// axis, the octant's bits above the quarter's).
+template <int DIM>
+__device__ __forceinline__ unsigned sub_cell_code(const float4 p, const Grid& g) {
+  const float x[3] = {p.x, p.y, p.z};
+  unsigned code = 0u;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    const float q = (float)((double)x[d] / (double)g.cs[d]);
+    int c = (int)q;
+    c = c < 0 ? 0 : (c >= g.cps[d] ? g.cps[d] - 1 : c);
+    int f = (int)((q - (float)c) * 4.0f);
+    f = f < 0 ? 0 : (f > 3 ? 3 : f);
+    code |= ((unsigned)(f >> 1) << (DIM + d)) | ((unsigned)(f & 1) << d);
+  }
+  return code;
+}
+
+// Sort key of particle `id` within its cell: the id alone, or (with positions)
+// the sub-cell code first and the id second.
+template <int DIM>
+__device__ __forceinline__ unsigned long long cell_sort_key(int id, const float* __restrict__ r,
+                                                            const Grid& g) {
+  unsigned long long k = (unsigned)id;
+  if (r) k |= (unsigned long long)sub_cell_code<DIM>(load_pos<DIM>(r, id), g) << 32;
+  return k;
+}
+
+// One wavefront per cell: rank-sort the ids that arrived in the cell.  Without
+// positions (r == nullptr) a cell lists its particles in ascending id, like the
+// reference's stable argsort; with positions (skin lists, whose order is free)
+// the particles of a cell follow a Morton curve over its sub-cells, so that
+// consecutive particles, which share wavefronts in the sweeps, are close.
+template <int DIM>
 __device__ __forceinline__ void cell_sort_body(const int32_t* __restrict__ cell_start,
                                                const int32_t* __restrict__ unsorted,
                                                int32_t* __restrict__ perm,
                                                int32_t* __restrict__ iperm, int64_t ncell,
-                                               const int32_t* __restrict__ gate, int blk) {
+                                               const int32_t* __restrict__ gate, int blk,
+                                               const float* __restrict__ r, const Grid& g) {
   if (gate && *gate == 0) return;
   const int lane = threadIdx.x & 63;
   const int64_t cell = ((int64_t)blk * blockDim.x + threadIdx.x) >> 6;
   if (cell >= ncell) return;
   const int b = cell_start[cell], e = cell_start[cell + 1], n = e - b;
   if (n <= 64) {
-    int id = lane < n ? unsorted[b + lane] : 0x7fffffff;
+    const unsigned long long key = lane < n ? cell_sort_key<DIM>(unsorted[b + lane], r, g)
+                                            : ~0ull;
     int rank = 0;
     for (int k = 0; k < n; ++k) {
-      int o = __shfl(id, k, 64);
-      rank += (o < id) ? 1 : 0;
+      const unsigned long long o = __shfl(key, k, 64);
+      rank += (o < key) ? 1 : 0;
     }
     if (lane < n) {
+      const int id = (int)(unsigned)(key & 0xffffffffull);
       perm[b + rank] = id;
       iperm[id] = b + rank;
     }
   } else {
     for (int k0 = lane; k0 < n; k0 += 64) {
-      int id = unsorted[b + k0];
+      const int id = unsorted[b + k0];
+      const unsigned long long key = cell_sort_key<DIM>(id, r, g);
       int rank = 0;
-      for (int k = 0; k < n; ++k) rank += (unsorted[b + k] < id) ? 1 : 0;
+      for (int k = 0; k < n; ++k) rank += (cell_sort_key<DIM>(unsorted[b + k], r, g) < key) ? 1 : 0;
       perm[b + rank] = id;
       iperm[id] = b + rank;
     }
   }
 }
 
+template <int DIM>
 __global__ __launch_bounds__(256) void k_cell_sort(const int32_t* __restrict__ cell_start,
                                                    const int32_t* __restrict__ unsorted,
                                                    int32_t* __restrict__ perm,
                                                    int32_t* __restrict__ iperm, int64_t ncell,
-                                                   const int32_t* __restrict__ gate = nullptr) {
-  cell_sort_body(cell_start, unsorted, perm, iperm, ncell, gate, blockIdx.x);
+                                                   const int32_t* __restrict__ gate,
+                                                   const float* __restrict__ r, Grid g) {
+  cell_sort_body<DIM>(cell_start, unsorted, perm, iperm, ncell, gate, blockIdx.x, r, g);
 }
 
 template <int DIM>
@@ -450,11 +488,13 @@ struct BuildArgs {
 
 // Lane h (of TPP) tests candidates base + h, base + h + TPP, ... of one cell
 // range [lo, hi) whose image is shifted by `img`; the TPP lanes of a particle
-// step through the range together, 32 candidates each per chunk, and collect
-// their hits in a bit mask.  Hits are numbered over the particle's lanes (lane
-// 0's hits of the chunk first, then lane 1's, ...) and hit k goes to sub-list
-// k mod TPP at position k / TPP: the particle's sub-lists differ in length by
-// at most one, so the lanes that later sweep them finish together.
+// step through the range together, 32 candidates each per chunk (so each load
+// of the particle's lanes reads TPP consecutive positions), and collect their
+// hits in a bit mask.  The particle's hits are numbered in candidate order
+// over all its lanes, and hit k goes to sub-list k mod TPP at position k / TPP:
+// the sub-lists differ in length by at most one, so the lanes that later sweep
+// them finish together, and at each step of a sweep the particles of a
+// wavefront read neighbours at similar places of the sorted order.
 // The first kStage entries of each sub-list are staged in LDS (in the column
 // of the thread that owns the sub-list) and written out row by row at the end,
 // so every pair-buffer row is written by whole wavefronts at once; later
@@ -501,24 +541,27 @@ __device__ __forceinline__ void test_range(const BuildArgs& a, int s, int h, con
         mask |= hit ? (1u << t) : 0u;
       }
     }
-    // number this chunk's hits over the particle's lanes
-    const int nh = __popc(mask);
-    int incl = nh;
+    // candidate base + t TPP + h follows the particle's hits at smaller t on
+    // every lane and at the same t on the lanes below h
+    unsigned m[TPP];
+    int total = 0;
 #pragma unroll
-    for (int o = 1; o < TPP; o <<= 1) {
-      const int y = __shfl_up(incl, o, TPP);
-      if (h >= o) incl += y;
+    for (int q = 0; q < TPP; ++q) {
+      m[q] = __shfl(mask, q, TPP);
+      total += __popc(m[q]);
     }
-    const int total = __shfl(incl, TPP - 1, TPP);
-    int k = count + incl - nh;
     while (mask) {
       const int t = __ffs(mask) - 1;
       mask &= mask - 1u;
+      const unsigned below = (1u << t) - 1u;
+      int k = count;
+#pragma unroll
+      for (int q = 0; q < TPP; ++q)
+        k += __popc(m[q] & below) + (q < h ? (int)((m[q] >> t) & 1u) : 0);
       const int32_t b = c0 + t * TPP;
       const int q = k % TPP, pos = k / TPP;
       if (pos < kStage) stage0[pos * kBlock + q] = b;
       else if (pos < a.lane_width) out0[((int64_t)q * a.lane_width + pos) * a.npad] = b;
-      ++k;
     }
     count += total;
   }
@@ -841,8 +884,9 @@ int bin_finish(sph_ws* ws, int bank, const float* r, int64_t* info, hipStream_t 
              k_scatter<<<cdiv(n, kBlock), kBlock, 0, st>>>(ws->pcell, ws->pslot, B.cell_start,
                                                          ws->unsorted, n));
   SPH_LAUNCH(ws, "cell_sort", st,
-             k_cell_sort<<<cdiv(ws->ncell * 64, 256), 256, 0, st>>>(B.cell_start, ws->unsorted,
-                                                                    B.perm, B.iperm, ws->ncell));
+             k_cell_sort<3><<<cdiv(ws->ncell * 64, 256), 256, 0, st>>>(
+                 B.cell_start, ws->unsorted, B.perm, B.iperm, ws->ncell, nullptr, nullptr,
+                 Grid{}));
   if (!r) return SPH_OK;
   if (c.dim == 2)
     SPH_LAUNCH(ws, "gather_pos", st,
@@ -988,10 +1032,14 @@ int skin_begin(sph_ws* ws, const float* r, const float* u, const float* dudt, co
   SPH_LAUNCH(ws, "skin_scatter", st,
              k_scatter<<<nb, kBlock, 0, st>>>(ws->pcell, ws->pslot, S.b.cell_start, ws->unsorted,
                                               n, S.flag));
-  SPH_LAUNCH(ws, "skin_sort", st,
-             k_cell_sort<<<cdiv(S.ncell * 64, 256), 256, 0, st>>>(S.b.cell_start, ws->unsorted,
-                                                                  S.b.perm, S.b.iperm, S.ncell,
-                                                                  S.flag));
+  if (c.dim == 2)
+    SPH_LAUNCH(ws, "skin_sort", st,
+               k_cell_sort<2><<<cdiv(S.ncell * 64, 256), 256, 0, st>>>(
+                   S.b.cell_start, ws->unsorted, S.b.perm, S.b.iperm, S.ncell, S.flag, r_out, gs));
+  else
+    SPH_LAUNCH(ws, "skin_sort", st,
+               k_cell_sort<3><<<cdiv(S.ncell * 64, 256), 256, 0, st>>>(
+                   S.b.cell_start, ws->unsorted, S.b.perm, S.b.iperm, S.ncell, S.flag, r_out, gs));
   return SPH_OK;
 }
 
@@ -1162,12 +1210,14 @@ __global__ void k_scatter_ens(const EnsBin* __restrict__ t, const EnsBlock* __re
   scatter_body(a.pcell, a.pslot, a.cell_start, a.unsorted, a.n, a.flag, m.blk);
 }
 
+template <int DIM>
 __global__ __launch_bounds__(256) void k_cell_sort_ens(const EnsBin* __restrict__ t,
                                                        const EnsBlock* __restrict__ map) {
   const EnsBlock m = map[blockIdx.x];
   if (m.sim < 0) return;
   const EnsBin& a = t[m.sim];
-  cell_sort_body(a.cell_start, a.unsorted, a.perm, a.iperm, a.ncell, a.flag, m.blk);
+  cell_sort_body<DIM>(a.cell_start, a.unsorted, a.perm, a.iperm, a.ncell, a.flag, m.blk, a.r,
+                      a.g);
 }
 
 template <int DIM, int TPP>
@@ -1267,8 +1317,12 @@ int ens_skin_begin(EnsNL* t, int dim, const EnsMap& part, const EnsMap& cells, s
   SPH_LAUNCH(prof, "ens_skin_scan", st, k_scan_ens<<<t->n, kScanThreads, 0, st>>>(t->bin));
   SPH_LAUNCH(prof, "ens_skin_scatter", st,
              k_scatter_ens<<<part.blocks, kBlock, 0, st>>>(t->bin, part.d));
-  SPH_LAUNCH(prof, "ens_skin_sort", st,
-             k_cell_sort_ens<<<cells.blocks, 256, 0, st>>>(t->bin, cells.d));
+  if (dim == 2)
+    SPH_LAUNCH(prof, "ens_skin_sort", st,
+               k_cell_sort_ens<2><<<cells.blocks, 256, 0, st>>>(t->bin, cells.d));
+  else
+    SPH_LAUNCH(prof, "ens_skin_sort", st,
+               k_cell_sort_ens<3><<<cells.blocks, 256, 0, st>>>(t->bin, cells.d));
   return SPH_OK;
 }
 
