@@ -498,8 +498,10 @@ struct BuildArgs {
 // The first kStage entries of each sub-list are staged in LDS (in the column
 // of the thread that owns the sub-list) and written out row by row at the end,
 // so every pair-buffer row is written by whole wavefronts at once; later
-// entries go straight to the buffer.
-constexpr int kStage = 32;
+// entries go straight to the buffer.  24 staged entries (24 KB per workgroup)
+// leave LDS for 6 workgroups per CU, which matches the 6 waves per SIMD the
+// build's registers allow.
+constexpr int kStage = 24;
 
 template <int DIM, int TPP, bool ORDERED>
 __device__ __forceinline__ void test_range(const BuildArgs& a, int s, int h, const float4 ps,
@@ -663,7 +665,7 @@ __device__ __forceinline__ void nl_build_body(const BuildArgs& a, const int blk)
 }
 
 template <int DIM, int TPP, bool ORDERED>
-__global__ __launch_bounds__(kBlock) void k_nl_build(BuildArgs a) {
+__global__ __launch_bounds__(kBlock, ORDERED ? 1 : 6) void k_nl_build(BuildArgs a) {
   nl_build_body<DIM, TPP, ORDERED>(a, logical_block(a.remap));
 }
 
@@ -1221,7 +1223,7 @@ __global__ __launch_bounds__(256) void k_cell_sort_ens(const EnsBin* __restrict_
 }
 
 template <int DIM, int TPP>
-__global__ __launch_bounds__(kBlock) void k_nl_build_ens(const BuildArgs* __restrict__ t,
+__global__ __launch_bounds__(kBlock, 6) void k_nl_build_ens(const BuildArgs* __restrict__ t,
                                                          const EnsBlock* __restrict__ map) {
   const EnsBlock m = map[blockIdx.x];
   if (m.sim < 0) return;

@@ -106,7 +106,7 @@ __device__ __forceinline__ void delta_L_run(const SweepArgs& a, const int blk) {
   }
   store_mat<DIM>(a, s, L);
 }
-SPH_SWEEP_KERNELS(k_delta_L, delta_L_run, (int DIM, int KIND), (DIM, KIND))
+SPH_SWEEP_KERNELS(k_delta_L, delta_L_run, (int DIM, int KIND), (DIM, KIND), 8)
 
 // Density-gradient terms of solver.py:60-66:
 //   grad_i = L_i sum_j (rho_j - rho_i) grad W_ij V_j
@@ -140,7 +140,7 @@ __device__ __forceinline__ void delta_grad_run(const SweepArgs& a, const int blk
   a.R.DG0[s] = f4(mat_vec<DIM>(a, s, si), 0.0f);
   a.R.DG1[s] = f4(gj, 0.0f);
 }
-SPH_SWEEP_KERNELS(k_delta_grad, delta_grad_run, (int DIM, int KIND), (DIM, KIND))
+SPH_SWEEP_KERNELS(k_delta_grad, delta_grad_run, (int DIM, int KIND), (DIM, KIND), 8)
 
 // ------------------------------------------------------------- renormalisation
 // solver.py:167-173 after a continuity update: rho = sum m_j W / min(sum (m/rho)_j W, 1)
@@ -176,7 +176,7 @@ __device__ __forceinline__ void renorm_run(const SweepArgs& a, const int blk) {
   a.R.Q3[s] = make_float4(pr, Qi.y, Qi.z, Qi.w);
   a.R.W3[s] = make_float4(Vi.x, Vi.y, Vi.z, pr);
 }
-SPH_SWEEP_KERNELS(k_renorm, renorm_run, (int DIM, int KIND), (DIM, KIND))
+SPH_SWEEP_KERNELS(k_renorm, renorm_run, (int DIM, int KIND), (DIM, KIND), 8)
 
 // ------------------------------------------------------------- walls
 // MODE 0: generalized wall BC of the SPH solver (solver.py:431-530)
@@ -249,7 +249,7 @@ __device__ __forceinline__ void walls_run(const SweepArgs& a, const int blk) {
   a.R.Q2[s] = make_float4(pr, Qi.y, Qi.z, T);
   a.R.W2[s] = f4(v, pr);
 }
-SPH_SWEEP_KERNELS(k_walls, walls_run, (int DIM, int KIND, int MODE), (DIM, KIND, MODE))
+SPH_SWEEP_KERNELS(k_walls, walls_run, (int DIM, int KIND, int MODE), (DIM, KIND, MODE), 8)
 
 // ------------------------------------------------------------- forces
 // SOLVER 0: acceleration_standard_fn (solver.py:221-256) + transport velocity
@@ -276,7 +276,7 @@ __device__ __forceinline__ void force_run(const SweepArgs& a, const int blk) {
   const float inv_rhoi = 1.0f / rhoi;
   const float Vi2 = (mi / rhoi) * (mi / rhoi);
   const V3 ui = v3(Ui), vi = v3(Vi);
-  const V3 Ai = rhoi * ui;          // rho_i u_i
+  const V3 hAi = (0.5f * rhoi) * ui;  // rho_i u_i / 2
   const V3 wi = vi - ui;            // v_i - u_i
   const float pbg = eos.p_fn(0.0f);  // background pressure (solver.py:802)
   const bool av = EXTRAS && p.artificial_alpha != 0.0f;
@@ -295,8 +295,11 @@ __device__ __forceinline__ void force_run(const SweepArgs& a, const int blk) {
   // every particle has the same viscosity (then eta_j = eta_i, the same value).
   constexpr bool kLean = SOLVER == 0 && !EXTRAS;
   const bool need_q = !kLean || *a.eta_varies != 0;
-  for_each_pair(a, s, h, [&](const int j) {
-    const float4 Pj = a.R.P[j];
+  const float eta_ii = 2.0f * etai * etai * rcp_fast(etai + etai + kEps);
+  struct Rec { float4 P; };
+  const auto load = [&](const int j) { Rec r; r.P = a.R.P[j]; return r; };
+  for_each_pair_ahead(a, s, h, load, [&](const int j, const Rec& rr) {
+    const float4 Pj = rr.P;
     const V3 dr = min_image<DIM>(Pi, Pj, a.geo);
     const float d2 = dist2<DIM>(dr.x, dr.y, dr.z);
     if (d2 == 0.0f) return;  // self pair / coincident: every term is exactly 0
@@ -314,13 +317,16 @@ __device__ __forceinline__ void force_run(const SweepArgs& a, const int blk) {
     const float inv_d = rcp_fast(d + kEps);
     const float Vj = mj * rcp_fast(rhoj);
     const float cpre = (Vi2 + Vj * Vj) * inv_mi * gw * inv_d;
-    const float eta_ij = 2.0f * etai * etaj * rcp_fast(etai + etaj + kEps);
+    // with one viscosity for all particles eta_ij is the particle's own value
+    float eta_ij = eta_ii;
+    if (need_q) eta_ij = 2.0f * etai * etaj * rcp_fast(etai + etaj + kEps);
     if (SOLVER == 0 || SOLVER == 2) {
       const float pij = (rhoj * pi_ + rhoi * pj) * rcp_fast(rhoi + rhoj);
       V3 t = (-pij) * dr + eta_ij * (ui - uj);
       if (TVF) {
         const V3 vj = v3(Wj);
-        const V3 Ar = 0.5f * (dot<DIM>(wi, dr) * Ai + dot<DIM>(vj - uj, dr) * (rhoj * uj));
+        // 0.5 (A_i + A_j) . r with A = rho u (v - u)^T
+        const V3 Ar = dot<DIM>(wi, dr) * hAi + (0.5f * dot<DIM>(vj - uj, dr) * rhoj) * uj;
         t = t + Ar;
       }
       acc = acc + cpre * t;
@@ -427,7 +433,8 @@ __device__ __forceinline__ void force_run(const SweepArgs& a, const int blk) {
   if (o.dTdt) o.dTdt[i] = heat ? dT : (a.in.dTdt ? a.in.dTdt[i] : 0.0f);
 }
 SPH_SWEEP_KERNELS(k_force, force_run, (int DIM, int KIND, int SOLVER, bool TVF, bool EXTRAS),
-                  (DIM, KIND, SOLVER, TVF, EXTRAS))
+                  (DIM, KIND, SOLVER, TVF, EXTRAS),
+                  (SOLVER == 0 && !EXTRAS && KIND >= 0 ? 8 : 1))
 
 // ------------------------------------------------------------- integrate
 // si_euler kick + transport velocity + drift + periodic wrap (integrator.py:26-30)

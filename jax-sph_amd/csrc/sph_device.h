@@ -57,15 +57,14 @@ __device__ __forceinline__ float metric_sq_exact(const float4 s, const float4 r,
   return acc;
 }
 
-// Minimum-image displacement for the physics sweeps: (a - k*side) - b with
-// k = rint((a - b)/side).  For every pair inside the cutoff this is the same
-// image as the exact formula (rounded differently by <= 1 ulp(side)) and
-// bit-identical to the pair build's (a - image) - b, so fused and separate
-// sweeps see the same distances.
+// Minimum-image displacement for the physics sweeps: d - k*side (one
+// rounding) with d = a - b and k = rint(d/side).  For every pair inside the
+// cutoff this is the same image as the exact formula, within about 1 ulp(side)
+// of it; the sweeps' cutoff test decides pairs that close to the cutoff with
+// the exact formula, so pair sets do not depend on this rounding.
 __device__ __forceinline__ float disp_fast(float a, float b, float side, float inv_side) {
-#pragma clang fp contract(off)
-  const float k = rintf((a - b) * inv_side);
-  return (a - k * side) - b;
+  const float d = a - b;
+  return __builtin_fmaf(-rintf(d * inv_side), side, d);
 }
 
 // Hardware reciprocal and square root (v_rcp_f32, v_sqrt_f32, about 1 ulp) for

@@ -161,8 +161,36 @@ __device__ __forceinline__ void for_each_pair(const SweepArgs& a, int s, int h, 
   int jn = c > 0 ? lst[0] : 0;
   for (int k = 0; k < c; ++k) {
     const int j = jn;
+    __builtin_assume(j >= 0);  // sorted indices: record addresses need no sign extension
     if (k + 1 < c) jn = lst[(int64_t)(k + 1) * a.npad];
     f(j);
+  }
+}
+
+// for_each_pair with the records of each pair requested one pair ahead:
+// load(j) gathers the records of pair k + 1 (its index was requested a pair
+// earlier still) before the body of pair k runs, so the gathers have a whole
+// body to arrive.  The records are gathered for every listed pair, also those
+// the body then skips; at the end the last pair's records are gathered again
+// instead of reading past the list.
+template <typename L, typename F>
+__device__ __forceinline__ void for_each_pair_ahead(const SweepArgs& a, int s, int h, L&& load,
+                                                    F&& f) {
+  const int c = a.cnt[h * a.npad + s];
+  if (c <= 0) return;
+  const int32_t* lst = a.ell + ((int64_t)h * a.lane_width) * a.npad + s;
+  int j = lst[0];
+  int jn = c > 1 ? lst[a.npad] : j;
+  __builtin_assume(j >= 0);
+  auto rec = load(j);
+  for (int k = 0; k < c; ++k) {
+    const int jc = j;
+    const auto rc = rec;
+    j = jn;
+    __builtin_assume(j >= 0);
+    if (k + 2 < c) jn = lst[(int64_t)(k + 2) * a.npad];
+    rec = load(j);
+    f(jc, rc);
   }
 }
 
@@ -229,14 +257,18 @@ __global__ void k_gather_state(SweepArgs a, const float* __restrict__ r, const f
 // body RUN<TARGS>(args, blk).  TPARAMS and TARGS are parenthesised template
 // parameter and argument lists.
 #define SPH_UNPAREN(...) __VA_ARGS__
-#define SPH_SWEEP_KERNELS(NAME, RUN, TPARAMS, TARGS)                                      \
+// MINW: waves per SIMD the register allocation must leave room for.  The
+// sweeps wait on gathers, and 8 waves hide more of that than the registers a
+// 7-wave allocation frees save; variants that would spill at 64 registers
+// pass 1 (no bound).
+#define SPH_SWEEP_KERNELS(NAME, RUN, TPARAMS, TARGS, MINW)                                \
   template <SPH_UNPAREN TPARAMS>                                                          \
-  __global__ __launch_bounds__(kB) void NAME(SweepArgs a) {                               \
+  __global__ __launch_bounds__(kB, MINW) void NAME(SweepArgs a) {                         \
     RUN<SPH_UNPAREN TARGS>(a, logical_block(a.remap));                                    \
   }                                                                                       \
   template <SPH_UNPAREN TPARAMS>                                                          \
-  __global__ __launch_bounds__(kB) void NAME##_ens(const SweepArgs* __restrict__ t,       \
-                                                   const EnsBlock* __restrict__ map) {    \
+  __global__ __launch_bounds__(kB, MINW) void NAME##_ens(const SweepArgs* __restrict__ t, \
+                                                         const EnsBlock* __restrict__ map) { \
     const EnsBlock m = map[blockIdx.x];                                                   \
     if (m.sim < 0) return;                                                                \
     RUN<SPH_UNPAREN TARGS>(t[m.sim], m.blk);                                              \
@@ -437,7 +469,7 @@ __device__ __forceinline__ void density_run(const SweepArgs& a, const int blk) {
   }
 }
 
-SPH_SWEEP_KERNELS(k_density, density_run, (int DIM, int KIND, int MODE), (DIM, KIND, MODE))
+SPH_SWEEP_KERNELS(k_density, density_run, (int DIM, int KIND, int MODE), (DIM, KIND, MODE), 8)
 
 
 // ------------------------------------------------------------- sweep plans

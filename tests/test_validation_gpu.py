@@ -15,9 +15,13 @@ The reference checks these by eye on plots; here each comparison has a bound
 set above the error measured at these resolutions, and the errors must fall
 as the resolution rises.  Measured (MI355X): 2D TGV -4.2% / -2.3% at t = 0.5
 and -8.3% / -4.4% at t = 1 (nx = 50 / 100); 3D TGV largest error over
-t <= 10 of 19.7% / 15.5% (nx = 24 / 32), under 3.4% for t <= 2; cavity
-max |u - u_Ghia| 0.034 (SPH) and 0.024 (SPH + TVF), max |v - v_Ghia| 0.046
-and 0.026.
+t <= 10 of 19.7% / 15.5% (nx = 24 / 32), under 3.4% for t <= 2.
+
+The cavity's centrelines at dx = 0.02 depend on the relaxed particle
+arrangement and on summation order at the level of a few hundredths: over
+relaxation seeds 123, 1, 2 and 3 (MI355X), max |u - u_Ghia| spans 0.024-0.032
+(SPH) and 0.022-0.044 (SPH + TVF), max |v - v_Ghia| 0.033-0.060 and
+0.022-0.050.  The bounds sit about 25% above the largest of these.
 """
 import json
 import os
@@ -71,7 +75,7 @@ def test_tgv3d_re100_energy_against_jax_fluids():
     assert worst[32] < worst[24]  # converges towards the reference solution
 
 
-@pytest.mark.parametrize("tvf,bu,bv", [(0.0, 0.045, 0.06), (1.0, 0.035, 0.035)])
+@pytest.mark.parametrize("tvf,bu,bv", [(0.0, 0.045, 0.075), (1.0, 0.055, 0.065)])
 def test_ldc_re100_centrelines_against_ghia(tvf, bu, bv):
     g = REFS["ldc_re100_ghia"]
     res = V.ldc_centrelines(dx=0.02, t_end=20.0, tvf=tvf)
