@@ -174,7 +174,7 @@ __device__ __forceinline__ void renorm_run(const SweepArgs& a, const int blk) {
   const float4 Vi = a.R.V0[s];
   a.R.U3[s] = make_float4(Ui.x, Ui.y, Ui.z, rho);
   a.R.Q3[s] = make_float4(pr, Qi.y, Qi.z, Qi.w);
-  a.R.W3[s] = make_float4(Vi.x, Vi.y, Vi.z, pr);
+  a.R.W3[s] = make_float4(Vi.x - Ui.x, Vi.y - Ui.y, Vi.z - Ui.z, pr);
 }
 SPH_SWEEP_KERNELS(k_renorm, renorm_run, (int DIM, int KIND), (DIM, KIND), 8)
 
@@ -247,7 +247,7 @@ __device__ __forceinline__ void walls_run(const SweepArgs& a, const int blk) {
   a.R.U2[s] = f4(u, rho);
   a.R.V2[s] = f4(v, Vi.w);
   a.R.Q2[s] = make_float4(pr, Qi.y, Qi.z, T);
-  a.R.W2[s] = f4(v, pr);
+  a.R.W2[s] = f4(v - u, pr);
 }
 SPH_SWEEP_KERNELS(k_walls, walls_run, (int DIM, int KIND, int MODE), (DIM, KIND, MODE), 8)
 
@@ -304,17 +304,18 @@ __device__ __forceinline__ void force_run(const SweepArgs& a, const int blk) {
     const float d2 = dist2<DIM>(dr.x, dr.y, dr.z);
     if (d2 == 0.0f) return;  // self pair / coincident: every term is exactly 0
     if (!in_cutoff<DIM>(a, d2, Pi, Pj)) return;
-    const float d = sqrt_fast(d2);
+    // 1/(d + EPS) is 1/d in float32 for every d above ~4e-9 (EPS = 2.2e-16)
+    const float inv_d = rsq_fast(d2);
+    const float d = d2 * inv_d;
     const float q = d * p.one_over_h;
     const float gw = K.dw(q);
     const float4 Uj = a.Uf[j];
     float4 Qj = make_float4(0.0f, etai, 0.0f, 0.0f);
     if (need_q) Qj = a.Qf[j];
-    float4 Wj = make_float4(0.0f, 0.0f, 0.0f, Qj.x);  // (v, p) of j
+    float4 Wj = make_float4(0.0f, 0.0f, 0.0f, Qj.x);  // (v - u, p) of j
     if (SOLVER != 1 && (TVF || !need_q)) Wj = a.Wf[j];
     const float mj = Pj.w, rhoj = Uj.w, pj = Wj.w, etaj = Qj.y;
     const V3 uj = v3(Uj);
-    const float inv_d = rcp_fast(d + kEps);
     const float Vj = mj * rcp_fast(rhoj);
     const float cpre = (Vi2 + Vj * Vj) * inv_mi * gw * inv_d;
     // with one viscosity for all particles eta_ij is the particle's own value
@@ -324,9 +325,8 @@ __device__ __forceinline__ void force_run(const SweepArgs& a, const int blk) {
       const float pij = (rhoj * pi_ + rhoi * pj) * rcp_fast(rhoi + rhoj);
       V3 t = (-pij) * dr + eta_ij * (ui - uj);
       if (TVF) {
-        const V3 vj = v3(Wj);
         // 0.5 (A_i + A_j) . r with A = rho u (v - u)^T
-        const V3 Ar = dot<DIM>(wi, dr) * hAi + (0.5f * dot<DIM>(vj - uj, dr) * rhoj) * uj;
+        const V3 Ar = dot<DIM>(wi, dr) * hAi + (0.5f * dot<DIM>(v3(Wj), dr) * rhoj) * uj;
         t = t + Ar;
       }
       acc = acc + cpre * t;
@@ -375,7 +375,7 @@ __device__ __forceinline__ void force_run(const SweepArgs& a, const int blk) {
       const V3 eq6 = (2.0f * mj * eta_ij * irr * inv_d * gw * mask) * vij;
       acc = acc + eq9 + eq6;
     }
-    accv = accv + (cpre * pbg) * dr;
+    accv = accv + cpre * dr;  // times p_bg after the sum
     if (av) {
       const bool fluid_j = (int)Qj.z == kFluid;
       if (fluid_i && fluid_j) {
@@ -397,7 +397,7 @@ __device__ __forceinline__ void force_run(const SweepArgs& a, const int blk) {
   if (heat) dT = lane_sum<TPP>(dT);
   if (h != 0) return;
   V3 dudt = acc + gi;
-  V3 dvdt = accv;
+  V3 dvdt = pbg * accv;
   V3 u_out = ui, v_out = vi;
   float p_out = pi_;
   // case boundary hook (applied to the output state, integrator.py:54)

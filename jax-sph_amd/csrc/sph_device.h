@@ -67,12 +67,14 @@ __device__ __forceinline__ float disp_fast(float a, float b, float side, float i
   return __builtin_fmaf(-rintf(d * inv_side), side, d);
 }
 
-// Hardware reciprocal and square root (v_rcp_f32, v_sqrt_f32, about 1 ulp) for
+// Hardware reciprocal, square root and reciprocal square root (v_rcp_f32,
+// v_sqrt_f32, v_rsq_f32, about 1 ulp) for
 // the pair sums, whose results are held to the reference at float32 rounding
 // noise; the compiler's IEEE forms cost 5-10 instructions each.  The
 // neighbour predicate never divides or takes roots, so pair sets stay exact.
 __device__ __forceinline__ float rcp_fast(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float sqrt_fast(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float rsq_fast(float x) { return __builtin_amdgcn_rsqf(x); }
 
 // x^2 + y^2 (+ z^2) with explicit fused multiply-adds (same bits everywhere).
 template <int DIM>

@@ -110,7 +110,7 @@ struct Recs {
   float* drho;
   float4 *U3, *Q3;                    // renormalised density stage
   float4 *DL0, *DL1, *DL2, *DG0, *DG1;  // delta-SPH matrices and gradient terms
-  float4 *W1, *W2, *W3;                // (v, p) after density, walls, renormalisation
+  float4 *W1, *W2, *W3;                // (v - u, p) after density, walls, renormalisation
 };
 
 struct SweepArgs {
@@ -128,7 +128,7 @@ struct SweepArgs {
   // which records hold the density stage's (u, rho) and (p, eta, tag, T):
   // U1/Q1, or U3/Q3 after renormalisation
   const float4 *Ud, *Qd;
-  const float4* Wf;     // (v, p) records the force sweep reads
+  const float4* Wf;     // (v - u, p) records the force sweep reads
   int32_t* eta_varies;  // sticky device flag: the gathered viscosities differ
   int heat_in_density;  // apply T += dt*dTdt in the density epilogue
   sph_bc_spec bc;
@@ -423,7 +423,7 @@ __device__ __forceinline__ int density_of(const SweepArgs& a, const int s, const
   a.R.U1[s] = make_float4(Ui.x, Ui.y, Ui.z, rho);
   a.R.Q1[s] = make_float4(pr, Qi.y, Qi.z, T);
   const float4 Vi = a.R.V0[s];
-  a.R.W1[s] = make_float4(Vi.x, Vi.y, Vi.z, pr);
+  a.R.W1[s] = make_float4(Vi.x - Ui.x, Vi.y - Ui.y, Vi.z - Ui.z, pr);
   a.R.drho[s] = drhodt;
   if (p.solver == 1 && p.bc_trick) {
     // u_tilde of every particle: Shepard average for no-slip walls, u otherwise
