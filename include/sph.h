@@ -162,7 +162,7 @@ int sph_nl_write_idx(sph_ws *ws, int bank, int32_t *idx, int64_t max_occupancy,
  * The reference rebuilds the neighbour list on every si_euler step
  * (integrator.py:46, dr_threshold = 0).  With skin lists, sph_step on bank 2
  * keeps pairs closer than cutoff + skin, rebuilds them on the device only
- * after some particle has moved more than 0.45 * skin since the last build,
+ * after some particle has moved more than 0.49 * skin since the last build,
  * and applies the reference predicate (partition.py:895-897) to every pair on
  * every step.  Pair sets, occupancy info[1], info[3] and the error bits are
  * those of a rebuild at every step; info[7] holds the largest skin list of one

@@ -222,7 +222,7 @@ extern "C" int sph_skin_enable(sph_ws* ws, float skin, int32_t width) {
   }
   S.skin = skin;
   S.reach_sq = (float)(reach * reach * (1.0 + 1e-6));
-  S.trigger_sq = (float)((0.45 * skin) * (0.45 * skin));
+  S.trigger_sq = (float)((0.49 * skin) * (0.49 * skin));
   S.ncell = ncell;
   S.lane_width = (int32_t)lw;
   S.on = true;

@@ -17,11 +17,13 @@ as the resolution rises.  Measured (MI355X): 2D TGV -4.2% / -2.3% at t = 0.5
 and -8.3% / -4.4% at t = 1 (nx = 50 / 100); 3D TGV largest error over
 t <= 10 of 19.7% / 15.5% (nx = 24 / 32), under 3.4% for t <= 2.
 
-The cavity's centrelines at dx = 0.02 depend on the relaxed particle
-arrangement and on summation order at the level of a few hundredths: over
-relaxation seeds 123, 1, 2 and 3 (MI355X), max |u - u_Ghia| spans 0.024-0.032
-(SPH) and 0.022-0.044 (SPH + TVF), max |v - v_Ghia| 0.033-0.060 and
-0.022-0.050.  The bounds sit about 25% above the largest of these.
+The cavity's centrelines at dx = 0.02 carry particle noise: single snapshots
+between t = 15 and 20 differ by 0.02-0.05, and the last one moves with the
+relaxed arrangement and with summation order.  The test therefore averages
+the snapshots at t = 15, 16, ..., 20.  Over relaxation seeds 123, 1, 2 and 3
+(MI355X) the averaged profiles give max |u - u_Ghia| 0.021-0.038 (SPH) and
+0.015-0.020 (SPH + TVF), max |v - v_Ghia| 0.022-0.035 and 0.015-0.020.  The
+bounds sit about 30% above the largest of these.
 """
 import json
 import os
@@ -75,10 +77,11 @@ def test_tgv3d_re100_energy_against_jax_fluids():
     assert worst[32] < worst[24]  # converges towards the reference solution
 
 
-@pytest.mark.parametrize("tvf,bu,bv", [(0.0, 0.045, 0.075), (1.0, 0.055, 0.065)])
+@pytest.mark.parametrize("tvf,bu,bv", [(0.0, 0.05, 0.046), (1.0, 0.027, 0.027)])
 def test_ldc_re100_centrelines_against_ghia(tvf, bu, bv):
     g = REFS["ldc_re100_ghia"]
-    res = V.ldc_centrelines(dx=0.02, t_end=20.0, tvf=tvf)
+    res = V.ldc_centrelines(dx=0.02, t_end=20.0, tvf=tvf, average=5.0)
+    assert res["snapshots"] == 6
     u = np.interp(g["y"], res["y"], res["u"])
     v = np.interp(g["x"], res["x"], res["v"])
     assert np.abs(u - np.array(g["u"])).max() < bu
