@@ -524,24 +524,45 @@ __device__ __forceinline__ void test_range(const BuildArgs& a, int s, int h, con
         const int b = c0 + (t0 + u) * TPP;
         pg[u] = a.pos[b < hi ? b : hi - 1];
       }
+      unsigned got = 0u;  // bit u: candidate t0 + u is a pair
+      if (a.fast) {
+        bool near = false;  // some candidate lies in the band around the cutoff
 #pragma unroll
-      for (int u = 0; u < G; ++u) {
-        const int t = t0 + u;
-        const int b = c0 + t * TPP;
-        const float4 pb = pg[u];
-        bool hit;
-        if (a.fast) {
+        for (int u = 0; u < G; ++u) {
+          const float4 pb = pg[u];
           const float d2 = dist2<DIM>(sx - pb.x, sy - pb.y, sz - pb.z);
-          hit = d2 < a.lo;
-          if (!hit && d2 < a.hi) hit = metric_sq_exact<DIM>(ps, pb, a.box, a.half) < a.cutoff_sq;
-        } else {
-          hit = metric_sq_exact<DIM>(ps, pb, a.box, a.half) < a.cutoff_sq;
+          got |= d2 < a.lo ? 1u << u : 0u;
+          near = near || (d2 >= a.lo && d2 < a.hi);
         }
-        if (t >= nb) hit = false;
-        if (a.mask_self && b == s) hit = false;
-        if (ORDERED && hit && !(a.perm[b] < sid)) hit = false;
-        mask |= hit ? (1u << t) : 0u;
+        if (near) {  // rare: the exact reference metric decides the band
+#pragma unroll
+          for (int u = 0; u < G; ++u) {
+            const float4 pb = pg[u];
+            const float d2 = dist2<DIM>(sx - pb.x, sy - pb.y, sz - pb.z);
+            if (d2 >= a.lo && d2 < a.hi &&
+                metric_sq_exact<DIM>(ps, pb, a.box, a.half) < a.cutoff_sq)
+              got |= 1u << u;
+          }
+        }
+      } else {
+#pragma unroll
+        for (int u = 0; u < G; ++u)
+          if (metric_sq_exact<DIM>(ps, pg[u], a.box, a.half) < a.cutoff_sq) got |= 1u << u;
       }
+      if (ORDERED) {  // candidates past the range are dropped before perm is read
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+          const int b = c0 + (t0 + u) * TPP;
+          if ((got >> u) & 1u && (t0 + u >= nb || !(a.perm[b] < sid))) got &= ~(1u << u);
+        }
+      }
+      mask |= got << t0;
+    }
+    // drop the candidates past the range, and the particle itself
+    if (nb < 32) mask &= (1u << nb) - 1u;
+    if (a.mask_self) {
+      const int ds = s - c0;
+      if (ds >= 0 && ds % TPP == 0 && ds / TPP < 32) mask &= ~(1u << (ds / TPP));
     }
     // candidate base + t TPP + h follows the particle's hits at smaller t on
     // every lane and at the same t on the lanes below h
@@ -555,15 +576,16 @@ __device__ __forceinline__ void test_range(const BuildArgs& a, int s, int h, con
     while (mask) {
       const int t = __ffs(mask) - 1;
       mask &= mask - 1u;
-      const unsigned below = (1u << t) - 1u;
-      int k = count;
+      const unsigned upto = (2u << t) - 1u;  // bits 0..t (all 32 for t = 31)
+      const unsigned below = upto >> 1;      // bits 0..t-1
+      unsigned k = (unsigned)count;
 #pragma unroll
-      for (int q = 0; q < TPP; ++q)
-        k += __popc(m[q] & below) + (q < h ? (int)((m[q] >> t) & 1u) : 0);
+      for (int q = 0; q < TPP; ++q) k += __popc(m[q] & (q < h ? upto : below));
       const int32_t b = c0 + t * TPP;
-      const int q = k % TPP, pos = k / TPP;
-      if (pos < kStage) stage0[pos * kBlock + q] = b;
-      else if (pos < a.lane_width) out0[((int64_t)q * a.lane_width + pos) * a.npad] = b;
+      const unsigned q = k % TPP, pos = k / TPP;
+      if (pos < (unsigned)kStage) stage0[pos * kBlock + q] = b;
+      else if (pos < (unsigned)a.lane_width)
+        out0[(int64_t)(q * (unsigned)a.lane_width + pos) * a.npad] = b;
     }
     count += total;
   }
