@@ -522,7 +522,9 @@ __device__ __forceinline__ void test_range(const BuildArgs& a, int s, int h, con
 #pragma unroll
       for (int u = 0; u < G; ++u) {
         const int b = c0 + (t0 + u) * TPP;
-        pg[u] = a.pos[b < hi ? b : hi - 1];
+        const int bc = b < hi ? b : hi - 1;
+        __builtin_assume(bc >= 0);  // sorted slots: no sign extension in the address
+        pg[u] = a.pos[bc];
       }
       unsigned got = 0u;  // bit u: candidate t0 + u is a pair
       if (a.fast) {
