@@ -91,6 +91,23 @@ __device__ __forceinline__ int wave_aggregated_inc(int32_t* __restrict__ counter
   return slot;
 }
 
+// atomicAdd(&counter[key], 1) for every active lane, with one atomic per run
+// of equal keys in consecutive lanes (particles in id order come in runs of
+// one cell), issued by all runs at once.  The active lanes must be a prefix
+// of the wavefront.  Returns nothing: the counts only.
+__device__ __forceinline__ void wave_run_inc(int32_t* __restrict__ counter, int key) {
+  const int lane = threadIdx.x & 63;
+  const int prev = __shfl_up(key, 1, 64);
+  const bool start = lane == 0 || prev != key;
+  const unsigned long long starts = __ballot(start);
+  const unsigned long long active = __ballot(1);
+  if (start) {
+    const unsigned long long later = lane == 63 ? 0ull : (starts >> (lane + 1)) << (lane + 1);
+    const int end = later ? __ffsll((long long)later) - 1 : 64 - __clzll((long long)active);
+    atomicAdd(&counter[key], end - lane);
+  }
+}
+
 template <int DIM>
 __device__ __forceinline__ void hash_count_body(const float* __restrict__ r, int n, const Grid& g,
                                                 int32_t* __restrict__ cell_count,
@@ -187,7 +204,7 @@ __device__ __forceinline__ void integrate_skin_body(
   }
   int c[3];
   cell_coords<DIM>(make_float4(x[0], x[1], x[2], 0.0f), g, c);
-  wave_aggregated_inc(cell_count, cell_hash<DIM>(c, g));
+  wave_run_inc(cell_count, cell_hash<DIM>(c, g));
   const float4 b = posb[i];
   const float pb[3] = {b.x, b.y, b.z};
   float m2 = 0.0f;

@@ -57,6 +57,8 @@ def test_skin_lists_equal_per_step_rebuild(case, ov, skin, steps, min_rebuilds):
         per_sender = np.bincount(want[1], minlength=n)
         assert int(info[3]) == int(per_sender.max()), f"step {k}: largest list"
         assert int(info[0]) == int(ref_nb.error), f"step {k}: error bits"
+        occ = nlo.count_cell_filling(r, R.ns.box, R.ns.cutoff, clamp=True).max()
+        assert int(info[2]) == int(occ), f"step {k}: largest cell occupancy"
     assert rebuilds >= min_rebuilds, f"only {rebuilds} rebuilds in {steps} steps"
     ref, ref64 = R.run(state, steps), R.run(state, steps, f64=True)
     G.assert_within_noise_floor(st, ref, ref64, 1e-5, fields=STATE, label=f"{case} skin")
