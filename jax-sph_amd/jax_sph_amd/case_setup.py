@@ -54,6 +54,7 @@ def defaults() -> Cfg:
     """defaults.py:6-148."""
     return Cfg.wrap({
         "config": None, "seed": 123, "no_jit": False, "gpu": 0, "dtype": "float32",
+        "xla_mem_fraction": 0.75,  # accepted for compatibility; unused here
         "case": {"source": None, "mode": "sim", "dim": 3, "dx": 0.05, "state0_path": None,
                  "state0_keys": ["r"], "r0_type": "cartesian", "r0_noise_factor": 0.0,
                  "g_ext_magnitude": 0.0, "viscosity": 0.01, "u_ref": 1.0, "c_ref_factor": 10.0,
@@ -103,8 +104,58 @@ def _merge(a: Dict, b: Dict):
     return a
 
 
+def _check_subset(superset: Dict, subset: Dict, full_key: str = "") -> None:
+    """main.py:9-17: every key of a config layer must exist in the defaults."""
+    for k, v in subset.items():
+        key = full_key + k
+        if k not in superset:
+            raise ValueError(f"cli_args must be a subset of the defaults. Wrong cli key: '{key}'")
+        if isinstance(v, dict) and isinstance(superset[k], dict):
+            _check_subset(superset[k], v, key + ".")
+
+
+def _load_yaml(path: str) -> Dict:
+    import yaml
+    with open(path) as f:
+        doc = yaml.safe_load(f)
+    return doc or {}
+
+
+def load_embedded_configs(path: str, overrides: Dict[str, Any] | None = None) -> Cfg:
+    """main.py:20-47: the YAML file ``path``, below it the files it ``extends``
+    (paths as written, up to ``JAX_SPH_DEFAULTS``), above it ``overrides``.
+    When the chain reaches the defaults, every key of the files and of the
+    overrides, ``case.special`` excepted, must be a default key."""
+    docs = [_load_yaml(path)]
+    from_defaults = False
+    while "extends" in docs[0]:
+        parent = docs[0].pop("extends")
+        if parent == "JAX_SPH_DEFAULTS":
+            from_defaults = True
+            break
+        docs.insert(0, _load_yaml(parent))
+    cfg = defaults() if from_defaults else Cfg()
+    if from_defaults:
+        for layer in docs + [overrides or {}]:
+            layer = {k: (dict(v) if isinstance(v, dict) else v) for k, v in layer.items()}
+            if isinstance(layer.get("case"), dict):
+                layer["case"].pop("special", None)
+            _check_subset(cfg, layer)
+    for layer in docs:
+        _merge(cfg, layer)
+    if overrides:
+        _merge(cfg, overrides)
+    cfg = Cfg.wrap(cfg)
+    cfg.config = path
+    return cfg
+
+
 def load_config(case: str, overrides: Dict[str, Any] | None = None) -> Cfg:
-    """defaults <- cases/<case>.yaml <- overrides (main.py:20-47 without files)."""
+    """defaults <- cases/<case>.yaml <- overrides (main.py:20-47).  ``case`` is
+    a case name (the case files' values are built in) or the path of a YAML
+    file in the reference's format (see :func:`load_embedded_configs`)."""
+    if case.endswith((".yaml", ".yml")):
+        return load_embedded_configs(case, overrides)
     cfg = defaults()
     _merge(cfg, CASE_YAML[case])
     if overrides:

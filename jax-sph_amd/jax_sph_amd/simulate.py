@@ -36,6 +36,8 @@ def simulate(cfg: Cfg, write_fn: Optional[Callable[[int, Dict], None]] = None,
     Case = load_case(cfg.case.source or cfg.config)
     (cfg, box, state, g_ext_fn, bc_fn, nw_fn, eos, key, displacement_fn,
      shift_fn) = Case(cfg).initialize(state0=state0)
+    if hasattr(write_fn, "bind"):  # a writer that sets up its directory (io_state.py:14-27)
+        write_fn.bind(cfg)
     solver = make_solver(cfg, displacement_fn, eos, g_ext_fn)
     forward = solver.forward_wrapper()
     n = state["r"].shape[0]
