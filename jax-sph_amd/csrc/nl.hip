@@ -186,21 +186,67 @@ __device__ __forceinline__ void integrate_skin_body(
     float bz, int32_t* __restrict__ cell_count, const float4* __restrict__ posb, float trigger_sq,
     int32_t* __restrict__ flag, int blk) {
 #pragma clang fp contract(off)
-  int i = blk * blockDim.x + threadIdx.x;
-  if (i >= n) return;
   const float side[3] = {bx, by, bz};
   float x[3] = {0.f, 0.f, 0.f};
+  int i;
+  if constexpr (DIM == 3) {
+    // The block's 3 x kBlock floats of each array pass through LDS, so every
+    // wave-wide load and store covers 64 consecutive floats; per-particle
+    // stride-3 accesses touch three times as many cache lines per instruction.
+    // Every block of a launch starts below n (launches use cdiv(n, kBlock)
+    // blocks), and every thread reaches the barriers.
+    __shared__ float s[4][3 * kBlock];
+    const int i0 = blk * kBlock;
+    i = i0 + threadIdx.x;
+    const int64_t base = (int64_t)i0 * 3;
+    const int lim = 3 * min(kBlock, n - i0);
+    for (int k = threadIdx.x; k < lim; k += kBlock) {
+      s[0][k] = r[base + k];
+      s[1][k] = u[base + k];
+      s[2][k] = dudt[base + k];
+      s[3][k] = dvdt[base + k];
+    }
+    __syncthreads();
+    float uo[3], vo[3];
 #pragma unroll
-  for (int d = 0; d < DIM; ++d) {
-    const int64_t k = (int64_t)i * DIM + d;
-    const float uu = u[k] + dt * dudt[k];
-    const float vv = uu + ctvf * dvdt[k];
-    float xx = r[k] + dt * vv;
-    if (xx >= side[d] || xx < 0.0f) xx = jmod(xx, side[d]);
-    u_out[k] = uu;
-    v_out[k] = vv;
-    r_out[k] = xx;
-    x[d] = xx;
+    for (int d = 0; d < 3; ++d) {
+      const int k = 3 * threadIdx.x + d;
+      uo[d] = s[1][k] + dt * s[2][k];
+      vo[d] = uo[d] + ctvf * s[3][k];
+      float xx = s[0][k] + dt * vo[d];
+      if (xx >= side[d] || xx < 0.0f) xx = jmod(xx, side[d]);
+      x[d] = xx;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const int k = 3 * threadIdx.x + d;
+      s[0][k] = x[d];
+      s[1][k] = uo[d];
+      s[2][k] = vo[d];
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < lim; k += kBlock) {
+      r_out[base + k] = s[0][k];
+      u_out[base + k] = s[1][k];
+      v_out[base + k] = s[2][k];
+    }
+    if (i >= n) return;
+  } else {
+    i = blk * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      const int64_t k = (int64_t)i * DIM + d;
+      const float uu = u[k] + dt * dudt[k];
+      const float vv = uu + ctvf * dvdt[k];
+      float xx = r[k] + dt * vv;
+      if (xx >= side[d] || xx < 0.0f) xx = jmod(xx, side[d]);
+      u_out[k] = uu;
+      v_out[k] = vv;
+      r_out[k] = xx;
+      x[d] = xx;
+    }
   }
   int c[3];
   cell_coords<DIM>(make_float4(x[0], x[1], x[2], 0.0f), g, c);
