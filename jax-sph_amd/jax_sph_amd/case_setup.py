@@ -16,6 +16,7 @@ from __future__ import annotations
 
 import copy
 import math
+import os
 from typing import Any, Dict
 
 import numpy as np
@@ -267,10 +268,17 @@ class SimulationSetup:
         self.name = cfg.case.source.split(".")[0]
         if cfg.case.mode not in ("sim", "rlx"):
             raise ValueError(f"case.mode must be 'sim' or 'rlx', not {cfg.case.mode!r}")
-        if cfg.case.r0_type != "cartesian":
-            # r0_type="relaxed" looks up data_relaxed/<name>.h5 (case_setup.py:277-306);
-            # here a relaxed state is passed as state0 (or an .npz state0_path) instead
-            raise NotImplementedError("r0_type 'relaxed' reads h5 files; pass state0 instead")
+        # the step computes in float32 only (the library's kernels); the
+        # reference's float64 mode (defaults.py:20, main.py:68-69) is refused
+        # rather than silently run at the lower precision
+        if str(cfg.get("dtype", "float32")) != "float32":
+            raise ValueError(f"dtype {cfg.dtype!r} is not supported: the SPH step computes "
+                             "in float32 only (set dtype=float32)")
+        if cfg.case.r0_type not in ("cartesian", "relaxed"):              # defaults.py:164-165
+            raise ValueError(f"r0_type must be in ['cartesian', 'relaxed'], "
+                             f"not {cfg.case.r0_type!r}")
+        if cfg.case.r0_type == "relaxed":                                  # case_setup.py:38-39
+            assert (cfg.case.state0_path is not None) and ("r" in cfg.case.state0_keys)
         # relaxation (case_setup.py:356-404): the case's own geometry at rest,
         # no body force, every wall at rest
         self.relax = cfg.case.mode == "rlx"
@@ -320,9 +328,35 @@ class SimulationSetup:
             return np.array([0.0, 1.0] if dim == 2 else [0.0, 1.0, 0.0]) * nw * dx
         return np.zeros(dim)
 
+    def relaxed_path(self) -> str:
+        """case_setup.py:281-283: ``data_relaxed/<case>_<dim>_<dx>_<seed>``, the
+        file a relaxation run (``case.mode=rlx``, ``io.data_path=data_relaxed/``)
+        writes; ``.npz`` here."""
+        c = self.cfg
+        return os.path.join("data_relaxed", "_".join(
+            [self.name, str(c.case.dim), str(c.case.dx), str(c.seed)]) + ".npz")
+
+    def _relaxed(self):
+        """case_setup.py:277-298: positions and tags of a relaxed state."""
+        path = self.relaxed_path()
+        if not os.path.isfile(path):
+            c = self.cfg
+            message = (f"python -m jax_sph_amd.main config={c.config} case.mode=rlx "
+                       f"seed={c.seed} case.dim={c.case.dim} case.dx={c.case.dx} "
+                       "solver.name=SPH solver.tvf=1 case.r0_noise_factor=0.25 "
+                       "io.write_type=['npz'] io.data_path=data_relaxed/")
+            raise FileNotFoundError(f"First execute this: \n{message}")
+        from .io_state import read_state
+        st = read_state(path)
+        return st["r"].astype(np.float32), st["tag"].astype(np.int32)
+
     def _positions(self, dim, dx, n_walls):
         name, sp = self.name, self.special
         f32 = np.float32
+        if self.case.r0_type == "relaxed" and name != "db":
+            # tgv/ldc/pf/cf take every particle and tag from the file
+            # (``_load_only_fluid = False``, cases/tgv.py:20-23, ldc.py:36-39)
+            return self._relaxed()
         if name == "tgv":
             box = self._box_size(dim, n_walls)
             r = pos_init_cartesian_2d(box, dx) if dim == 2 else pos_init_cartesian_3d(box, dx)
@@ -340,6 +374,11 @@ class SimulationSetup:
         else:
             off = self.offset_vec(dim)
         r_f = (lat + off.astype(f32)).astype(f32)
+        if self.case.r0_type == "relaxed":
+            # the dam break reads only the fluid and builds its walls
+            # (``_load_only_fluid = True``, cases/db.py:42-43, 76-80)
+            r_rel, tag_rel = self._relaxed()
+            r_f = r_rel[tag_rel == Tag.FLUID]
         r_w = self._walls(dx, n_walls, dim).astype(f32)
         r = np.concatenate([r_w, r_f]).astype(f32)
         tag = np.concatenate([np.full(len(r_w), Tag.SOLID_WALL, np.int32),

@@ -24,7 +24,7 @@ readable; a list two updates old raises ``StaleNeighborListError``.
 Skin lists (extra keyword ``skin``, default ``"auto"`` = 0.1 * cutoff; 0
 turns them off): when the fused ``si_euler`` step advances a list, it keeps
 pairs within ``cutoff + skin`` and rebuilds them on the device only after a
-particle has moved more than ``0.45 * skin``; the reference predicate is
+particle has moved more than ``0.49 * skin``; the reference predicate is
 applied to every pair on every step, so pair sets, occupancy and error bits
 are those of a per-step rebuild.  A list produced that way is stale after the
 next fused step.  If a skin list outgrows its buffer the step reports

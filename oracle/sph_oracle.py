@@ -457,6 +457,15 @@ class WCSPH:
         rgj = _seg(gj, i_s, N)
         return kg, V_i, V_j, L_i, L_j, rgi, rgj
 
+    # solver.py:147-152 and 359-364: the right state of a wall neighbour is the
+    # left (fluid) state extrapolated hydrostatically to the wall particle,
+    # p_R = p_L + rho_L g_i.(-r_ij), rho_R = rho_fn(p_R); a fluid neighbour's is
+    # its own state
+    def _riemann_right_state(self, wall_j, p_L, rho_L, g_i, dr, p_j, rho_j):
+        p_R = np.where(wall_j, p_L + rho_L * (g_i * -dr).sum(1), p_j)
+        rho_R = np.where(wall_j, self.eos.rho_fn(p_R), rho_j)
+        return p_R, rho_R
+
     # solver.py:108-164
     def _rho_evol_riemann(self, e_s, rho, mass, u, p, dr, gw, wall_j, nw, g_ext, i_s, j_s):
         t = self.t
@@ -466,8 +475,8 @@ class WCSPH:
         u_L = np.where(wall_j, (u_i * -n_j).sum(1), (u_i * -e_s).sum(1))
         p_L, rho_L = p[i_s], rho[i_s]
         u_R = np.where(wall_j, -u_L + t(2) * (u_j * n_j).sum(1), (u_j * -e_s).sum(1))
-        p_R = np.where(wall_j, p_L + rho_L * (g_ext[i_s] * -dr).sum(1), p[j_s])
-        rho_R = np.where(wall_j, self.eos.rho_fn(p_R), rho[j_s])
+        p_R, rho_R = self._riemann_right_state(wall_j, p_L, rho_L, g_ext[i_s], dr, p[j_s],
+                                               rho[j_s])
         U_avg = (u_L + u_R) / t(2)
         v_avg = (u_i + u_j) / t(2)
         rho_avg = (rho_L + rho_R) / t(2)
@@ -486,8 +495,8 @@ class WCSPH:
         u_L = np.where(wall_j, (u_i * -n_j).sum(1), (u_i * -e_s).sum(1))
         p_L, rho_L = p[i_s], rho[i_s]
         u_R = np.where(wall_j, -u_L + t(2) * (u_j * n_j).sum(1), (u_j * -e_s).sum(1))
-        p_R = np.where(wall_j, p_L + rho_L * (g_ext[i_s] * -dr).sum(1), p[j_s])
-        rho_R = np.where(wall_j, self.eos.rho_fn(p_R), rho[j_s])
+        p_R, rho_R = self._riemann_right_state(wall_j, p_L, rho_L, g_ext[i_s], dr, p[j_s],
+                                               rho[j_s])
         P_avg = (p_L + p_R) / t(2)
         rho_avg = (rho_L + rho_R) / t(2)
         eta_ij = t(2) * eta[i_s] * eta[j_s] / (eta[i_s] + eta[j_s] + eps)

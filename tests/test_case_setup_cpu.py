@@ -108,6 +108,49 @@ def test_state0_overwrites_fluid_positions_and_round_trips():
             device="cpu", state0=short)
 
 
+def test_relaxed_start_reads_the_relaxation_file(monkeypatch):
+    """case_setup.py:38-39, 277-298: r0_type="relaxed" needs a state0 path with
+    "r" among its keys and reads data_relaxed/<case>_<dim>_<dx>_<seed> (every
+    particle and tag for the cavity; only the fluid for the dam break, whose
+    walls are built); a missing file raises FileNotFoundError."""
+    with tempfile.TemporaryDirectory() as d:
+        monkeypatch.chdir(d)
+        for case, ov in (("ldc", {"case": {"dx": 0.04}}),
+                         ("db", {"case": {"dx": 0.08}})):
+            _, _, st0, *_ = _init(case, ov)
+            fluid = (st0["tag"] == Tag.FLUID)
+            rel = {k: v.clone() for k, v in st0.items()}
+            rel["r"][fluid] += 0.003
+            ov_rel = {"case": dict(ov["case"], r0_type="relaxed",
+                                   state0_path=os.path.join(d, "none.npz"))}
+            setup = cs.load_case(case)(cs.load_config(case, ov_rel))
+            with pytest.raises(FileNotFoundError):
+                setup.initialize(device="cpu")
+            os.makedirs("data_relaxed", exist_ok=True)
+            assert setup.relaxed_path() == os.path.join(
+                "data_relaxed", f"{case}_2_{ov['case']['dx']}_123.npz")
+            io_state.write_state(setup.relaxed_path(), rel)
+            # state0_path holds the same relaxed positions (a restart from them)
+            io_state.write_state(os.path.join(d, "none.npz"), rel)
+            st = setup.initialize(device="cpu")[2]
+            np.testing.assert_array_equal(st["r"].numpy(), rel["r"].numpy())
+            np.testing.assert_array_equal(st["tag"].numpy(), rel["tag"].numpy())
+        with pytest.raises(AssertionError):
+            cs.load_case("ldc")(cs.load_config("ldc", {"case": {"dx": 0.04,
+                                                                "r0_type": "relaxed"}}))
+        with pytest.raises(ValueError):
+            cs.load_case("ldc")(cs.load_config("ldc", {"case": {"r0_type": "noisy"}}))
+
+
+def test_float64_is_refused():
+    """The reference's dtype switch (defaults.py:20) selects float64; the step
+    here computes in float32 only, so any other dtype raises instead of
+    running at a precision the configuration did not ask for."""
+    with pytest.raises(ValueError, match="float32"):
+        cs.load_case("tgv")(cs.load_config("tgv", {"dtype": "float64"}))
+    cs.load_case("tgv")(cs.load_config("tgv", {"dtype": "float32"}))
+
+
 def test_wall_normals_point_into_the_fluid():
     """utils.py:169-194 at set-up: on each straight wall the normal is the
     inward axis direction (bottom +y, lid -y, left +x, right -x)."""

@@ -375,6 +375,129 @@ def test_riemann_wall_velocity_is_shepard_average():
     np.testing.assert_allclose(out["drhodt"][fl], 0.0, atol=1e-12)
 
 
+def _hydrostatic_riemann_tank(kernel, h_fac, planted_sign_slip=False):
+    """Closed channel (walls below and above, periodic in x) full of fluid at
+    rest in hydrostatic equilibrium p = rho0 g (H - y), with g = (0, -1) normal
+    to both walls: the RIE solver with continuity and wall normals.  Fluid
+    masses are rho dx^2, so every particle occupies the same volume.  With
+    ``planted_sign_slip`` the wall neighbour's extrapolated state uses
+    g.(+r_ij) instead of g.(-r_ij)."""
+    r, tag, box, dx = _channel()
+    gmag = 1.0
+    eos = so.RIEMANNEoS(1.0, 0.0, 1.0, np.float64)
+    s = _solver(box, dx, 2, solver="RIE", is_bc_trick=True, is_rho_evol=True, eos=eos,
+                kernel=kernel, h_fac=h_fac, g=lambda x: np.tile([0.0, -gmag], (len(x), 1)))
+    if planted_sign_slip:
+        right = s._riemann_right_state
+        s._riemann_right_state = lambda wj, pL, rL, g, dr, pj, rj: right(wj, pL, rL, g, -dr,
+                                                                         pj, rj)
+    st = H.base_state(r, dx=dx, eta=0.0, dtype=np.float64)
+    st["tag"] = tag
+    fl = tag == 0
+    p = gmag * (box[1] - r[:, 1])
+    st["p"] = np.where(fl, p, 0.0)
+    st["rho"] = np.where(fl, eos.rho_fn(p), 1.0)
+    st["mass"] = st["rho"] * dx * dx
+    nw = np.zeros_like(r)
+    nw[(tag == 1) & (r[:, 1] < box[1] / 2), 1] = 1.0
+    nw[(tag == 1) & (r[:, 1] > box[1] / 2), 1] = -1.0
+    st["nw"] = nw
+    out = s.forward(st, H.pairs(r, box, s.kernel.cutoff, np.float64))
+    y = r[:, 1]
+    band = 3 * dx + s.kernel.cutoff
+    near = fl & ((y < band) | (y > box[1] - band))
+    return out, fl, near, gmag
+
+
+@pytest.mark.parametrize("kernel,h_fac", [("QSK", 1.0), ("WC2K", 1.3)])
+def test_riemann_wall_state_is_hydrostatic_under_gravity(kernel, h_fac):
+    """solver.py:147-152, 359-364: a wall neighbour's right state is
+    p_R = p_L + rho_L g.(-r_ij), the fluid pressure carried hydrostatically to
+    the wall particle.  In a tank at rest in hydrostatic equilibrium that
+    makes the wall look like more fluid of the same column, so next to the
+    walls (gravity into the bottom wall, away from the top one) the net
+    acceleration dudt = -grad p / rho + g and the Riemann density rate vanish
+    to the kernel's consistency error, as they do in the core.  A sign slip in
+    the extrapolation leaves a net force of the order of g next to each wall."""
+    out, fl, near, g = _hydrostatic_riemann_tank(kernel, h_fac)
+    a, drho = np.abs(out["dudt"]), np.abs(out["drhodt"])
+    core = fl & ~near
+    # no more error next to the walls than in the core (lattice consistency error
+    # of the kernel: 0.5% of g for QSK, 3% for WC2K at h = 1.3 dx)
+    assert a[near].max() <= 1.2 * a[core].max() + 1e-3 * g, (a[near].max(), a[core].max())
+    assert a[near].max() < 0.05 * g
+    assert drho[near].max() < 1e-3
+    np.testing.assert_allclose(out["drhodt"][~fl], 0.0, atol=0)
+    bad, _, _, _ = _hydrostatic_riemann_tank(kernel, h_fac, planted_sign_slip=True)
+    assert np.abs(bad["dudt"][near]).max() > 0.5 * g
+    assert np.abs(bad["drhodt"][near]).max() > 0.05
+
+
+def _shepard_brute_force(r, box, kernel, walls, fluid, T):
+    """Fluid-only Shepard average at each wall particle over all fluid
+    particles (minimum-image distances, no pair list)."""
+    d = r[walls][:, None, :] - r[fluid][None, :, :]
+    d = d - np.round(d / box) * box
+    w = kernel.w(np.sqrt((d * d).sum(-1)))
+    ws = w.sum(1)
+    return (w * T[fluid][None, :]).sum(1) / np.where(ws > 0, ws, 1.0), ws
+
+
+@pytest.mark.parametrize("solver", ["SPH", "RIE"])
+@pytest.mark.parametrize("slope", [0.0, 2.0])
+def test_wall_temperature_is_fluid_shepard_average(solver, slope):
+    """solver.py:518-526 (GWBC) and 554-565 (Riemann heat BC): solid and moving
+    walls take the Shepard average of the fluid temperature over fluid
+    neighbours only; Dirichlet walls keep theirs; then every particle is
+    advanced with the incoming dT/dt (solver.py:836), and the new dT/dt sees
+    the walls' new temperature.
+
+    Fluid T = 1.7 + slope y; walls start at T = 5 with incoming dT/dt = 100
+    (fluid 0), so an average that took wall neighbours in, or a kick applied
+    before the boundary condition, changes the result.  The expected values
+    come from an all-pairs sum over the fluid, not from the pair list."""
+    r, tag, box, dx = _channel()
+    top = (tag == 1) & (r[:, 1] > box[1] / 2)
+    tag = tag.copy()
+    tag[top & (r[:, 0] < box[0] / 2)] = 2            # moving wall
+    tag[top & (r[:, 0] >= box[0] / 2)] = 3           # Dirichlet wall
+    kw = {"is_bc_trick": True, "is_rho_evol": True, "is_heat_conduction": True, "dt": 1e-4}
+    if solver == "RIE":
+        kw.update(solver="RIE", eos=so.RIEMANNEoS(1.0, 0.0, 1.0, np.float64))
+    s = _solver(box, dx, 2, **kw)
+    st = H.base_state(r, dx=dx, dtype=np.float64)
+    st["tag"] = tag
+    fl = tag == 0
+    st["T"] = np.where(fl, 1.7 + slope * r[:, 1], 5.0)
+    dTdt_in = np.where(fl, 0.0, 100.0)
+    st["dTdt"] = dTdt_in
+    st["kappa"] = np.full(len(r), 0.3)
+    st["Cp"] = np.full(len(r), 2.0)
+    out = s.forward(st, H.pairs(r, box, s.kernel.cutoff, np.float64))
+    kick = 1e-4 * dTdt_in
+    shep = (tag == 1) | (tag == 2)
+    sa, wsum = _shepard_brute_force(r, box, s.kernel, shep, fl, st["T"])
+    seen = wsum > 1e-3 * wsum.max()
+    np.testing.assert_allclose(out["T"][shep][seen], sa[seen] + kick[shep][seen], rtol=1e-12)
+    if slope == 0.0:
+        np.testing.assert_allclose(out["T"][shep][seen], 1.7 + 0.01, rtol=1e-12)
+    # a wall particle beyond every fluid particle's support gets 0 / (0 + EPS) = 0
+    far = np.flatnonzero(shep)[wsum == 0.0]
+    np.testing.assert_allclose(out["T"][far], kick[far], atol=1e-300)
+    np.testing.assert_array_equal(out["T"][tag == 3], 5.0 + kick[tag == 3])
+    np.testing.assert_allclose(out["T"][fl], st["T"][fl], rtol=0)
+    if slope != 0.0:
+        return
+    # the new dT/dt uses the boundary-corrected temperatures: fluid next to the
+    # Dirichlet wall (T = 5) heats up, fluid next to the Shepard walls sees a
+    # wall at about its own temperature
+    y = r[:, 1]
+    row = fl & (y > box[1] - 4 * dx) & (y < box[1] - 3 * dx)
+    hot = row & (r[:, 0] > box[0] * 0.6) & (r[:, 0] < box[0] * 0.9)
+    mild = row & (r[:, 0] > box[0] * 0.1) & (r[:, 0] < box[0] * 0.4)
+    assert out["dTdt"][hot].min() > 20 * np.abs(out["dTdt"][mild]).max()
+
+
 # ------------------------------------------------------------------- si_euler
 def test_si_euler_free_streaming_and_ordering():
     r, box, dx = _periodic_case(8, 2)
