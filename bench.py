@@ -112,7 +112,17 @@ def fwd_adjoint_leg(steps=100, every=10, seed=0):
         sg.replay(steps // every)
         return sg.state["u"].clone()
 
-    u_tgt = plain(state["u"]).double()
+    # the float64 oracle's target rollout (tests/golden/c4_target_u.npy), the
+    # target the committed finite differences were taken with; without the
+    # fixture, the float32 rollout of the analytic field
+    gdir = os.path.join(ROOT, "tests", "golden")
+    tpath = os.path.join(gdir, "c4_target_u.npy")
+    if os.path.exists(tpath):
+        u_tgt = torch.from_numpy(np.load(tpath, allow_pickle=False)).cuda()
+        target_kind = "float64 oracle rollout of the analytic field (tests/golden/c4_target_u.npy)"
+    else:
+        u_tgt = plain(state["u"]).double()
+        target_kind = "float32 rollout of the analytic field"
     plain(u0)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -123,10 +133,10 @@ def fwd_adjoint_leg(steps=100, every=10, seed=0):
 
     gr = adjoint.GraphRollout(state, solver, exact_fn, shift, dt, steps, checkpoint_every=every)
 
-    def fwd_adj():
+    def fwd_adj(loss_fn=lambda u: ((u - u_tgt) ** 2).mean()):
         uu = u0.clone().requires_grad_(True)
         u_n, _ = gr(uu, state["r"])
-        loss = ((u_n.double() - u_tgt) ** 2).mean()
+        loss = loss_fn(u_n.double())
         loss.backward()
         return loss, uu.grad
 
@@ -139,21 +149,25 @@ def fwd_adjoint_leg(steps=100, every=10, seed=0):
     out = {"config": f"2D TGV dx=0.01 ({n} particles), tvf=0, {steps} steps, checkpoint "
                      f"every {every}, loss = MSE(u_final, target rollout)",
            "t_forward_s": t_fwd, "t_forward_plus_adjoint_s": t_fa,
-           "cost_ratio": t_fa / t_fwd, "loss": float(loss)}
-    fx_path = os.path.join(ROOT, "tests", "golden", "c4_fd.json")
-    if os.path.exists(fx_path):
+           "cost_ratio": t_fa / t_fwd, "loss": float(loss), "target": target_kind}
+    fx_path = os.path.join(gdir, "c4_fd.json")
+    if os.path.exists(fx_path) and os.path.exists(tpath):
         fx = json.load(open(fx_path))
-        g = grad.double()
-        ad = np.array([float((g * torch.randn(n, 2, generator=gen).cuda().double()).sum())
-                       for _ in range(len(fx["fd"]))])
-        fd = np.array(fx["fd"])
-        rel = np.abs(ad - fd) / np.abs(fd)
+        dirs = [torch.randn(n, 2, generator=gen).cuda().double() for _ in range(len(fx["fd"]))]
+        w = torch.randn(n, 2, generator=gen).cuda().double()
+        _, g_lin = fwd_adj(lambda u: (w * u).sum())
+
+        def check(g, fd):
+            ad = np.array([float((g.double() * d).sum()) for d in dirs])
+            fd = np.array(fd)
+            rel = np.abs(ad - fd) / np.abs(fd)
+            return {"rel_err_vector": float(np.linalg.norm(ad - fd) / np.linalg.norm(fd)),
+                    "rel_err_median": float(np.median(rel)), "rel_err_max": float(np.max(rel))}
+
         out["grad_check"] = {"against": "float64 oracle central differences, eps=1e-3, "
-                                        "tests/golden/c4_fd.json",
-                             "directions": len(fd),
-                             "rel_err_vector": float(np.linalg.norm(ad - fd) / np.linalg.norm(fd)),
-                             "rel_err_median": float(np.median(rel)),
-                             "rel_err_max": float(np.max(rel))}
+                                        "tests/golden/c4_fd.json", "directions": len(dirs),
+                             **check(grad, fx["fd"]),
+                             "linear_functional": check(g_lin, fx["fd_linear"])}
     return out
 
 

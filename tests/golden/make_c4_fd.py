@@ -9,6 +9,15 @@ first draw); 32 directions = the next 32 draws of the same generator; eps =
 1e-3.  The values pin the gradient of the hand-written adjoint at this size
 (the reference would give it with jax.grad, which cannot run here).
 
+Also written, from the same rollouts:
+* tests/golden/c4_target_u.npy: the float64 target rollout's final velocities
+  (n, 2), the target both the test and the benchmark use, so that the loss
+  differentiated on the GPU is the loss the differences were taken of;
+* "fd_linear": central differences of the linear functional <w, u_final>,
+  w = the draw after the 32 directions.  Its gradient does not depend on the
+  rollout's residual u_final - u_target, so it pins the transposed step at
+  full size independently of the float32 forward's rounding.
+
 Run from the repository root: ``python tests/golden/make_c4_fd.py`` (about
 40 CPU-minutes, spread over worker processes).
 """
@@ -39,15 +48,16 @@ def setup():
     gen = torch.Generator().manual_seed(SEED)
     u0 = (state["u"] + 0.05 * torch.randn(n, 2, generator=gen)).numpy().astype(np.float32)
     dirs = [torch.randn(n, 2, generator=gen).numpy().astype(np.float32) for _ in range(N_DIRS)]
+    w = torch.randn(n, 2, generator=gen).numpy().astype(np.float32)
     base = G.to_f64(G.to_np(state))
-    return cfg, box, solver, base, u0, dirs
+    return cfg, box, solver, base, u0, dirs, w
 
 
 def rollout(u0_64):
     from oracle import nl_oracle as nlo
     from oracle import sph_oracle as so
     from tests import gpu_helpers as G
-    cfg, box, solver, base, _, _ = CTX
+    cfg, box, solver, base = CTX[:4]
     ns = nlo.NeighborSearch(box, solver._kernel_fn.cutoff, mask_self=False, dtype=np.float64)
     adv = so.si_euler(0.0, G.oracle_solver64(solver, box).forward, box, lambda s: s, ns.update,
                       np.float64)
@@ -61,7 +71,7 @@ def rollout(u0_64):
 
 
 def job(k):
-    _, _, _, base, u0, dirs = CTX
+    _, _, _, base, u0, dirs, _ = CTX
     if k < 0:
         return k, rollout(base["u"])          # target from the analytic field
     kk, sign = divmod(k, 2)
@@ -84,17 +94,23 @@ if __name__ == "__main__":
     with mp.Pool(int(os.environ.get("WORKERS", "7")), initializer=init) as pool:
         res = dict(pool.map(job, jobs, chunksize=1))
     target = res[-1]
-    fd = []
+    w = CTX[6].astype(np.float64)
+    fd, fd_lin = [], []
     for k in range(N_DIRS):
         lp = float(np.mean((res[2 * k] - target) ** 2))
         lm = float(np.mean((res[2 * k + 1] - target) ** 2))
         fd.append((lp - lm) / (2 * EPS))
+        fd_lin.append(float((w * (res[2 * k] - res[2 * k + 1])).sum()) / (2 * EPS))
+    np.save(os.path.join(ROOT, "tests", "golden", "c4_target_u.npy"), target)
     out = {"problem": "2D TGV dx=0.01 (10000 particles), tvf=0, QSK, Tait c0=10, nu=0.01, "
                       f"{STEPS} si_euler steps, dt={CTX[0].solver.dt!r}",
            "loss": "mean((u_final - u_target)^2) over all particles and components",
            "u0": "analytic + 0.05 * torch.randn(n, 2, generator=torch.Generator().manual_seed(0))",
            "directions": f"next {N_DIRS} torch.randn(n, 2) draws of the same generator",
            "eps": EPS, "steps": STEPS, "seed": SEED, "dtype": "float64 oracle",
-           "fd": fd, "seconds": time.time() - t0}
+           "target": "tests/golden/c4_target_u.npy: float64 final u of the analytic start",
+           "fd": fd,
+           "linear": "sum(w * u_final), w = the torch.randn(n, 2) draw after the directions",
+           "fd_linear": fd_lin, "seconds": time.time() - t0}
     json.dump(out, open(os.path.join(ROOT, "tests", "golden", "c4_fd.json"), "w"), indent=1)
     print(json.dumps({k: v for k, v in out.items() if k != "fd"}), fd[:4])
