@@ -138,6 +138,17 @@ int sph_ws_tune(sph_ws *ws, const char *key, int64_t value);
  * emulated cell capacity; `counts` is caller scratch of prod(cps) int32. */
 int sph_nl_fill_max(int32_t dim, int32_t n, const float *r, const float *cell_size,
                     const int32_t *cps, int32_t *counts, int64_t *info, hipStream_t stream);
+/* sph_nl_fill_max without caller scratch and without any allocation.  No
+ * counterpart in the reference: it is for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The cell counts live in a static device array of
+ * SPH_FILL_MAX_STATIC_CELLS int32 (8 MiB) per GPU, held until the process
+ * ends; a grid with more cells returns SPH_ERR_SIZE.  Every call on one GPU
+ * shares that array (cleared by a kernel at the start of each call), so calls
+ * that overlap on one GPU (different streams or threads) race. */
+#define SPH_FILL_MAX_STATIC_CELLS (1 << 21)
+int sph_nl_fill_max_static(int32_t dim, int32_t n, const float *r, const float *cell_size,
+                           const int32_t *cps, int64_t *info, hipStream_t stream);
 /* Rebuild test of update (partition.py:1010-1018): info[4] = any(|r - r_ref|^2 > thr_sq). */
 int sph_nl_needs_rebuild(sph_ws *ws, const float *r, const float *r_ref, float threshold_sq,
                          int64_t *info, hipStream_t stream);

@@ -10,6 +10,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <mutex>
+#include <string>
 
 #include "ens.h"
 #include "sph_device.h"
@@ -1444,22 +1446,30 @@ static int check_ws(sph_ws* ws, int bank) {
   return SPH_OK;
 }
 
-extern "C" int sph_nl_fill_max(int32_t dim, int32_t n, const float* r, const float* cell_size,
-                               const int32_t* cps, int32_t* counts, int64_t* info,
-                               hipStream_t st) {
-  if (!r || !cell_size || !cps || !counts || !info || n < 0 || (dim != 2 && dim != 3)) {
-    set_error("sph_nl_fill_max: bad argument");
-    return SPH_ERR_ARG;
+namespace sph {
+// Grid of a cell-occupancy count; false (with the error set) on a bad argument.
+static bool fill_grid(const char* who, int32_t dim, int32_t n, const float* r,
+                      const float* cell_size, const int32_t* cps, const int64_t* info, Grid* g,
+                      int64_t* nc) {
+  if (!r || !cell_size || !cps || !info || n < 0 || (dim != 2 && dim != 3)) {
+    set_error(std::string(who) + ": bad argument");
+    return false;
   }
-  Grid g;
-  int64_t nc = 1;
+  *nc = 1;
   for (int d = 0; d < 3; ++d) {
-    g.cs[d] = d < dim ? cell_size[d] : 1.0f;
-    g.cps[d] = d < dim ? cps[d] : 1;
-    if (g.cps[d] < 1) { set_error("cps must be >= 1"); return SPH_ERR_ARG; }
-    nc *= g.cps[d];
+    g->cs[d] = d < dim ? cell_size[d] : 1.0f;
+    g->cps[d] = d < dim ? cps[d] : 1;
+    if (g->cps[d] < 1) { set_error("cps must be >= 1"); return false; }
+    *nc *= g->cps[d];
   }
-  g.use_cells = 1;
+  g->use_cells = 1;
+  return true;
+}
+
+// Clears `counts` (nc cells), histograms r into it and writes the largest
+// count to info[2]; every buffer is cleared by a kernel, none by a memset.
+static int fill_max_run(int32_t dim, int32_t n, const float* r, const Grid& g, int64_t nc,
+                        int32_t* counts, int64_t* info, hipStream_t st) {
   sph_ws* ws = nullptr;  // no profiling without a workspace
   SPH_LAUNCH(ws, "fill", st, k_fill<int32_t><<<cdiv(nc, kBlock), kBlock, 0, st>>>(counts, nc, 0));
   SPH_LAUNCH(ws, "fill", st, k_fill<int64_t><<<1, 64, 0, st>>>(info + 2, 1, 0));
@@ -1477,6 +1487,68 @@ extern "C" int sph_nl_fill_max(int32_t dim, int32_t n, const float* r, const flo
   blocks = blocks > 1024 ? 1024 : blocks;
   SPH_LAUNCH(ws, "max_reduce", st, k_max_reduce<<<blocks, kBlock, 0, st>>>(counts, nc, info + 2));
   return SPH_OK;
+}
+
+// Cell counts of sph_nl_fill_max_static: one copy per GPU, held until the
+// process ends.  Host code never passes this name to a kernel (on the host it
+// is a stand-in, not a device address); it uses the per-device address that
+// hipGetSymbolAddress returns.
+static __device__ __align__(256) int32_t scratch[SPH_FILL_MAX_STATIC_CELLS];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static int32_t* g_scratch_ptr[kMaxDevices];  // nullptr until looked up on that device
+
+// Device address of `scratch` on the current device, looked up once per device.
+static int scratch_of_current_device(int32_t** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    set_error("sph_nl_fill_max_static: no current HIP device");
+    return SPH_ERR_HIP;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      set_error(std::string("sph_nl_fill_max_static: hipGetSymbolAddress: ") +
+                hipGetErrorString(e));
+      return SPH_ERR_HIP;
+    }
+    g_scratch_ptr[dev] = static_cast<int32_t*>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
+  return SPH_OK;
+}
+}  // namespace sph
+
+extern "C" int sph_nl_fill_max(int32_t dim, int32_t n, const float* r, const float* cell_size,
+                               const int32_t* cps, int32_t* counts, int64_t* info,
+                               hipStream_t st) {
+  Grid g;
+  int64_t nc = 0;
+  if (!counts) { set_error("sph_nl_fill_max: bad argument"); return SPH_ERR_ARG; }
+  if (!fill_grid("sph_nl_fill_max", dim, n, r, cell_size, cps, info, &g, &nc)) return SPH_ERR_ARG;
+  return fill_max_run(dim, n, r, g, nc, counts, info, st);
+}
+
+extern "C" int sph_nl_fill_max_static(int32_t dim, int32_t n, const float* r,
+                                      const float* cell_size, const int32_t* cps, int64_t* info,
+                                      hipStream_t st) {
+  Grid g;
+  int64_t nc = 0;
+  if (!fill_grid("sph_nl_fill_max_static", dim, n, r, cell_size, cps, info, &g, &nc))
+    return SPH_ERR_ARG;
+  if (nc > SPH_FILL_MAX_STATIC_CELLS) {
+    set_error("sph_nl_fill_max_static: the grid has " + std::to_string(nc) +
+              " cells, more than the " + std::to_string(SPH_FILL_MAX_STATIC_CELLS) +
+              " its static scratch holds (use sph_nl_fill_max with caller scratch)");
+    return SPH_ERR_SIZE;
+  }
+  int32_t* counts = nullptr;
+  const int rc = scratch_of_current_device(&counts);
+  if (rc != SPH_OK) return rc;
+  return fill_max_run(dim, n, r, g, nc, counts, info, st);
 }
 
 extern "C" int sph_nl_needs_rebuild(sph_ws* ws, const float* r, const float* r_ref,

@@ -110,6 +110,8 @@ def lib():
         "sph_ws_tune": (c_int, [vp, c_char_p, c_int64]),
         "sph_nl_fill_max": (c_int, [c_int32, c_int32, vp, POINTER(c_float), POINTER(c_int32), vp,
                                     vp, vp]),
+        "sph_nl_fill_max_static": (c_int, [c_int32, c_int32, vp, POINTER(c_float),
+                                           POINTER(c_int32), vp, vp]),
         "sph_nl_needs_rebuild": (c_int, [vp, vp, vp, c_float, vp, vp]),
         "sph_nl_build": (c_int, [vp, c_int, vp, vp, c_int64, vp, vp]),
         "sph_nl_finalize": (c_int, [vp, c_int64, vp, vp]),
