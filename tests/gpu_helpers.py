@@ -125,3 +125,40 @@ def oracle_solver64(s: "js.solver.WCSPH", box):
                     s._kernel_fn.h / s.dx, s.is_bc_trick, s.is_rho_evol, s.artificial_alpha,
                     s.is_free_slip, s.is_heat_conduction, np.float64, diff_delta=s.diff_delta,
                     diff_alpha=s.diff_alpha, is_rho_renorm=s.is_rho_renorm)
+
+
+class OracleRef:
+    """float32 and float64 oracle rollouts of one case (the oracle's own
+    neighbour search rebuilt every step, as the reference's si_euler does)."""
+
+    def __init__(self, cfg, box, solver, bc):
+        self.cfg, self.box = cfg, box
+        self.ns = nlo.NeighborSearch(box, solver._kernel_fn.cutoff, mask_self=False)
+        self.ns64 = nlo.NeighborSearch(box, solver._kernel_fn.cutoff, mask_self=False,
+                                       dtype=np.float64)
+        bcf = oracle_bc(bc, cfg.case.dim)
+        self.adv = so.si_euler(cfg.solver.tvf, oracle_solver(solver, box).forward, box, bcf,
+                               self.ns.update)
+        self.adv64 = so.si_euler(cfg.solver.tvf, oracle_solver64(solver, box).forward, box,
+                                 bcf, self.ns64.update, np.float64)
+
+    def run(self, state, steps, dt=None, f64=False):
+        dt = self.cfg.solver.dt if dt is None else dt
+        st = to_np(state)
+        ns, adv = (self.ns64, self.adv64) if f64 else (self.ns, self.adv)
+        if f64:
+            st = to_f64(st)
+        nb = ns.allocate(st["r"])
+        for _ in range(steps):
+            st, nb = adv(dt, st, nb)
+        return st
+
+
+def periodic_rel_err(a, b, box):
+    """rel_err of positions under the minimum image: max |a - b|_image / max |b|."""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    L = np.asarray(box, np.float64)[: a.shape[1]]
+    d = a - b
+    d -= L * np.round(d / L)
+    return float(np.abs(d).max() / np.abs(b).max())

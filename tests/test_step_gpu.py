@@ -17,32 +17,6 @@ from jax_sph_amd.utils import get_ekin  # noqa: E402
 from tests import gpu_helpers as G  # noqa: E402
 
 
-class _Ref:
-    """float32 and float64 oracle rollouts of the same case."""
-
-    def __init__(self, cfg, box, solver, bc):
-        self.cfg, self.box = cfg, box
-        self.ns = nlo.NeighborSearch(box, solver._kernel_fn.cutoff, mask_self=False)
-        self.ns64 = nlo.NeighborSearch(box, solver._kernel_fn.cutoff, mask_self=False,
-                                       dtype=np.float64)
-        bcf = G.oracle_bc(bc, cfg.case.dim)
-        self.adv = so.si_euler(cfg.solver.tvf, G.oracle_solver(solver, box).forward, box, bcf,
-                               self.ns.update)
-        self.adv64 = so.si_euler(cfg.solver.tvf, G.oracle_solver64(solver, box).forward, box,
-                                 bcf, self.ns64.update, np.float64)
-
-    def run(self, state, steps, dt=None, f64=False):
-        dt = self.cfg.solver.dt if dt is None else dt
-        st = G.to_np(state)
-        ns, adv = (self.ns64, self.adv64) if f64 else (self.ns, self.adv)
-        if f64:
-            st = G.to_f64(st)
-        nb = ns.allocate(st["r"])
-        for _ in range(steps):
-            st, nb = adv(dt, st, nb)
-        return st
-
-
 def _make(case, ov, backend="jaxmd_vmap", skin="auto"):
     cfg, box, state, g, bc, eos, disp, shift, solver = G.setup(case, ov)
     fn = partition.neighbor_list(disp, box, r_cutoff=solver._kernel_fn.cutoff, backend=backend,
@@ -50,7 +24,7 @@ def _make(case, ov, backend="jaxmd_vmap", skin="auto"):
                                  format=partition.Sparse, skin=skin)
     nb = fn.allocate(state["r"])
     adv = integrator.si_euler(cfg.solver.tvf, solver.forward_wrapper(), shift, bc, None)
-    return cfg, state, nb, adv, _Ref(cfg, box, solver, bc), fn, shift, solver, bc
+    return cfg, state, nb, adv, G.OracleRef(cfg, box, solver, bc), fn, shift, solver, bc
 
 
 STATE = ("r", "u", "v", "rho")
