@@ -32,6 +32,17 @@ FP32_VALU_TFLOPS = 157.3
 # density: reads r, mass, tag (5 x 4 B), writes rho, p (2 x 4 B)
 # nl_build: reads r (12 B), writes one int32 per pair (4 B x pairs / N)
 ALG_BYTES_3D = {"force": 80.0, "density": 28.0}
+# algorithmic float32 operations per pair inside the cutoff (add, mul, div,
+# sqrt = 1, FMA = 2), counted on the reference formulas, 3D, QSK:
+# density (solver.py:176-178, kernel.py:51-72): displacement 3, d^2 5, sqrt 1,
+#   q 1, W(q) 20 (three clamped fifth powers, weights, sigma), sum 1 = 31;
+# force (solver.py:199-256, tvf = 1, one viscosity): displacement 3, d^2 5,
+#   sqrt 1, q 1, W'(q) 17, 1/(d + EPS) 2, weighted volume 4, prefactor 2,
+#   eta_ij 5, p_ij 5, A_i r 9, A_j r 9, (A_i + A_j) r / 2 6, -p_ij r 3,
+#   eta_ij u_ij 6, sum of terms 6, times prefactor 3, accumulate 3,
+#   background-pressure acceleration 4 + accumulate 3 = 97.
+ALG_FLOPS_PER_PAIR_3D = {"force": 97.0, "density": 31.0}
+COUNTERS = os.path.join(ROOT, "profiles", "counters_tgv3d_64cubed_sweeps.json")
 
 
 def parse():
@@ -169,6 +180,80 @@ def fwd_adjoint_leg(steps=100, every=10, seed=0):
                              **check(grad, fx["fd"]),
                              "linear_functional": check(g_lin, fx["fd_linear"])}
     return out
+
+
+def fwd_adjoint_3d_leg(n_side=64, steps=100, every=10, seed=0):
+    """The 3D headline configuration differentiated: 3D TGV n_side^3 with the
+    transport-velocity formulation (tvf = 1), a rollout of `steps` si_euler
+    steps with a checkpoint every `every` steps, both passes replayed as HIP
+    graphs (adjoint.GraphRollout on exact pair lists), the gradient of
+    <w, u_final> + <w, r_final> with respect to the initial velocity.  Reports
+    t(forward with checkpoints + adjoint) against the plain forward of the
+    same rollout: graph-replayed fused steps on the skin lists the benchmark
+    times, and on exact lists rebuilt every step as the adjoint's forward
+    runs."""
+    import torch
+    from jax_sph_amd import adjoint, integrator, partition
+    from jax_sph_amd import case_setup as cs
+    from jax_sph_amd.graph import StepGraph
+    cfg = cs.load_config("tgv", {"case": {"dim": 3, "dx": 2 * np.pi / n_side,
+                                          "viscosity": 1 / 1600},
+                                 "solver": {"tvf": 1.0, "t_end": 1.0}})
+    (cfg, box, state, g_ext_fn, bc_fn, nw_fn, eos, key, disp, shift) = \
+        cs.load_case("tgv")(cfg).initialize()
+    solver = cs.make_solver(cfg, disp, eos, g_ext_fn)
+    dt = cfg.solver.dt
+    cut = solver._kernel_fn.cutoff
+    n = state["r"].shape[0]
+    gen = torch.Generator().manual_seed(seed)
+    u0 = (state["u"].cpu() + 0.05 * torch.randn(n, 3, generator=gen)).cuda()
+    w = torch.randn(n, 3, generator=gen).cuda()
+    adv = integrator.si_euler(1.0, solver.forward_wrapper(), shift, bc_fn, nw_fn)
+    timings = {}
+    for name, skin in (("t_forward_skin_s", "auto"), ("t_forward_exact_s", 0.0)):
+        fn = partition.neighbor_list(disp, box, r_cutoff=cut, capacity_multiplier=1.25,
+                                     mask_self=False, format=partition.Sparse, skin=skin)
+        sg = StepGraph(adv, state, fn.allocate(state["r"]), dt, steps_per_replay=every)
+        base = {k: v.clone() for k, v in sg.state.items()}
+        for rep in range(2):   # warm-up, then timed
+            for k in sg.state:
+                sg.state[k].copy_(base[k])
+            sg.state["u"].copy_(u0)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            sg.replay(steps // every)
+            torch.cuda.synchronize()
+            timings[name] = time.perf_counter() - t0
+        assert not sg.did_buffer_overflow
+        del sg, base
+    exact_fn = partition.neighbor_list(disp, box, r_cutoff=cut, capacity_multiplier=1.25,
+                                       mask_self=False, format=partition.Sparse, skin=0.0)
+    t0 = time.perf_counter()
+    gr = adjoint.GraphRollout(state, solver, exact_fn, shift, dt, steps, tvf=1.0,
+                              checkpoint_every=every)
+    setup_s = time.perf_counter() - t0
+
+    def fwd_adj():
+        uu = u0.clone().requires_grad_(True)
+        u_n, r_n = gr(uu, state["r"])
+        ((w * u_n.double()).sum() + (w * r_n.double()).sum()).backward()
+        return uu.grad
+
+    fwd_adj()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    grad = fwd_adj()
+    torch.cuda.synchronize()
+    t_fa = time.perf_counter() - t0
+    assert bool(torch.isfinite(grad).all())
+    return {"config": f"3D TGV {n_side}^3 ({n} particles), tvf=1, Re=1600, {steps} steps, "
+                      f"checkpoint every {every}, gradient of <w,u_final>+<w,r_final> in u0",
+            "t_forward_skin_s": timings["t_forward_skin_s"],
+            "t_forward_exact_s": timings["t_forward_exact_s"],
+            "t_forward_plus_adjoint_s": t_fa, "graph_setup_s": setup_s,
+            "cost_ratio": t_fa / timings["t_forward_skin_s"],
+            "cost_ratio_vs_exact_forward": t_fa / timings["t_forward_exact_s"],
+            "particle_steps_per_s_fwd_adjoint": n * steps / t_fa}
 
 
 ENSEMBLE_DX = (0.025, 0.02, 0.0125, 0.01, 0.005)
@@ -357,7 +442,8 @@ def main():
     ws.prof_enable(False)
     prof = ws.prof_read()
     overflow |= int(nb.error.code)
-    # dominant kernel and its roofline
+    # dominant kernel and its two roofline legs: algorithmic bytes against HBM
+    # bandwidth, algorithmic flops against the float32 vector peak
     name, (tot_ms, cnt) = max(prof.items(), key=lambda kv: kv[1][0])
     avg_s = tot_ms / cnt / 1e3
     pairs = nb.occupancy
@@ -365,14 +451,48 @@ def main():
     alg["nl_build"] = 12.0 + 4.0 * pairs / N
     per_launch = alg.get(name, float("nan")) * N
     achieved = per_launch / avg_s / 1e9
+    flops = ALG_FLOPS_PER_PAIR_3D.get(name, float("nan")) * pairs
+    tflops = flops / avg_s / 1e12
     # HBM bytes per launch of that kernel: rocprofv3 FETCH_SIZE / WRITE_SIZE of
     # this workload, scaled by a plain copy's counter readings (profiles/)
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "hbm_traffic_tgv3d_64cubed_fused_step.json")
-    if n_side == 64 and a.skin == "auto" and os.path.exists(tfile):
+    here = n_side == 64 and a.skin == "auto"
+    if here and os.path.exists(tfile):
         for k, v in json.load(open(tfile))["kernels"].items():
             if k == f"k_{name}" or k.startswith(f"k_{name}<"):
                 traffic = v["traffic_bytes"]
+    # what bounds it, from the committed counters of this workload: the VALU
+    # issue share against the HBM bandwidth share of the measured traffic
+    counters = None
+    if here and os.path.exists(COUNTERS):
+        for k, v in json.load(open(COUNTERS))["kernels"].items():
+            if k == f"k_{name}" or k.startswith(f"k_{name}<"):
+                counters = v
+    hbm_leg = {"achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+               "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+               "alg_bytes_per_launch": per_launch,
+               "traffic_frac_of_peak": (traffic / avg_s / 1e9 / HBM_PEAK_GBS
+                                        if traffic else None)}
+    valu_leg = {"achieved": tflops, "peak": FP32_VALU_TFLOPS, "unit": "TFLOP/s",
+                "frac": tflops / FP32_VALU_TFLOPS, "alg_flops_per_launch": flops,
+                "issue_frac": counters["valu_issue_share"] if counters else None}
+    bound = "hbm"
+    if counters and counters["valu_issue_share"] > (hbm_leg["traffic_frac_of_peak"] or 0.0):
+        bound = "valu"
+    main_leg = valu_leg if bound == "valu" else hbm_leg
+    roofline = {"bound": bound, "kernel": name, "achieved": main_leg["achieved"],
+                "peak": main_leg["peak"], "unit": main_leg["unit"], "frac": main_leg["frac"],
+                "traffic": traffic, "us_mean": avg_s * 1e6,
+                "timing": "HIP events around every library launch, in a second pass of as "
+                          "many steps (the events add host gaps between kernels: see "
+                          "ms_per_step_profiled_pass; kernel durations agree with the "
+                          "rocprofv3 trace in profiles/)",
+                "hbm": hbm_leg, "valu": valu_leg,
+                "counters": ({"source": os.path.relpath(COUNTERS, ROOT),
+                              "valu_issue_share": counters["valu_issue_share"],
+                              "ta_busy_share": counters.get("ta_busy_share")}
+                             if counters else None)}
     kernels = {k: {"ms_total": round(v[0], 4), "launches": v[1],
                    "us_mean": round(v[0] / v[1] * 1e3, 2)} for k, v in
                sorted(prof.items(), key=lambda kv: -kv[1][0])}
@@ -393,17 +513,16 @@ def main():
                                f"QSK h=dx, Re=1600, c0=10, dt={dt:.6g}, fused si_euler step",
                    "particles": N, "pairs_per_step": pairs, "dt": dt,
                    "verlet_skin": ws.skin, "xcd_remap": a.xcd_remap},
-        "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic,
-                     "alg_bytes_per_launch": per_launch, "us_mean": avg_s * 1e6},
+        "roofline": roofline,
         "kernels": kernels,
         "ms_per_step_profiled_pass": el_prof / a.steps * 1e3,
         "partition_error_code": overflow,
         "fwd_adjoint": None,
+        "fwd_adjoint_3d": None,
     }
     if not a.no_adjoint:
         out["fwd_adjoint"] = fwd_adjoint_leg()
+        out["fwd_adjoint_3d"] = fwd_adjoint_3d_leg(n_side)
     out["ensemble"] = None if a.no_ensemble else ensemble_leg(steps=a.ensemble_steps)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(a.cpu_sample_side)

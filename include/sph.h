@@ -206,22 +206,29 @@ int sph_step(sph_ws *ws, int bank, const sph_solver_params *prm, float dt, float
              hipStream_t stream);
 
 /* ---- adjoint ------------------------------------------------------------- */
-/* Cotangents of the step's carried state (r, u, a = dudt), (N, dim), orig order. */
+/* Cotangents of the step's state, (N, dim), orig order: positions r, velocity
+ * u, acceleration a = dudt, and for the transport-velocity step the transport
+ * velocity v and its acceleration b = dvdt.  v and b of bar_next may be NULL
+ * (zero); b of bar may be NULL (not written). */
 typedef struct {
-  const float *r, *u, *a;
+  const float *r, *u, *a, *v, *b;
 } sph_cotangent_in;
 typedef struct {
-  float *r, *u, *a;
+  float *r, *u, *a, *b;
 } sph_cotangent_out;
 /* Vector-Jacobian product of one si_euler step (integrator.py:22-56), the
  * transpose that jax.grad builds for the reference's step.  Standard SPH,
- * density summation, no walls/heat/artificial viscosity, tvf = 0, QSK or WC2K.
- * `next` is state n+1 (r, u, rho, p, mass, eta, tag) and `bank` must hold the
- * pairs of next->r; given the cotangents of state n+1 writes those of state n
- * (bar may alias nothing in bar_next).  records_ready != 0 promises that the
- * last kernels on this workspace were the sph_step that produced `next` into
- * `bank` (its sorted records are reused instead of gathered again). */
-int sph_step_vjp(sph_ws *ws, int bank, const sph_solver_params *prm, float dt,
+ * density summation, no walls/heat/artificial viscosity, QSK or WC2K; with
+ * prm->tvf_stress the transport-velocity step (solver.py:199-256: the
+ * A = rho u (v - u) stress and the background-pressure dvdt), whose
+ * v' = u' + tvf_coef * dvdt (tvf_coef = tvf dt / 2, 0 without tvf).
+ * `next` is state n+1 (r, u, v with tvf, rho, p, mass, eta, tag) and `bank`
+ * must hold the pairs of next->r; given the cotangents of state n+1 writes
+ * those of state n (bar may alias nothing in bar_next).  records_ready != 0
+ * promises that the last kernels on this workspace were the sph_step that
+ * produced `next` into `bank` (its sorted records are reused instead of
+ * gathered again). */
+int sph_step_vjp(sph_ws *ws, int bank, const sph_solver_params *prm, float dt, float tvf_coef,
                  const sph_state_in *next, const sph_cotangent_in *bar_next,
                  sph_cotangent_out *bar, int32_t records_ready, hipStream_t stream);
 

@@ -2,22 +2,33 @@
 //
 // The reference differentiates its step with jax.grad (reverse mode through
 // XLA).  Here the transpose of the step is written out.  Scope: the standard
-// SPH solver with density summation, no walls, no heat, no artificial
-// viscosity and tvf = 0 (v = u), which is the 2D Taylor-Green inverse problem's
-// configuration.  State n -> n+1 (integrator.py:26-54, solver.py:176-178,
-// 221-256, 801):
+// SPH solver with density summation, no walls, no heat and no artificial
+// viscosity, with or without the transport-velocity formulation (tvf).  State
+// n -> n+1 (integrator.py:26-54, solver.py:176-178, 199-256, 801-802):
 //
-//   u' = u + dt a,   r' = wrap(r + dt u'),   a' = F(r', u')
+//   u' = u + dt a,   v' = u' + c b  (c = tvf dt / 2),   r' = wrap(r + dt v')
+//   a' = F(r', u', v'),   b' = G(r')
 //   rho_i = m_i sum_j W(d_ij),  p_i = P(rho_i)
-//   F_i   = sum_j c_ij [ -p_ij r_ij + eta_ij (u_i - u_j) ] + g
+//   F_i   = sum_j c_ij T_ij + g,   G_i = p_bg sum_j c_ij r_ij
+//   T_ij  = -p_ij r_ij + eta_ij (u_i - u_j)
+//           + [rho_i u_i (w_i . r_ij) + rho_j u_j (w_j . r_ij)] / 2,  w = v - u
 //   c_ij  = (V_i^2 + V_j^2) / m_i * W'(d) / (d + eps),  V = m / rho
 //   p_ij  = (rho_j p_i + rho_i p_j) / (rho_i + rho_j)
 //
-// Given the cotangents (rb', ub', ab') of state n+1 it returns those of state
-// n:  rb' += J_r^T ab',  ub' += J_u^T ab',  ub' += dt rb',  then
-// rb = rb', ub = ub', ab = dt ub'.  Both Jacobian products are gathers over the
-// pair list of r' (two sweeps: the second needs the density cotangent of the
-// neighbours), with no atomics, like the forward sweeps.
+// (without tvf, v' = u', w = 0 and G is not evaluated).  Since m_i c_ij is
+// symmetric and T_ij, r_ij antisymmetric, the cotangent-weighted sums read
+//   <abar', F> + <bbar', G> = 1/2 sum_i sum_j S_ij,
+//   S_ij = (V_i^2 + V_j^2) W'/(d + eps) [l_ij . T_ij + p_bg n_ij . r_ij],
+//   l_ij = abar'_i / m_i - abar'_j / m_j,  n_ij = bbar'_i / m_i - bbar'_j / m_j,
+// with S_ij = S_ji, so the derivative with respect to particle k's own
+// quantities is sum_j dS_kj / dx_k: one gather sweep over k's pairs.
+//
+// Given the cotangents (rbar', ubar', vbar', abar', bbar') of state n+1 it
+// returns those of state n: rbar' += J_r^T (abar', bbar'), ubar' += J_u^T abar',
+// vbar' += J_v^T abar', vbar' += dt rbar', ubar' += vbar', then rbar = rbar',
+// ubar = ubar', abar = dt ubar', bbar = c vbar'.  Both Jacobian products are
+// gathers over the pair list of r' (two sweeps: the second needs the density
+// cotangent of the neighbours), with no atomics, like the forward sweeps.
 #include <hip/hip_runtime.h>
 
 #include "sph_device.h"
@@ -29,29 +40,38 @@ namespace sph {
 constexpr int kAB = 256;
 
 struct AdjArgs {
-  SweepArgs s;               // records of state n+1 (P, U1 = (u, rho), Q1 = (p, eta, tag, .))
+  SweepArgs s;               // records of state n+1 (P, U1 = (u, rho), Q1 = (p, eta, tag, .),
+                             // V0 = (v, .))
   const float* rb_in;        // (N, d) cotangent of r'  (orig order)
   const float* ub_in;        // (N, d) cotangent of u'
   const float* ab_in;        // (N, d) cotangent of a' = dudt'
+  const float* vb_in;        // (N, d) cotangent of v', or null (zero)
+  const float* bb_in;        // (N, d) cotangent of b' = dvdt', or null (zero)
   float* rb_out;             // (N, d) cotangents of state n
   float* ub_out;
   float* ab_out;
+  float* bb_out;             // with tvf: cotangent of b = dvdt (null: not written)
   float4* la;                // sorted cotangent of a'
+  float4* lb;                // sorted cotangent of b' (tvf)
   float dt;
+  float c_tvf;               // v' = u' + c_tvf b
 };
 
-template <int DIM>
+template <int DIM, bool TVF>
 __global__ void k_adj_gather(AdjArgs a) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= a.s.n) return;
-  a.la[s] = f4(load_vec<DIM>(a.ab_in, a.s.perm[s]), 0.0f);
+  const int i = a.s.perm[s];
+  a.la[s] = f4(load_vec<DIM>(a.ab_in, i), 0.0f);
+  if (TVF) a.lb[s] = f4(a.bb_in ? load_vec<DIM>(a.bb_in, i) : V3{0.f, 0.f, 0.f}, 0.0f);
 }
 
-// Sweep A: density cotangent (times m, fluid only), the u' cotangent of F and
-// the direct r' cotangent of F, per sorted particle k.
-template <int DIM, int KIND>
+// Sweep A: density cotangent (times m, fluid only), the u' and v' cotangents
+// of F and the direct r' cotangent of F and G, per sorted particle k.
+template <int DIM, int KIND, bool TVF>
 __global__ __launch_bounds__(kAB) void k_adj_forces(AdjArgs aa, float4* __restrict__ ab0,
-                                                    float4* __restrict__ ab1) {
+                                                    float4* __restrict__ ab1,
+                                                    float4* __restrict__ ab2) {
   constexpr int TPP = tpp_of(DIM);
   const SweepArgs& a = aa.s;
   const int tid = logical_block(a.remap) * blockDim.x + threadIdx.x;
@@ -66,8 +86,13 @@ __global__ __launch_bounds__(kAB) void k_adj_forces(AdjArgs aa, float4* __restri
   const float dPk = eos.dp_drho(rhok);
   const float inv_mk = 1.0f / mk;
   const V3 uk = v3(Uk), lk = v3(Lk4);
+  // transport-velocity terms: w = v - u, the background pressure of G and the
+  // sorted cotangent of b'
+  const V3 wk = TVF ? v3(a.R.V0[k]) - uk : V3{0.f, 0.f, 0.f};
+  const V3 nk = TVF ? v3(aa.lb[k]) : V3{0.f, 0.f, 0.f};
+  const float pbg = TVF ? eos.p_fn(0.0f) : 0.0f;
   float rhob = 0.0f;
-  V3 ub = {0.f, 0.f, 0.f}, rb = {0.f, 0.f, 0.f};
+  V3 ub = {0.f, 0.f, 0.f}, rb = {0.f, 0.f, 0.f}, vb = {0.f, 0.f, 0.f};
   for_each_pair(a, k, h, [&](const int j) {
     const float4 Pj = a.R.P[j];
     const V3 dr = min_image<DIM>(Pk, Pj, a.geo);
@@ -92,30 +117,54 @@ __global__ __launch_bounds__(kAB) void k_adj_forces(AdjArgs aa, float4* __restri
     const float rs = rhok + rhoj;
     const float pkj = (rhoj * pk + rhok * pj) / rs;
     const float etakj = 2.0f * etak * etaj / (etak + etaj + kEps);
-    const V3 T = (-pkj) * dr + etakj * (uk - v3(Uj));
+    const V3 uj = v3(Uj);
+    V3 T = (-pkj) * dr + etakj * (uk - uj);
+    V3 wj = {0.f, 0.f, 0.f};
+    if (TVF) {
+      wj = v3(a.R.V0[j]) - uj;
+      T = T + (0.5f * rhok * dot<DIM>(wk, dr)) * uk + (0.5f * rhoj * dot<DIM>(wj, dr)) * uj;
+    }
     const float Dp = rhoj * (dPk * rs + (pj - pk)) / (rs * rs);  // d p_kj / d rho_k (via p_k)
     const V3 lj = v3(Lj4);
-    const V3 lc = ckj * lk - cjk * lj;
+    const V3 lc = ckj * lk - cjk * lj;   // (V_k^2 + V_j^2) W'/(d + eps) l_kj
     const V3 lA = Akj * lk - Ajk * lj;
     const V3 lAd = Adkj * lk - Adjk * lj;
     rhob += (-2.0f * Vk2 / rhok) * dot<DIM>(lA, T) - Dp * dot<DIM>(lc, dr);
     ub = ub + etakj * lc;
     rb = rb + Vs * ((dot<DIM>(lAd, T) / d) * dr - pkj * lA);
+    if (TVF) {
+      const V3 nj = v3(aa.lb[j]);
+      const V3 nA = Akj * nk - Ajk * nj;     // W'/(d + eps) n_kj
+      const V3 nAd = Adkj * nk - Adjk * nj;
+      const float uk_lc = dot<DIM>(uk, lc), wk_r = dot<DIM>(wk, dr);
+      // G: the volume factor and the pair geometry
+      rhob += (-2.0f * Vk2 / rhok) * pbg * dot<DIM>(nA, dr);
+      rb = rb + (Vs * pbg) * ((dot<DIM>(nAd, dr) / d) * dr + nA);
+      // the stress A_k = rho_k u_k w_k^T of T: through rho_k, u_k, v_k and r
+      rhob += 0.5f * wk_r * uk_lc;
+      ub = ub + (0.5f * rhok) * (wk_r * lc - uk_lc * dr);
+      vb = vb + (0.5f * rhok * uk_lc) * dr;
+      rb = rb + (0.5f * rhok * uk_lc) * wk + (0.5f * rhoj * dot<DIM>(uj, lc)) * wj;
+    }
   });
   rhob = lane_sum<TPP>(rhob);
   ub = lane_sum<TPP>(ub);
   rb = lane_sum<TPP>(rb);
+  if (TVF) vb = lane_sum<TPP>(vb);
   if (h != 0) return;
   const bool fluid = (int)Qk.z == kFluid;
   ab0[k] = f4(ub, fluid ? mk * rhob : 0.0f);  // rho = m sum W only for fluid (solver.py:796)
   ab1[k] = f4(rb, 0.0f);
+  if (TVF) ab2[k] = f4(vb, 0.0f);
 }
 
 // Sweep B: the r' cotangent through the density sums, then the kinematic
-// transpose of the kick/drift, written in original order.
-template <int DIM, int KIND>
+// transpose of the kick, transport velocity and drift, written in original
+// order.
+template <int DIM, int KIND, bool TVF>
 __global__ __launch_bounds__(kAB) void k_adj_density(AdjArgs aa, const float4* __restrict__ ab0,
-                                                     const float4* __restrict__ ab1) {
+                                                     const float4* __restrict__ ab1,
+                                                     const float4* __restrict__ ab2) {
   constexpr int TPP = tpp_of(DIM);
   const SweepArgs& a = aa.s;
   const int tid = logical_block(a.remap) * blockDim.x + threadIdx.x;
@@ -141,26 +190,35 @@ __global__ __launch_bounds__(kAB) void k_adj_density(AdjArgs aa, const float4* _
   const int i = a.perm[k];
   const float4 A0 = ab0[k], A1 = ab1[k];
   const V3 rbt = load_vec<DIM>(aa.rb_in, i) + v3(A1) + rb;
-  V3 ubt = load_vec<DIM>(aa.ub_in, i) + v3(A0);
-  ubt = ubt + aa.dt * rbt;  // r' = r + dt u'
+  V3 vbt = aa.dt * rbt;  // r' = r + dt v'
+  if (aa.vb_in) vbt = vbt + load_vec<DIM>(aa.vb_in, i);
+  if (TVF) vbt = vbt + v3(ab2[k]);
+  const V3 ubt = load_vec<DIM>(aa.ub_in, i) + v3(A0) + vbt;  // v' = u' + c b
   store_vec<DIM>(aa.rb_out, i, rbt);
   store_vec<DIM>(aa.ub_out, i, ubt);
   store_vec<DIM>(aa.ab_out, i, aa.dt * ubt);  // u' = u + dt a
+  if (aa.bb_out) store_vec<DIM>(aa.bb_out, i, aa.c_tvf * vbt);
 }
 
-template <int DIM, int KIND>
+template <int DIM, int KIND, bool TVF>
 int launch_adjoint(sph_ws* ws, AdjArgs& a, bool records_ready, hipStream_t st) {
   const int n = a.s.n;
   dim3 grid(xcd_grid(cdivl((int64_t)n * tpp_of(DIM), kAB), a.s.remap));
   if (!records_ready)
     SPH_LAUNCH(ws, "density", st, (k_density<DIM, KIND, 0><<<grid, kAB, 0, st>>>(a.s)));
   SPH_LAUNCH(ws, "adj_gather", st,
-             k_adj_gather<DIM><<<(unsigned)cdivl(n, kAB), kAB, 0, st>>>(a));
+             (k_adj_gather<DIM, TVF><<<(unsigned)cdivl(n, kAB), kAB, 0, st>>>(a)));
   SPH_LAUNCH(ws, "adj_forces", st,
-             (k_adj_forces<DIM, KIND><<<grid, kAB, 0, st>>>(a, ws->AB0, ws->AB1)));
+             (k_adj_forces<DIM, KIND, TVF><<<grid, kAB, 0, st>>>(a, ws->AB0, ws->AB1, ws->AB2)));
   SPH_LAUNCH(ws, "adj_density", st,
-             (k_adj_density<DIM, KIND><<<grid, kAB, 0, st>>>(a, ws->AB0, ws->AB1)));
+             (k_adj_density<DIM, KIND, TVF><<<grid, kAB, 0, st>>>(a, ws->AB0, ws->AB1, ws->AB2)));
   return SPH_OK;
+}
+
+template <int DIM, int KIND>
+int launch_adjoint_tvf(sph_ws* ws, AdjArgs& a, bool tvf, bool records_ready, hipStream_t st) {
+  return tvf ? launch_adjoint<DIM, KIND, true>(ws, a, records_ready, st)
+             : launch_adjoint<DIM, KIND, false>(ws, a, records_ready, st);
 }
 
 }  // namespace sph
@@ -168,8 +226,9 @@ int launch_adjoint(sph_ws* ws, AdjArgs& a, bool records_ready, hipStream_t st) {
 using namespace sph;
 
 extern "C" int sph_step_vjp(sph_ws* ws, int bank, const sph_solver_params* prm, float dt,
-                            const sph_state_in* next, const sph_cotangent_in* bar_next,
-                            sph_cotangent_out* bar, int32_t records_ready, hipStream_t st) {
+                            float tvf_coef, const sph_state_in* next,
+                            const sph_cotangent_in* bar_next, sph_cotangent_out* bar,
+                            int32_t records_ready, hipStream_t st) {
   if (!prm || !next || !bar_next || !bar) { set_error("sph_step_vjp: null argument"); return SPH_ERR_ARG; }
   int rc = check_bank(ws, bank, prm, "sph_step_vjp");
   if (rc) return rc;
@@ -178,29 +237,41 @@ extern "C" int sph_step_vjp(sph_ws* ws, int bank, const sph_solver_params* prm, 
     return SPH_ERR_ARG;
   }
   if (!bar_next->r || !bar_next->u || !bar_next->a || !bar->r || !bar->u || !bar->a) {
-    set_error("sph_step_vjp: every cotangent array is required");
+    set_error("sph_step_vjp: the r, u and a cotangent arrays are required");
     return SPH_ERR_ARG;
   }
   if (prm->solver != 0 || prm->density_evolution || prm->bc_trick || prm->heat_conduction ||
-      prm->artificial_alpha != 0.0f || prm->tvf_stress) {
+      prm->artificial_alpha != 0.0f) {
     set_error("sph_step_vjp: only the standard SPH solver with density summation, no walls, "
-              "no heat, no artificial viscosity and tvf = 0 is differentiated");
+              "no heat and no artificial viscosity is differentiated");
     return SPH_ERR_ARG;
   }
   if (prm->kernel != SPH_KERNEL_QSK && prm->kernel != SPH_KERNEL_WC2K) {
     set_error("sph_step_vjp: kernel must be QSK or WC2K");
     return SPH_ERR_ARG;
   }
+  const bool tvf = prm->tvf_stress != 0;
+  if (tvf && !next->v) {
+    set_error("sph_step_vjp: the transport-velocity step needs v of state n+1");
+    return SPH_ERR_ARG;
+  }
+  if (!tvf && tvf_coef != 0.0f) {
+    set_error("sph_step_vjp: tvf_coef != 0 needs tvf_stress (the transport-velocity step)");
+    return SPH_ERR_ARG;
+  }
   sph_state_in in = *next;
-  in.v = next->u;   // tvf = 0: v == u
+  if (!tvf) in.v = next->u;   // tvf = 0: v == u
   in.g_ext = nullptr;
   sph_state_out none{};
   AdjArgs a;
   a.s = make_args(ws, bank, prm, &in, &none);
   a.rb_in = bar_next->r; a.ub_in = bar_next->u; a.ab_in = bar_next->a;
-  a.rb_out = bar->r; a.ub_out = bar->u; a.ab_out = bar->a;
+  a.vb_in = bar_next->v; a.bb_in = bar_next->b;
+  a.rb_out = bar->r; a.ub_out = bar->u; a.ab_out = bar->a; a.bb_out = bar->b;
   a.la = ws->LA;
+  a.lb = ws->LB;
   a.dt = dt;
+  a.c_tvf = tvf_coef;
   const int n = a.s.n;
   const bool ready = records_ready != 0;
   dim3 g1((unsigned)cdivl(n, kAB));
@@ -215,8 +286,10 @@ extern "C" int sph_step_vjp(sph_ws* ws, int bank, const sph_solver_params* prm, 
                                                        nullptr));
   }
   if (prm->dim == 2)
-    return prm->kernel == SPH_KERNEL_QSK ? launch_adjoint<2, SPH_KERNEL_QSK>(ws, a, ready, st)
-                                         : launch_adjoint<2, SPH_KERNEL_WC2K>(ws, a, ready, st);
-  return prm->kernel == SPH_KERNEL_QSK ? launch_adjoint<3, SPH_KERNEL_QSK>(ws, a, ready, st)
-                                       : launch_adjoint<3, SPH_KERNEL_WC2K>(ws, a, ready, st);
+    return prm->kernel == SPH_KERNEL_QSK
+               ? launch_adjoint_tvf<2, SPH_KERNEL_QSK>(ws, a, tvf, ready, st)
+               : launch_adjoint_tvf<2, SPH_KERNEL_WC2K>(ws, a, tvf, ready, st);
+  return prm->kernel == SPH_KERNEL_QSK
+             ? launch_adjoint_tvf<3, SPH_KERNEL_QSK>(ws, a, tvf, ready, st)
+             : launch_adjoint_tvf<3, SPH_KERNEL_WC2K>(ws, a, tvf, ready, st);
 }

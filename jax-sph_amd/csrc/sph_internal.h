@@ -102,6 +102,7 @@ struct sph_ws {
   float* drho = nullptr; // sorted drhodt
   // adjoint sweeps: sorted cotangent of dudt, then the per-particle partials
   float4 *LA = nullptr, *AB0 = nullptr, *AB1 = nullptr;
+  float4 *LB = nullptr, *AB2 = nullptr;  // transport-velocity step: dvdt' and v' cotangents
   // profiling
   bool prof_on = false;
   std::vector<sph::ProfEntry> prof_pending;

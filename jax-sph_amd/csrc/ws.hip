@@ -136,6 +136,7 @@ extern "C" int sph_ws_create(const sph_nl_config* cfg, sph_ws** out) {
   A(ws->NW, N); A(ws->UT, N);
   A(ws->drho, N);
   A(ws->LA, N); A(ws->AB0, N); A(ws->AB1, N);
+  A(ws->LB, N); A(ws->AB2, N);
   A(ws->U3, N); A(ws->Q3, N);
   A(ws->W1, N); A(ws->W2, N); A(ws->W3, N);
   A(ws->eta_varies, 1);
@@ -234,7 +235,7 @@ extern "C" int sph_ws_destroy(sph_ws* ws) {
   hipDeviceSynchronize();
   void* ptrs[] = {ws->cell_count, ws->pcell, ws->pslot, ws->unsorted, ws->off64, ws->scan_tmp,
                   ws->cnt64, ws->part_build, ws->part_sweep, ws->P, ws->U0, ws->U1, ws->U2, ws->V0, ws->V2, ws->Q0, ws->Q1,
-                  ws->Q2, ws->NW, ws->UT, ws->drho, ws->LA, ws->AB0, ws->AB1,
+                  ws->Q2, ws->NW, ws->UT, ws->drho, ws->LA, ws->AB0, ws->AB1, ws->LB, ws->AB2,
                   ws->U3, ws->Q3, ws->DL0, ws->DL1, ws->DL2, ws->DG0, ws->DG1,
                   ws->W1, ws->W2, ws->W3, ws->eta_varies};
   for (void* p : ptrs)

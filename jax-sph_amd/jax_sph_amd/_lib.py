@@ -68,11 +68,12 @@ class BCSpec(Structure):
 
 
 class CotangentIn(Structure):
-    _fields_ = [("r", c_void_p), ("u", c_void_p), ("a", c_void_p)]
+    _fields_ = [("r", c_void_p), ("u", c_void_p), ("a", c_void_p), ("v", c_void_p),
+                ("b", c_void_p)]
 
 
 class CotangentOut(Structure):
-    _fields_ = [("r", c_void_p), ("u", c_void_p), ("a", c_void_p)]
+    _fields_ = [("r", c_void_p), ("u", c_void_p), ("a", c_void_p), ("b", c_void_p)]
 
 
 class EnsembleMember(Structure):
@@ -127,8 +128,9 @@ def lib():
                              c_int64, vp, vp]),
         "sph_integrate": (c_int, [c_int32, c_int64, vp, vp, vp, vp, vp, vp, vp, c_float, c_float,
                                   POINTER(c_float), vp]),
-        "sph_step_vjp": (c_int, [vp, c_int, POINTER(SolverParams), c_float, POINTER(StateIn),
-                                 POINTER(CotangentIn), POINTER(CotangentOut), c_int32, vp]),
+        "sph_step_vjp": (c_int, [vp, c_int, POINTER(SolverParams), c_float, c_float,
+                                 POINTER(StateIn), POINTER(CotangentIn), POINTER(CotangentOut),
+                                 c_int32, vp]),
         "sph_ensemble_create": (c_int, [POINTER(EnsembleMember), c_int32, POINTER(c_void_p)]),
         "sph_ensemble_step": (c_int, [vp, vp]),
         "sph_ensemble_tune": (c_int, [vp, c_char_p, c_int64]),

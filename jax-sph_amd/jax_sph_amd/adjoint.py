@@ -17,8 +17,10 @@ Memory is O(n_steps / k + k) states instead of O(n_steps).
 
 Differentiated configuration (other settings raise ``NotImplementedError``):
 standard SPH (``solver="SPH"``), density summation, no walls, no heat
-conduction, no artificial viscosity, ``tvf = 0``, Quintic or Wendland-C2
-kernel, and a constant body force.  Gradients flow to the initial velocity
+conduction, no artificial viscosity, Quintic or Wendland-C2 kernel, a
+constant body force, with or without the transport-velocity formulation
+(any ``tvf``: the stress ``A = rho u (v - u)`` and the background-pressure
+acceleration, solver.py:199-256).  Gradients flow to the initial velocity
 and positions.
 """
 from __future__ import annotations
@@ -49,8 +51,6 @@ def _check(solver: WCSPH, tvf: float):
         why.append("heat conduction is not differentiated")
     if solver.artificial_alpha != 0.0:
         why.append("artificial viscosity is not differentiated")
-    if tvf != 0.0:
-        why.append("tvf must be 0")
     if solver._kernel_fn.name not in ("QSK", "WC2K"):
         why.append("kernel must be QSK or WC2K")
     if solver.g_const is None:
@@ -59,20 +59,39 @@ def _check(solver: WCSPH, tvf: float):
         raise NotImplementedError("differentiable rollout: " + "; ".join(why))
 
 
+def _tvf_coef(tvf: float, dt: float) -> np.float32:
+    """c of v' = u' + c dvdt, rounded as the fused step rounds it."""
+    return _f(_f(tvf * 0.5) * _f(dt))
+
+
+def _cot_in(rb, ub, ab, bb=None):
+    """sph_cotangent_in of (r, u, a[, b]); the cotangent of v' is zero (the
+    next step recomputes v from u and dvdt)."""
+    return _lib.CotangentIn(rb.data_ptr(), ub.data_ptr(), ab.data_ptr(), None,
+                            bb.data_ptr() if bb is not None else None)
+
+
+def _cot_out(rb, ub, ab, bb=None):
+    return _lib.CotangentOut(rb.data_ptr(), ub.data_ptr(), ab.data_ptr(),
+                             bb.data_ptr() if bb is not None else None)
+
+
 class _Runner:
     """Static part of a differentiable rollout: solver, exact neighbour search,
     the fused step, and the step transpose."""
 
     def __init__(self, solver: WCSPH, neighbor_fn, shift_fn, dt: float, n_steps: int,
-                 checkpoint_every: int, base_state: Dict[str, torch.Tensor]):
+                 checkpoint_every: int, base_state: Dict[str, torch.Tensor], tvf: float = 0.0):
         self.solver = solver
         self.neighbor_fn = neighbor_fn
         self.dt = float(_f(dt))
         self.n_steps = int(n_steps)
         self.k = max(1, int(checkpoint_every))
         self.base = solver.state_in(base_state)
-        self.advance = si_euler(0.0, solver.forward_wrapper(), shift_fn, None, None)
-        self.prm = solver.params(tvf_stress=False)
+        self.tvf = float(tvf)
+        self.advance = si_euler(self.tvf, solver.forward_wrapper(), shift_fn, None, None)
+        self.prm = solver.params(tvf_stress=self.tvf != 0.0)
+        self.ctvf = float(_tvf_coef(self.tvf, self.dt))
         self.nb: Optional[NeighborList] = None
         self.reallocations = 0
 
@@ -115,7 +134,9 @@ class _Runner:
 
     def backward(self, ckpts, rb, ub):
         ab = torch.zeros_like(ub)
-        bufs = [torch.empty_like(ub) for _ in range(3)]
+        tvf = self.tvf != 0.0
+        bb = torch.zeros_like(ub) if tvf else None  # cotangent of dvdt
+        bufs = [torch.empty_like(ub) for _ in range(4 if tvf else 3)]
         L = _lib.lib()
         for seg in reversed(range(len(ckpts))):
             start = seg * self.k
@@ -129,15 +150,17 @@ class _Runner:
                 nbn = self._list_at(nxt["r"])
                 ws = nbn.workspace
                 cin = self.solver.c_state_in(nxt)
-                bar_in = _lib.CotangentIn(rb.data_ptr(), ub.data_ptr(), ab.data_ptr())
-                rb2, ub2, ab2 = bufs
-                bar_out = _lib.CotangentOut(rb2.data_ptr(), ub2.data_ptr(), ab2.data_ptr())
+                bar_in = _cot_in(rb, ub, ab, bb)
+                bar_out = _cot_out(*bufs)
                 _lib.check(L.sph_step_vjp(ws.handle, nbn.bank, ctypes.byref(self.prm), self.dt,
-                                          ctypes.byref(cin), ctypes.byref(bar_in),
+                                          self.ctvf, ctypes.byref(cin), ctypes.byref(bar_in),
                                           ctypes.byref(bar_out), 0, _lib.stream()),
                            "sph_step_vjp")
-                bufs = [rb, ub, ab]
-                rb, ub, ab = rb2, ub2, ab2
+                old = [rb, ub, ab] + ([bb] if tvf else [])
+                rb, ub, ab = bufs[:3]
+                if tvf:
+                    bb = bufs[3]
+                bufs = old
         return rb, ub
 
 
@@ -161,9 +184,11 @@ class _Rollout(torch.autograd.Function):
 
 EVOLVING = ("r", "u", "v", "dudt", "dvdt", "rho", "p", "drhodt")
 # what one step of the differentiated configuration reads from the state it
-# starts from: positions, velocity and acceleration (v = u + 0 * dvdt; the
-# fluid density and pressure are recomputed by the summation and the EOS)
-CARRIED = ("r", "u", "dudt")
+# starts from: positions, velocity and the accelerations (v = u + c dvdt is
+# recomputed; so are the fluid density and pressure, by the summation and the
+# EOS).  Without tvf, c = 0 and dvdt is not read.
+def _carried(tvf: float) -> Tuple[str, ...]:
+    return ("r", "u", "dudt", "dvdt") if tvf != 0.0 else ("r", "u", "dudt")
 
 
 class GraphRollout:
@@ -199,8 +224,11 @@ class GraphRollout:
         self.n_steps = int(n_steps)
         self.solver = solver
         self.dt = float(_f(dt))
-        self.prm = solver.params(tvf_stress=False)
-        self.advance = si_euler(0.0, solver.forward_wrapper(), shift_fn, None, None)
+        self.tvf = float(tvf)
+        self.ctvf = float(_tvf_coef(self.tvf, self.dt))
+        self.prm = solver.params(tvf_stress=self.tvf != 0.0)
+        self.advance = si_euler(self.tvf, solver.forward_wrapper(), shift_fn, None, None)
+        carried = _carried(self.tvf)
         base = solver.state_in(state)
         n_seg = self.n_steps // self.k
         if workspace_sets is None:   # concurrency pays at small sizes; memory rules large ones
@@ -209,20 +237,21 @@ class GraphRollout:
         self.sets = [[neighbor_fn.allocate(base["r"]) for _ in range(self.k)]
                      for _ in range(self.n_sets)]
         # carried fields live in one (3, N, d) buffer: a checkpoint is one copy
-        self.base_buf = torch.stack([base[key] for key in CARRIED]).contiguous()
+        self.base_buf = torch.stack([base[key] for key in carried]).contiguous()
         self.S_buf = self.base_buf.clone()
         self.S = {key: v.clone() for key, v in base.items()}   # forward segment state
-        for j, key in enumerate(CARRIED):
+        for j, key in enumerate(carried):
             self.S[key] = self.S_buf[j]
         self.C_bufs, self.C = [], []
         for _ in range(self.n_sets):
             cb = self.base_buf.clone()
             c = {key: v.clone() for key, v in base.items()}
-            for j, key in enumerate(CARRIED):
+            for j, key in enumerate(carried):
                 c[key] = cb[j]
             self.C_bufs.append(cb)
             self.C.append(c)
-        self.bar = [torch.zeros_like(base["u"]) for _ in range(3)]  # r, u, a cotangents
+        # r, u, a (and dvdt) cotangents
+        self.bar = [torch.zeros_like(base["u"]) for _ in range(len(carried))]
         self.bits = torch.zeros((), dtype=torch.int64, device=base["u"].device)
         self.sides = [torch.cuda.Stream() for _ in range(max(1, int(side_streams)))]
         self.generation = 0
@@ -253,19 +282,19 @@ class GraphRollout:
                 self.bits.copy_(torch.bitwise_or(self.bits, bits))
             gV = torch.cuda.CUDAGraph()
             with torch.cuda.graph(gV):
-                rb, ub, ab = self.bar
+                cur = list(self.bar)
                 for st_j, nb_j in reversed(steps):
-                    out = [torch.empty_like(rb) for _ in range(3)]
-                    bar_in = _lib.CotangentIn(rb.data_ptr(), ub.data_ptr(), ab.data_ptr())
-                    bar_out = _lib.CotangentOut(*(t.data_ptr() for t in out))
+                    out = [torch.empty_like(cur[0]) for _ in range(len(cur))]
+                    bar_in = _cot_in(*cur)
+                    bar_out = _cot_out(*out)
                     _lib.check(L.sph_step_vjp(nb_j.workspace.handle, nb_j.bank,
-                                              ctypes.byref(self.prm), self.dt,
+                                              ctypes.byref(self.prm), self.dt, self.ctvf,
                                               ctypes.byref(self.solver.c_state_in(st_j)),
                                               ctypes.byref(bar_in), ctypes.byref(bar_out),
                                               1,  # records left by this step's own R pass
                                               _lib.stream()), "sph_step_vjp")
-                    rb, ub, ab = out
-                for dst, src in zip(self.bar, (rb, ub, ab)):
+                    cur = out
+                for dst, src in zip(self.bar, cur):
                     dst.copy_(src)
             self.gR.append(gR)
             self.gV.append(gV)
@@ -303,7 +332,8 @@ class GraphRollout:
         main = torch.cuda.current_stream()
         self.bar[0].copy_(rb_final)
         self.bar[1].copy_(ub_final)
-        self.bar[2].zero_()
+        for b in self.bar[2:]:
+            b.zero_()
         n = len(ckpts)
         ev_r = [torch.cuda.Event() for _ in range(n)]
         ev_v = [torch.cuda.Event() for _ in range(n)]
@@ -369,6 +399,6 @@ def rollout(state: Dict[str, torch.Tensor], solver: WCSPH, neighbor_fn, shift_fn
     _check(solver, tvf)
     if getattr(getattr(neighbor_fn, "search", None), "skin", 0.0):
         raise ValueError("the differentiable rollout needs a neighbor_list with skin=0")
-    runner = _Runner(solver, neighbor_fn, shift_fn, dt, n_steps, checkpoint_every, state)
+    runner = _Runner(solver, neighbor_fn, shift_fn, dt, n_steps, checkpoint_every, state, tvf)
     u, r = _Rollout.apply(state["u"], state["r"], runner)
     return u, r, runner

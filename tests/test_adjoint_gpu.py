@@ -31,9 +31,9 @@ def _setup(kernel="QSK", dx=0.05, seed=0):
     return cfg, box, st, solver, fn, shift, gen
 
 
-def _oracle_rollout(st_np, solver, box, dt, n_steps):
+def _oracle_rollout(st_np, solver, box, dt, n_steps, tvf=0.0):
     ns = nlo.NeighborSearch(box, solver._kernel_fn.cutoff, mask_self=False, dtype=np.float64)
-    adv = so.si_euler(0.0, G.oracle_solver64(solver, box).forward, box, lambda s: s,
+    adv = so.si_euler(tvf, G.oracle_solver64(solver, box).forward, box, lambda s: s,
                       ns.update, np.float64)
     st = G.to_f64(st_np)
     nb = ns.allocate(st["r"])
@@ -81,6 +81,98 @@ def test_rollout_gradient_matches_oracle_fd(kernel, n_steps, k):
         assert abs(ad - fd) <= 1e-3 * abs(fd) + 1e-6, (trial, ad, fd)
 
 
+def _setup_tvf(dim, kernel, n_side, seed=0):
+    """Taylor-Green vortex with the transport-velocity formulation (tvf = 1):
+    2D on the unit square or 3D on the (2 pi)^3 cube, with jittered
+    positions, a perturbed velocity and a nonzero initial dvdt (so the first
+    step's v = u + c dvdt differs from u)."""
+    dx = (2 * np.pi if dim == 3 else 1.0) / n_side
+    ov = {"case": {"dim": dim, "dx": dx}, "kernel": {"name": kernel}, "solver": {"tvf": 1.0}}
+    cfg, box, state, g, bc, eos, disp, shift, solver = G.setup("tgv", ov)
+    gen = torch.Generator().manual_seed(seed)
+    n = state["r"].shape[0]
+    st = dict(state)
+    st["u"] = (state["u"].cpu() + 0.05 * torch.randn(n, dim, generator=gen)).cuda()
+    st["r"] = shift(state["r"], (0.1 * dx * torch.randn(n, dim, generator=gen)).cuda())
+    st["dvdt"] = (10.0 * torch.randn(n, dim, generator=gen)).cuda()
+    fn = partition.neighbor_list(disp, box, r_cutoff=solver._kernel_fn.cutoff,
+                                 capacity_multiplier=1.25, mask_self=False,
+                                 format=partition.Sparse, skin=0.0)
+    return cfg, box, st, solver, fn, shift, gen
+
+
+@pytest.mark.parametrize("dim,kernel,n_side,n_steps,k", [
+    (2, "QSK", 20, 1, 1), (2, "QSK", 20, 5, 2), (2, "WC2K", 20, 4, 4),
+    (3, "QSK", 10, 1, 1), (3, "QSK", 10, 4, 2), (3, "WC2K", 12, 3, 3),
+])
+def test_tvf_rollout_gradient_matches_oracle_fd(dim, kernel, n_side, n_steps, k):
+    """The transport-velocity step (tvf = 1: the A = rho u (v - u) stress in
+    dudt and the background-pressure dvdt, solver.py:199-256, carried through
+    v = u + tvf dt/2 dvdt, integrator.py:26-30): directional derivatives of a
+    linear functional of the final (u, r) with respect to the initial u and r
+    against central differences of the float64 oracle rollout, in 2D and 3D."""
+    cfg, box, st, solver, fn, shift, gen = _setup_tvf(dim, kernel, n_side)
+    dt = cfg.solver.dt
+    n = st["r"].shape[0]
+    w_u = torch.randn(n, dim, generator=gen).cuda()
+    w_r = torch.randn(n, dim, generator=gen).cuda()
+    u0 = st["u"].clone().requires_grad_(True)
+    r0 = st["r"].clone().requires_grad_(True)
+    st_in = dict(st)
+    st_in["u"], st_in["r"] = u0, r0
+    u, r, runner = adjoint.rollout(st_in, solver, fn, shift, dt, n_steps, tvf=1.0,
+                                   checkpoint_every=k)
+    ((w_u * u).sum() + (w_r * r).sum()).backward()
+    gu, gr = u0.grad.double().cpu().numpy(), r0.grad.double().cpu().numpy()
+    assert np.isfinite(gu).all() and np.isfinite(gr).all()
+    base = G.to_np(st)
+    wu, wr = w_u.double().cpu().numpy(), w_r.double().cpu().numpy()
+
+    def L(eps, du, dr, tvf=1.0):
+        s = G.to_f64(dict(base))
+        s["u"] = s["u"] + eps * du
+        s["r"] = np.mod(s["r"] + eps * dr, box)
+        out = _oracle_rollout(s, solver, box, dt, n_steps, tvf=tvf)
+        return float((wu * out["u"]).sum() + (wr * out["r"]).sum())
+
+    rng = np.random.default_rng(1)
+    eps = 1e-6
+    for trial in range(3):
+        du = rng.standard_normal((n, dim))
+        dr = rng.standard_normal((n, dim)) * (0.0 if trial == 0 else 1.0)
+        fd = (L(eps, du, dr) - L(-eps, du, dr)) / (2 * eps)
+        ad = float((gu * du).sum() + (gr * dr).sum())
+        assert abs(ad - fd) <= 1e-3 * abs(fd) + 1e-6, (trial, ad, fd)
+        if n_steps >= 2 and trial == 2:
+            # the transport-velocity terms move this derivative by far more
+            # than the tolerance: a transpose without them would fail
+            fd0 = (L(eps, du, dr, 0.0) - L(-eps, du, dr, 0.0)) / (2 * eps)
+            assert abs(fd - fd0) > 0.02 * abs(fd), (fd, fd0)
+
+
+def test_tvf_graph_rollout_equals_eager_rollout():
+    """GraphRollout of the transport-velocity step (its checkpoints carry
+    dvdt) gives the eager rollout's values and gradients bit for bit."""
+    cfg, box, st, solver, fn, shift, gen = _setup_tvf(3, "QSK", 10)
+    n = st["r"].shape[0]
+    w = torch.randn(n, 3, generator=gen).cuda()
+    u0 = st["u"].clone().requires_grad_(True)
+    s = dict(st)
+    s["u"] = u0
+    u, r, _ = adjoint.rollout(s, solver, fn, shift, cfg.solver.dt, 6, tvf=1.0,
+                              checkpoint_every=3)
+    ((w * u).sum() + (w * r).sum()).backward()
+    gr_ = adjoint.GraphRollout(st, solver, fn, shift, cfg.solver.dt, 6, tvf=1.0,
+                               checkpoint_every=3)
+    for _ in range(2):
+        uu = st["u"].clone().requires_grad_(True)
+        ug, rg = gr_(uu, st["r"])
+        ((w * ug).sum() + (w * rg).sum()).backward()
+        torch.cuda.synchronize()
+        assert torch.equal(ug, u) and torch.equal(rg, r)
+        assert torch.equal(uu.grad, u0.grad)
+
+
 def test_checkpoint_interval_does_not_change_the_gradient():
     cfg, box, st, solver, fn, shift, gen = _setup()
     n = st["r"].shape[0]
@@ -98,8 +190,9 @@ def test_checkpoint_interval_does_not_change_the_gradient():
 
 def test_unsupported_configurations_raise():
     cfg, box, st, solver, fn, shift, gen = _setup()
+    rie = G.setup("tgv", {"case": {"dim": 2, "dx": 0.05}, "solver": {"name": "RIE"}})[-1]
     with pytest.raises(NotImplementedError):
-        adjoint.rollout(st, solver, fn, shift, cfg.solver.dt, 2, tvf=1.0)
+        adjoint.rollout(st, rie, fn, shift, cfg.solver.dt, 2)
     skin_fn = partition.neighbor_list(solver.displacement_fn, box,
                                       r_cutoff=solver._kernel_fn.cutoff, mask_self=False,
                                       format=partition.Sparse)
