@@ -113,6 +113,23 @@ def test_autotune_keeps_the_trajectory():
             assert torch.equal(ens.state(k)[f], ref.state(k)[f]), (k, f)
 
 
+def test_3d_members_equal_their_own_fused_steps():
+    """A 3D ensemble (two Taylor-Green cubes with the transport-velocity
+    formulation, 16^3 and 17^3 particles; 17^3 is not a multiple of the
+    256-particle integrate blocks): every member equals its own fused steps
+    bitwise."""
+    spec = [("tgv", {"case": {"dim": 3, "dx": 2 * np.pi / 16}, "solver": {"tvf": 1.0}}),
+            ("tgv", {"case": {"dim": 3, "dx": 2 * np.pi / 17}, "solver": {"tvf": 1.0}})]
+    ens = Ensemble.from_configs(_cfgs(spec))
+    steps = 6
+    ens.step(steps)
+    torch.cuda.synchronize()
+    assert not ens.did_buffer_overflow
+    for k, cfg in enumerate(_cfgs(spec)):
+        st, nb = _run_single(cfg, steps)
+        _assert_equal(ens, k, st, nb, "3d")
+
+
 def test_member_matches_oracle():
     """A member after 5 batched steps against the float32/float64 oracles and
     the oracle's exact pair set at its new positions."""
