@@ -9,8 +9,9 @@ periodic wrap, cell binning, neighbour search, density + EOS, forces.
 Prints one JSON line.  ``value`` = particle-steps/s over the timed steps (inputs
 resident in HBM).  ``roofline`` = the dominant kernel's algorithmic bytes per
 launch / its mean device duration (HIP events on the library's stream).
-``cpu_baseline`` = the numpy oracle (a CPU restatement of jax-sph) on a bounded
-sample of the same workload, on this host.
+``cpu_baseline`` = the OpenMP C restatement of the step (oracle/sph_cpu.c, a
+CPU restatement of jax-sph) on the same workload, on this host's threads, for
+about 15 s.
 """
 import argparse
 import json
@@ -59,7 +60,7 @@ def parse():
     ap.add_argument("--ensemble-steps", type=int, default=1000)
     ap.add_argument("--ensemble-only", action="store_true",
                     help="run only the ensemble leg and print its JSON")
-    ap.add_argument("--cpu-sample-side", type=int, default=20)
+    ap.add_argument("--cpu-sample-side", type=int, default=64)
     ap.add_argument("--skin", default="auto",
                     help="Verlet skin of the fused step's pair lists ('auto' = 0.1 cutoff, "
                          "0 = exact lists rebuilt every step)")
@@ -341,11 +342,26 @@ def ensemble_leg(steps=1000, members=64, seed=0, graph_steps=20):
     }
 
 
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
 def cpu_baseline(n_side, seconds=15.0):
-    """Oracle si_euler on a 3D TGV lattice of n_side^3 particles (same physics)."""
+    """The same 3D TGV workload (n_side^3 particles, tvf = 1, QSK, Re 1600,
+    CFL dt) stepped by the OpenMP C restatement of the jax-sph step
+    (oracle/sph_cpu.c, checked against the numpy oracle in
+    tests/test_cpu_port.py) on every host thread the run may use, for about
+    `seconds` of wall time."""
     sys.path.insert(0, ROOT)
-    from oracle import nl_oracle as nlo
+    from oracle import cpu_port
     from oracle import sph_oracle as so
+    threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
     dx = 2 * np.pi / n_side
     box = np.full(3, 2 * np.pi)
     i = np.arange(n_side)
@@ -358,30 +374,26 @@ def cpu_baseline(n_side, seconds=15.0):
     u[:, 0] = np.sin(x) * np.cos(y) * np.cos(z)
     u[:, 1] = -np.cos(x) * np.sin(y) * np.cos(z)
     f = np.float32
-    st = {"r": r, "tag": np.zeros(N, np.int64), "u": u, "v": u.copy(),
-          "dudt": np.zeros_like(r), "dvdt": np.zeros_like(r), "drhodt": np.zeros(N, f),
+    st = {"r": r, "u": u, "v": u.copy(), "dudt": np.zeros_like(r), "dvdt": np.zeros_like(r),
           "rho": np.ones(N, f), "p": np.zeros(N, f), "mass": np.full(N, f(dx ** 3), f),
-          "eta": np.full(N, f(1 / 1600), f), "dTdt": np.zeros(N, f), "T": np.ones(N, f),
-          "kappa": np.zeros(N, f), "Cp": np.zeros(N, f), "nw": np.zeros_like(r)}
+          "eta": np.full(N, f(1 / 1600), f)}
+    kernel = so.Kernel("QSK", dx, 3)
     eos = so.TaitEoS(100.0, 1.0, 0.0, 1.0)
     dt = 0.25 * dx / 11.0
-    solver = so.WCSPH(box, eos, lambda q: np.zeros_like(q), dx, 3, dt, 10.0)
-    ns = nlo.NeighborSearch(box, solver.kernel.cutoff, mask_self=False)
-    nb = ns.allocate(st["r"])
-    adv = so.si_euler(1.0, solver.forward, box, None, ns.update)
-    st, nb = adv(dt, st, nb)  # warm-up
+    step = cpu_port.CPUStep(box, kernel, eos, dt, 1.0, threads=threads)
+    step(st)  # warm-up
     steps, t0 = 0, time.perf_counter()
     while True:
-        st, nb = adv(dt, st, nb)
+        step(st)
         steps += 1
         if time.perf_counter() - t0 > seconds:
             break
     el = time.perf_counter() - t0
-    return {"value": N * steps / el, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "sample": f"numpy oracle (CPU restatement of jax-sph), 3D TGV {n_side}^3 = {N} "
-                      f"particles, {steps} si_euler steps in {el:.1f} s, single thread, "
-                      f"{platform.processor() or platform.machine()}, "
-                      f"nproc={os.cpu_count()}"}
+    return {"value": N * steps / el, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": f"OpenMP C restatement of the jax-sph step (oracle/sph_cpu.c; a CPU "
+                      f"restatement of jax-sph, not jax-sph), 3D TGV {n_side}^3 = {N} "
+                      f"particles, tvf=1, {steps} si_euler steps in {el:.1f} s on {threads} "
+                      f"threads, {_cpu_model()}, nproc={os.cpu_count()}"}
 
 
 def main():
