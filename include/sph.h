@@ -218,10 +218,13 @@ typedef struct {
 } sph_cotangent_out;
 /* Vector-Jacobian product of one si_euler step (integrator.py:22-56), the
  * transpose that jax.grad builds for the reference's step.  Standard SPH,
- * density summation, no walls/heat/artificial viscosity, QSK or WC2K; with
- * prm->tvf_stress the transport-velocity step (solver.py:199-256: the
- * A = rho u (v - u) stress and the background-pressure dvdt), whose
- * v' = u' + tvf_coef * dvdt (tvf_coef = tvf dt / 2, 0 without tvf).
+ * density summation, no wall boundary treatment (bc_trick)/heat/artificial
+ * viscosity, QSK or WC2K; with prm->tvf_stress the transport-velocity step
+ * (solver.py:199-256: the A = rho u (v - u) stress and the background-pressure
+ * dvdt), whose v' = u' + tvf_coef * dvdt (tvf_coef = tvf dt / 2, 0 without
+ * tvf).  `bc` (nullable) is the step's case boundary hook as in sph_step:
+ * the velocities and accelerations it sets are constants of the step, so the
+ * particles it holds carry only a position cotangent.
  * `next` is state n+1 (r, u, v with tvf, rho, p, mass, eta, tag) and `bank`
  * must hold the pairs of next->r; given the cotangents of state n+1 writes
  * those of state n (bar may alias nothing in bar_next).  records_ready != 0
@@ -229,8 +232,9 @@ typedef struct {
  * produced `next` into `bank` (its sorted records are reused instead of
  * gathered again). */
 int sph_step_vjp(sph_ws *ws, int bank, const sph_solver_params *prm, float dt, float tvf_coef,
-                 const sph_state_in *next, const sph_cotangent_in *bar_next,
-                 sph_cotangent_out *bar, int32_t records_ready, hipStream_t stream);
+                 const sph_bc_spec *bc, const sph_state_in *next,
+                 const sph_cotangent_in *bar_next, sph_cotangent_out *bar,
+                 int32_t records_ready, hipStream_t stream);
 
 /* si_euler kick/drift alone (integrator.py:26-30), for callers that compose
  * the step from separate update/forward calls: u' = u + dt dudt,

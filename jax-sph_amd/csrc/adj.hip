@@ -55,15 +55,29 @@ struct AdjArgs {
   float4* lb;                // sorted cotangent of b' (tvf)
   float dt;
   float c_tvf;               // v' = u' + c_tvf b
+  int bc_kind;               // case boundary hook of the step (sph_bc_spec::kind)
 };
+
+// Whether the case boundary hook (integrator.py:54; ldc.py:130-144, db.py:299-307,
+// case_setup.py:393-402) sets particle k's u, v, dudt and dvdt to constants:
+// their cotangents then flow nowhere.
+__device__ __forceinline__ bool bc_holds(int kind, int tag) {
+  if (kind == 1) return tag == kSolidWall || tag == kMovingWall;
+  if (kind == 2) return tag == kSolidWall;
+  if (kind == 3) return is_wall(tag);
+  return false;
+}
 
 template <int DIM, bool TVF>
 __global__ void k_adj_gather(AdjArgs a) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= a.s.n) return;
   const int i = a.s.perm[s];
-  a.la[s] = f4(load_vec<DIM>(a.ab_in, i), 0.0f);
-  if (TVF) a.lb[s] = f4(a.bb_in ? load_vec<DIM>(a.bb_in, i) : V3{0.f, 0.f, 0.f}, 0.0f);
+  // the hook replaces the accelerations of the particles it holds
+  const bool held = bc_holds(a.bc_kind, a.s.in.tag[i]);
+  const V3 zero = {0.f, 0.f, 0.f};
+  a.la[s] = f4(held ? zero : load_vec<DIM>(a.ab_in, i), 0.0f);
+  if (TVF) a.lb[s] = f4(held || !a.bb_in ? zero : load_vec<DIM>(a.bb_in, i), 0.0f);
 }
 
 // Sweep A: density cotangent (times m, fluid only), the u' and v' cotangents
@@ -193,7 +207,13 @@ __global__ __launch_bounds__(kAB) void k_adj_density(AdjArgs aa, const float4* _
   V3 vbt = aa.dt * rbt;  // r' = r + dt v'
   if (aa.vb_in) vbt = vbt + load_vec<DIM>(aa.vb_in, i);
   if (TVF) vbt = vbt + v3(ab2[k]);
-  const V3 ubt = load_vec<DIM>(aa.ub_in, i) + v3(A0) + vbt;  // v' = u' + c b
+  V3 ubt = load_vec<DIM>(aa.ub_in, i) + v3(A0) + vbt;  // v' = u' + c b
+  if (bc_holds(aa.bc_kind, a.in.tag[i])) {
+    // held by the case hook: u, v, dudt and dvdt of this particle are constants
+    // in every step, so only its position carries a cotangent
+    ubt = V3{0.f, 0.f, 0.f};
+    vbt = ubt;
+  }
   store_vec<DIM>(aa.rb_out, i, rbt);
   store_vec<DIM>(aa.ub_out, i, ubt);
   store_vec<DIM>(aa.ab_out, i, aa.dt * ubt);  // u' = u + dt a
@@ -226,7 +246,7 @@ int launch_adjoint_tvf(sph_ws* ws, AdjArgs& a, bool tvf, bool records_ready, hip
 using namespace sph;
 
 extern "C" int sph_step_vjp(sph_ws* ws, int bank, const sph_solver_params* prm, float dt,
-                            float tvf_coef, const sph_state_in* next,
+                            float tvf_coef, const sph_bc_spec* bc, const sph_state_in* next,
                             const sph_cotangent_in* bar_next, sph_cotangent_out* bar,
                             int32_t records_ready, hipStream_t st) {
   if (!prm || !next || !bar_next || !bar) { set_error("sph_step_vjp: null argument"); return SPH_ERR_ARG; }
@@ -272,6 +292,11 @@ extern "C" int sph_step_vjp(sph_ws* ws, int bank, const sph_solver_params* prm, 
   a.lb = ws->LB;
   a.dt = dt;
   a.c_tvf = tvf_coef;
+  a.bc_kind = bc ? bc->kind : 0;
+  if (a.bc_kind < 0 || a.bc_kind > 3) {
+    set_error("sph_step_vjp: unknown case boundary hook");
+    return SPH_ERR_ARG;
+  }
   const int n = a.s.n;
   const bool ready = records_ready != 0;
   dim3 g1((unsigned)cdivl(n, kAB));

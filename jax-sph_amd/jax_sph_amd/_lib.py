@@ -129,8 +129,8 @@ def lib():
         "sph_integrate": (c_int, [c_int32, c_int64, vp, vp, vp, vp, vp, vp, vp, c_float, c_float,
                                   POINTER(c_float), vp]),
         "sph_step_vjp": (c_int, [vp, c_int, POINTER(SolverParams), c_float, c_float,
-                                 POINTER(StateIn), POINTER(CotangentIn), POINTER(CotangentOut),
-                                 c_int32, vp]),
+                                 POINTER(BCSpec), POINTER(StateIn), POINTER(CotangentIn),
+                                 POINTER(CotangentOut), c_int32, vp]),
         "sph_ensemble_create": (c_int, [POINTER(EnsembleMember), c_int32, POINTER(c_void_p)]),
         "sph_ensemble_step": (c_int, [vp, vp]),
         "sph_ensemble_tune": (c_int, [vp, c_char_p, c_int64]),

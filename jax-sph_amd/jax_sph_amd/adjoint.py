@@ -16,11 +16,14 @@ positions.  :func:`rollout` wraps an ``n_steps`` rollout in a
 Memory is O(n_steps / k + k) states instead of O(n_steps).
 
 Differentiated configuration (other settings raise ``NotImplementedError``):
-standard SPH (``solver="SPH"``), density summation, no walls, no heat
-conduction, no artificial viscosity, Quintic or Wendland-C2 kernel, a
-constant body force, with or without the transport-velocity formulation
-(any ``tvf``: the stress ``A = rho u (v - u)`` and the background-pressure
-acceleration, solver.py:199-256).  Gradients flow to the initial velocity
+standard SPH (``solver="SPH"``), density summation, no wall boundary
+treatment (``is_bc_trick``), no heat conduction, no artificial viscosity,
+Quintic or Wendland-C2 kernel, a constant body force, with or without the
+transport-velocity formulation (any ``tvf``: the stress ``A = rho u (v - u)``
+and the background-pressure acceleration, solver.py:199-256).  Wall particles
+may be present, held by the case's boundary hook (``bc_fn``, a
+``case_setup.CaseBC``: the lid-driven cavity of the reference's gradient
+check, notebooks/iclr24_grads.ipynb).  Gradients flow to the initial velocity
 and positions.
 """
 from __future__ import annotations
@@ -32,15 +35,17 @@ import numpy as np
 import torch
 
 from . import _lib
-from .integrator import si_euler
+from .integrator import _bc_spec, si_euler
 from .jax_md.partition import NeighborList
 from .solver import WCSPH
 
 _f = np.float32
 
 
-def _check(solver: WCSPH, tvf: float):
+def _check(solver: WCSPH, tvf: float, bc_fn=None):
     why = []
+    if bc_fn is not None and _bc_spec(bc_fn) is None:
+        why.append("bc_fn must be a case boundary hook (case_setup.CaseBC)")
     if solver.solver != "SPH":
         why.append("solver must be 'SPH'")
     if solver.is_rho_evol:
@@ -81,7 +86,8 @@ class _Runner:
     the fused step, and the step transpose."""
 
     def __init__(self, solver: WCSPH, neighbor_fn, shift_fn, dt: float, n_steps: int,
-                 checkpoint_every: int, base_state: Dict[str, torch.Tensor], tvf: float = 0.0):
+                 checkpoint_every: int, base_state: Dict[str, torch.Tensor], tvf: float = 0.0,
+                 bc_fn=None):
         self.solver = solver
         self.neighbor_fn = neighbor_fn
         self.dt = float(_f(dt))
@@ -89,9 +95,10 @@ class _Runner:
         self.k = max(1, int(checkpoint_every))
         self.base = solver.state_in(base_state)
         self.tvf = float(tvf)
-        self.advance = si_euler(self.tvf, solver.forward_wrapper(), shift_fn, None, None)
+        self.advance = si_euler(self.tvf, solver.forward_wrapper(), shift_fn, bc_fn, None)
         self.prm = solver.params(tvf_stress=self.tvf != 0.0)
         self.ctvf = float(_tvf_coef(self.tvf, self.dt))
+        self.bc = _bc_spec(bc_fn)
         self.nb: Optional[NeighborList] = None
         self.reallocations = 0
 
@@ -153,9 +160,9 @@ class _Runner:
                 bar_in = _cot_in(rb, ub, ab, bb)
                 bar_out = _cot_out(*bufs)
                 _lib.check(L.sph_step_vjp(ws.handle, nbn.bank, ctypes.byref(self.prm), self.dt,
-                                          self.ctvf, ctypes.byref(cin), ctypes.byref(bar_in),
-                                          ctypes.byref(bar_out), 0, _lib.stream()),
-                           "sph_step_vjp")
+                                          self.ctvf, ctypes.byref(self.bc), ctypes.byref(cin),
+                                          ctypes.byref(bar_in), ctypes.byref(bar_out), 0,
+                                          _lib.stream()), "sph_step_vjp")
                 old = [rb, ub, ab] + ([bb] if tvf else [])
                 rb, ub, ab = bufs[:3]
                 if tvf:
@@ -214,8 +221,8 @@ class GraphRollout:
 
     def __init__(self, state: Dict[str, torch.Tensor], solver: WCSPH, neighbor_fn, shift_fn,
                  dt: float, n_steps: int, tvf: float = 0.0, checkpoint_every: int = 10,
-                 workspace_sets: Optional[int] = None, side_streams: int = 3):
-        _check(solver, tvf)
+                 workspace_sets: Optional[int] = None, side_streams: int = 3, bc_fn=None):
+        _check(solver, tvf, bc_fn)
         if getattr(getattr(neighbor_fn, "search", None), "skin", 0.0):
             raise ValueError("the differentiable rollout needs a neighbor_list with skin=0")
         self.k = int(checkpoint_every)
@@ -227,7 +234,8 @@ class GraphRollout:
         self.tvf = float(tvf)
         self.ctvf = float(_tvf_coef(self.tvf, self.dt))
         self.prm = solver.params(tvf_stress=self.tvf != 0.0)
-        self.advance = si_euler(self.tvf, solver.forward_wrapper(), shift_fn, None, None)
+        self.advance = si_euler(self.tvf, solver.forward_wrapper(), shift_fn, bc_fn, None)
+        self.bc = _bc_spec(bc_fn)
         carried = _carried(self.tvf)
         base = solver.state_in(state)
         n_seg = self.n_steps // self.k
@@ -289,6 +297,7 @@ class GraphRollout:
                     bar_out = _cot_out(*out)
                     _lib.check(L.sph_step_vjp(nb_j.workspace.handle, nb_j.bank,
                                               ctypes.byref(self.prm), self.dt, self.ctvf,
+                                              ctypes.byref(self.bc),
                                               ctypes.byref(self.solver.c_state_in(st_j)),
                                               ctypes.byref(bar_in), ctypes.byref(bar_out),
                                               1,  # records left by this step's own R pass
@@ -388,17 +397,19 @@ class _GraphFn(torch.autograd.Function):
 
 
 def rollout(state: Dict[str, torch.Tensor], solver: WCSPH, neighbor_fn, shift_fn, dt: float,
-            n_steps: int, tvf: float = 0.0, checkpoint_every: int = 10
+            n_steps: int, tvf: float = 0.0, checkpoint_every: int = 10, bc_fn=None
             ) -> Tuple[torch.Tensor, torch.Tensor, "_Runner"]:
     """``n_steps`` si_euler steps from ``state``; returns the final (u, r).
 
     ``state["u"]`` and ``state["r"]`` may require grad: the returned tensors
     are differentiable with respect to them through the hand-written adjoint.
     ``neighbor_fn`` must build exact lists (``skin=0``), so the backward pass
-    can recompute the forward trajectory exactly."""
-    _check(solver, tvf)
+    can recompute the forward trajectory exactly.  ``bc_fn`` is the case's
+    boundary hook (a ``case_setup.CaseBC``), applied after every step."""
+    _check(solver, tvf, bc_fn)
     if getattr(getattr(neighbor_fn, "search", None), "skin", 0.0):
         raise ValueError("the differentiable rollout needs a neighbor_list with skin=0")
-    runner = _Runner(solver, neighbor_fn, shift_fn, dt, n_steps, checkpoint_every, state, tvf)
+    runner = _Runner(solver, neighbor_fn, shift_fn, dt, n_steps, checkpoint_every, state, tvf,
+                     bc_fn)
     u, r = _Rollout.apply(state["u"], state["r"], runner)
     return u, r, runner

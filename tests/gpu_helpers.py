@@ -49,13 +49,14 @@ def oracle_bc(bc: "cs.CaseBC", dim):
             return st
         m1 = (tag == 1)[:, None] if bc.kind != 3 else np.isin(tag, (1, 2, 3))[:, None]
         m2 = (tag == 2)[:, None] if bc.kind == 1 else np.zeros_like(m1)
-        lid = bc.u_lid[:dim].astype(np.float32)
+        t = st["u"].dtype  # float32 or float64 oracle: keep its precision
+        lid = bc.u_lid[:dim].astype(np.float32).astype(t)
         for k in ("u", "v"):
-            st[k] = np.where(m2, lid, np.where(m1, np.float32(0), st[k])).astype(np.float32)
+            st[k] = np.where(m2, lid, np.where(m1, t.type(0), st[k])).astype(t)
         for k in ("dudt", "dvdt"):
-            st[k] = np.where(m1 | m2, np.float32(0), st[k]).astype(np.float32)
+            st[k] = np.where(m1 | m2, t.type(0), st[k]).astype(t)
         if bc.kind == 2:
-            st["p"] = np.where(m1[:, 0], np.float32(0), st["p"]).astype(np.float32)
+            st["p"] = np.where(m1[:, 0], t.type(0), st["p"]).astype(t)
         return st
     return f
 

@@ -13,7 +13,7 @@ from oracle import sph_oracle as so
 pytestmark = pytest.mark.gpu
 
 js = pytest.importorskip("jax_sph_amd")
-from jax_sph_amd import adjoint, partition  # noqa: E402
+from jax_sph_amd import adjoint, integrator, partition  # noqa: E402
 from tests import gpu_helpers as G  # noqa: E402
 
 
@@ -31,10 +31,11 @@ def _setup(kernel="QSK", dx=0.05, seed=0):
     return cfg, box, st, solver, fn, shift, gen
 
 
-def _oracle_rollout(st_np, solver, box, dt, n_steps, tvf=0.0):
+def _oracle_rollout(st_np, solver, box, dt, n_steps, tvf=0.0, bc_fn=None):
     ns = nlo.NeighborSearch(box, solver._kernel_fn.cutoff, mask_self=False, dtype=np.float64)
-    adv = so.si_euler(tvf, G.oracle_solver64(solver, box).forward, box, lambda s: s,
-                      ns.update, np.float64)
+    bcf = G.oracle_bc(bc_fn, solver.dim) if bc_fn is not None else (lambda s: s)
+    adv = so.si_euler(tvf, G.oracle_solver64(solver, box).forward, box, bcf, ns.update,
+                      np.float64)
     st = G.to_f64(st_np)
     nb = ns.allocate(st["r"])
     for _ in range(n_steps):
@@ -171,6 +172,65 @@ def test_tvf_graph_rollout_equals_eager_rollout():
         torch.cuda.synchronize()
         assert torch.equal(ug, u) and torch.equal(rg, r)
         assert torch.equal(uu.grad, u0.grad)
+
+
+@pytest.mark.parametrize("case", ["tgv", "ldc"])
+def test_reference_gradient_check_configuration(case):
+    """The reference's own gradient validation (notebooks/iclr24_grads.ipynb,
+    cell 5): dx = 1/20, viscosity 0.1, 10 warm-up steps, then the gradient of
+    0.5 sum(u^2) after 5 more steps with respect to the positions, against
+    finite differences.  Its WCSPH call passes is_bc_trick in the eta_limiter
+    slot, so the cavity runs without the wall boundary treatment: its walls
+    are plain particles held by the case hook (walls at rest, lid at u = 1).
+    Here the differences come from the float64 oracle (central, eps = 1e-6)
+    along random directions, to 1e-3 relative."""
+    ov = {"case": {"dx": 0.05, "viscosity": 0.1}, "solver": {"t_end": 1.0}}
+    if case == "tgv":
+        ov["case"]["dim"] = 2
+    else:
+        ov["solver"]["is_bc_trick"] = False
+    cfg, box, state, g, bc, eos, disp, shift, solver = G.setup(case, ov)
+    assert not solver.is_bc_trick
+    if case == "ldc":
+        assert bc.kind == 1 and (state["tag"] != 0).any()
+    dt = cfg.solver.dt
+    cut = solver._kernel_fn.cutoff
+    warm = partition.neighbor_list(disp, box, r_cutoff=cut, mask_self=False,
+                                   format=partition.Sparse)
+    nb = warm.allocate(state["r"])
+    adv = integrator.si_euler(0.0, solver.forward_wrapper(), shift, bc, None)
+    st = state
+    for _ in range(10):
+        st, nb = adv(dt, st, nb)
+    fn = partition.neighbor_list(disp, box, r_cutoff=cut, capacity_multiplier=1.25,
+                                 mask_self=False, format=partition.Sparse, skin=0.0)
+    r0 = st["r"].clone().requires_grad_(True)
+    s_in = dict(st)
+    s_in["r"] = r0
+    u, r, _ = adjoint.rollout(s_in, solver, fn, shift, dt, 5, checkpoint_every=5, bc_fn=bc)
+    (0.5 * (u.double() ** 2).sum()).backward()
+    gr = r0.grad.double().cpu().numpy()
+    assert np.isfinite(gr).all()
+    base = G.to_np(st)
+    n, dim = base["r"].shape
+
+    def L(eps, dr):
+        s = G.to_f64(dict(base))
+        s["r"] = np.mod(s["r"] + eps * dr, box)
+        out = _oracle_rollout(s, solver, box, dt, 5, bc_fn=bc)
+        return 0.5 * float((out["u"] ** 2).sum())
+
+    # along the gradient itself (no cancellation in the directional sum), and
+    # along random directions, whose sum over 2N terms cancels to about
+    # 1/sqrt(2N) of sum |g_i d_i|: there the float32 adjoint is held to that scale
+    rng = np.random.default_rng(2)
+    eps = 1e-6
+    dirs = [gr / np.linalg.norm(gr)] + [rng.standard_normal((n, dim)) for _ in range(3)]
+    for trial, dr in enumerate(dirs):
+        fd = (L(eps, dr) - L(-eps, dr)) / (2 * eps)
+        ad = float((gr * dr).sum())
+        scale = abs(fd) if trial == 0 else float(np.abs(gr * dr).sum())
+        assert abs(ad - fd) <= 1e-3 * scale, (case, trial, ad, fd, scale)
 
 
 def test_checkpoint_interval_does_not_change_the_gradient():
