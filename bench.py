@@ -279,9 +279,10 @@ def ensemble_configs(members=64, seed=0):
 
 
 def ensemble_leg(steps=1000, members=64, seed=0, graph_steps=20):
-    """64 uneven members in one launch per step phase (graph-replayed, layout
-    autotuned) against the same 64 runs one after the other, each on its own
-    fused step, eager and graph-replayed."""
+    """64 uneven members in one launch per step phase (graph-replayed; each
+    group's skin width and the block layout autotuned) against the same 64 runs
+    one after the other, each on its own fused step with the same skin, eager
+    and graph-replayed."""
     import torch
     from jax_sph_amd.ensemble import Ensemble, setup_member
     from jax_sph_amd.graph import StepGraph
@@ -292,8 +293,13 @@ def ensemble_leg(steps=1000, members=64, seed=0, graph_steps=20):
     t0 = time.perf_counter()
     ens = Ensemble.from_configs(cfgs)
     setup_s = time.perf_counter() - t0
+    del ens
+    # launch shapes: each group's Verlet-skin width, then the block layout
+    t0 = time.perf_counter()
+    ens, tune = Ensemble.tuned_from_configs(cfgs, steps=graph_steps, reps=2)
+    tune["seconds"] = time.perf_counter() - t0
+    factors = ens.skin_factors
     info = ens.info()
-    tune = ens.autotune(steps=graph_steps, reps=2)
     ens.capture(graph_steps)
     ens.replay(1)  # warm-up; the timed replays continue the run
     torch.cuda.synchronize()
@@ -305,8 +311,8 @@ def ensemble_leg(steps=1000, members=64, seed=0, graph_steps=20):
     del ens
     seq_eager = seq_graph = 0.0
     overflow_seq = 0
-    for cfg in ensemble_configs(members, seed)[0]:
-        adv, st, nb, dt = setup_member(cfg)
+    for cfg, f in zip(ensemble_configs(members, seed)[0], factors):
+        adv, st, nb, dt = setup_member(cfg, skin_factor=f)  # the member's own skin
         st, nb = adv(dt, st, nb)  # warm-up
         torch.cuda.synchronize()
         t0 = time.perf_counter()

@@ -17,6 +17,9 @@ the block layouts of the batched launches (member order, XCD-contiguous
 workgroups) on graph-replayed steps and keeps the fastest; it restores the
 state afterwards and makes every member rebuild its lists, so a run tuned
 right after construction follows the untuned trajectory bit for bit.
+:meth:`tuned_from_configs` also searches each group's Verlet-skin width (the
+skin cells' size, i.e. the cell tile of the pair sweeps); members keep
+equalling their own fused steps with the skin chosen for them.
 """
 from __future__ import annotations
 
@@ -39,13 +42,26 @@ _f = np.float32
 _WRITTEN = ("r", "u", "v", "dudt", "dvdt", "rho", "p", "drhodt")
 
 
-def setup_member(cfg: Cfg, skin="auto", capacity_multiplier: float = 1.25):
+def group_key(cfg: Cfg) -> Tuple:
+    """The solver settings that select a member's sweep variants (density
+    mode, walls, force solver, transport velocity, kernel, extras): members
+    with equal keys share the batched sweep launches of one group."""
+    s = cfg.solver
+    return (s.name, bool(s.density_evolution), bool(s.is_bc_trick), float(s.tvf) != 0.0,
+            cfg.kernel.name, float(s.artificial_alpha) != 0.0, bool(s.heat_conduction))
+
+
+def setup_member(cfg: Cfg, skin="auto", capacity_multiplier: float = 1.25,
+                 skin_factor: Optional[float] = None):
     """One simulation set up as ``simulate`` does it (simulate.py:35-99): case
     state, solver, ``jaxmd_vmap`` neighbour list, fused ``si_euler``.
-    Returns ``(advance, state, neighbors, dt)``."""
+    ``skin_factor`` (optional) sets the Verlet skin to that multiple of the
+    member's cutoff.  Returns ``(advance, state, neighbors, dt)``."""
     Case = load_case(cfg.case.source or cfg.config)
     (cfg, box, state, g_ext_fn, bc_fn, nw_fn, eos, key, disp, shift) = Case(cfg).initialize()
     solver = make_solver(cfg, disp, eos, g_ext_fn)
+    if skin_factor is not None:
+        skin = float(skin_factor) * float(solver._kernel_fn.cutoff)
     fn = _partition.neighbor_list(disp, box, r_cutoff=solver._kernel_fn.cutoff,
                                   backend="jaxmd_vmap", capacity_multiplier=capacity_multiplier,
                                   mask_self=False, format=Sparse, skin=skin)
@@ -142,11 +158,71 @@ class Ensemble:
     # ---------------------------------------------------------------- build
     @classmethod
     def from_configs(cls, cfgs: Sequence[Cfg], skin="auto", capacity_multiplier: float = 1.25,
-                     order: int = 1, xcd: bool = True) -> "Ensemble":
+                     order: int = 1, xcd: bool = True,
+                     skin_factors: Optional[Sequence[float]] = None) -> "Ensemble":
         """Sets up each config as ``simulate`` does (case, solver, neighbour
-        list, ``si_euler``) and builds the ensemble of them."""
-        members = [setup_member(cfg, skin, capacity_multiplier) for cfg in cfgs]
-        return cls(members, order=order, xcd=xcd)
+        list, ``si_euler``) and builds the ensemble of them.  ``skin_factors``
+        (optional, one per config) sets each member's Verlet skin to that
+        multiple of its cutoff."""
+        factors = list(skin_factors) if skin_factors is not None else [None] * len(cfgs)
+        members = [setup_member(cfg, skin, capacity_multiplier, f)
+                   for cfg, f in zip(cfgs, factors)]
+        ens = cls(members, order=order, xcd=xcd)
+        ens.skin_factors = factors
+        return ens
+
+    @classmethod
+    def tuned_from_configs(cls, cfgs: Sequence[Cfg], factors: Sequence[float] = (0.05, 0.1, 0.15),
+                           steps: int = 10, reps: int = 2,
+                           capacity_multiplier: float = 1.25) -> Tuple["Ensemble", Dict]:
+        """The ensemble of ``cfgs`` with its launch shapes autotuned: the
+        Verlet skin of each group's members (the width of its skin cells, the
+        "cell tile" of the pair sweeps: wider lists against rarer rebuilds),
+        then the block layout (:meth:`autotune`).  The skins are searched
+        jointly over ``factors`` (multiples of each member's cutoff), then
+        group by group with the other groups at their best.  Each candidate
+        is a freshly set-up ensemble timed on graph-replayed steps from the
+        initial state; the one returned starts from the initial state too."""
+        cfgs = list(cfgs)
+        keys = [group_key(c) for c in cfgs]
+        groups = sorted(set(keys), key=str)
+        trials = {}
+
+        def time_of(per_group):
+            tag = tuple(per_group[g] for g in groups)
+            if tag not in trials:
+                ens = cls.from_configs(cfgs, capacity_multiplier=capacity_multiplier,
+                                       skin_factors=[per_group[k] for k in keys])
+                res = ens.autotune(steps=steps, reps=reps)
+                trials[tag] = (min(res["ms_per_step"].values()), res["best"])
+                del ens
+            return trials[tag][0]
+
+        best = {g: factors[0] for g in groups}
+        best_t = None
+        for f in factors:
+            cand = {g: f for g in groups}
+            t = time_of(cand)
+            if best_t is None or t < best_t:
+                best, best_t = cand, t
+        if len(groups) > 1:
+            for g in groups:
+                for f in factors:
+                    cand = dict(best)
+                    cand[g] = f
+                    t = time_of(cand)
+                    if t < best_t:
+                        best, best_t = cand, t
+        layout = trials[tuple(best[g] for g in groups)][1]
+        ens = cls.from_configs(cfgs, capacity_multiplier=capacity_multiplier,
+                               order=layout["order"], xcd=layout["xcd"],
+                               skin_factors=[best[k] for k in keys])
+        report = {"groups": [str(g) for g in groups],
+                  "skin_factor_per_group": [best[g] for g in groups],
+                  "layout": layout, "ms_per_step": best_t,
+                  "trials_ms_per_step": {"/".join(f"{x:g}" for x in tag): t
+                                         for tag, (t, _) in trials.items()}}
+        return ens, report
 
     def __del__(self):
         try:

@@ -130,6 +130,27 @@ def test_3d_members_equal_their_own_fused_steps():
         _assert_equal(ens, k, st, nb, "3d")
 
 
+def test_tuned_ensemble_members_equal_their_own_steps():
+    """Ensemble.tuned_from_configs searches each group's Verlet-skin width
+    and the block layout; the members of the ensemble it returns equal their
+    own fused steps with the skin chosen for them, bitwise."""
+    from jax_sph_amd.ensemble import setup_member
+    spec = MIXED[:3]
+    ens, rep = Ensemble.tuned_from_configs(_cfgs(spec), factors=(0.05, 0.15), steps=4, reps=1)
+    assert len(rep["skin_factor_per_group"]) == len(rep["groups"]) == 2
+    assert len(rep["trials_ms_per_step"]) >= 2
+    assert ens.steps == 0
+    steps = 6
+    ens.step(steps)
+    torch.cuda.synchronize()
+    for k, (cfg, f) in enumerate(zip(_cfgs(spec), ens.skin_factors)):
+        assert f in (0.05, 0.15)
+        adv, st, nb, dt = setup_member(cfg, skin_factor=f)
+        for _ in range(steps):
+            st, nb = adv(dt, st, nb)
+        _assert_equal(ens, k, st, nb, "tuned")
+
+
 def test_member_matches_oracle():
     """A member after 5 batched steps against the float32/float64 oracles and
     the oracle's exact pair set at its new positions."""
