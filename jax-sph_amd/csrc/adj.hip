@@ -112,7 +112,7 @@ __global__ __launch_bounds__(kAB) void k_adj_forces(AdjArgs aa, float4* __restri
     const V3 dr = min_image<DIM>(Pk, Pj, a.geo);
     const float d2 = dist2<DIM>(dr.x, dr.y, dr.z);
     if (d2 == 0.0f) return;  // self pair: every term and derivative vanishes
-    if (!in_cutoff<DIM>(a, d2, Pk, Pj, k, j)) return;
+    if (!in_cutoff<DIM>(a, d2, Pk, Pj)) return;
     const float4 Uj = a.R.U1[j], Qj = a.R.Q1[j], Lj4 = aa.la[j];
     const float mj = Pj.w, rhoj = Uj.w, pj = Qj.x, etaj = Qj.y;
     const float d = sqrtf(d2);
@@ -194,7 +194,7 @@ __global__ __launch_bounds__(kAB) void k_adj_density(AdjArgs aa, const float4* _
     const V3 dr = min_image<DIM>(Pk, Pj, a.geo);
     const float d2 = dist2<DIM>(dr.x, dr.y, dr.z);
     if (d2 == 0.0f) return;
-    if (!in_cutoff<DIM>(a, d2, Pk, Pj, k, j)) return;
+    if (!in_cutoff<DIM>(a, d2, Pk, Pj)) return;
     const float d = sqrtf(d2);
     const float w1 = K.dw(d * p.one_over_h);
     rb = rb + ((Bk + ab0[j].w) * w1 / d) * dr;

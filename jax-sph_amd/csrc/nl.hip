@@ -876,10 +876,6 @@ struct SkinIdxArgs {
   int n, npad, tpp, lane_width;
   // reference cells no wider than the cutoff: the pair's reference cells must
   // be adjacent (the grid of the reference's candidate search)
-  int adj;
-  float rcs[3];
-  int rcps[3];
-  float cut_lo;  // below it every pair lies in adjacent reference cells
 };
 
 template <int DIM, bool WRITE>
@@ -894,12 +890,7 @@ __global__ void k_skin_pairs(SkinIdxArgs a) {
     const int32_t* lst = a.ell + ((int64_t)h * a.lane_width) * a.npad + s;
     for (int k = 0; k < c; ++k) {
       const int j = a.perm[lst[(int64_t)k * a.npad]];
-      const float4 rj = load_pos<DIM>(a.r, j);
-      const float d2 = metric_sq_exact<DIM>(ri, rj, a.box, a.half);
-      if (!(d2 < a.cutoff_sq)) continue;
-      if (a.adj && !(d2 < a.cut_lo) &&
-          !ref_cells_adjacent<DIM>(ref_cell_index<DIM>(ri, a.rcs, a.rcps),
-                                   ref_cell_index<DIM>(rj, a.rcs, a.rcps), a.rcps))
+      if (!(metric_sq_exact<DIM>(ri, load_pos<DIM>(a.r, j), a.box, a.half) < a.cutoff_sq))
         continue;
       if (WRITE && p < a.max_occ) {
         a.idx[p] = j;
@@ -1661,13 +1652,6 @@ extern "C" int sph_nl_write_idx_skin(sph_ws* ws, const float* r, int32_t* idx,
   a.npad = ws->npad;
   a.tpp = ws->tpp;
   a.lane_width = ws->sk.lane_width;
-  a.adj = ws->sk.adj_check ? 1 : 0;
-  for (int d = 0; d < 3; ++d) {
-    a.rcs[d] = c.cell_size[d];
-    a.rcps[d] = d < c.dim ? c.cps[d] : 1;
-  }
-  float cut_hi;
-  cut_band(c, &a.cut_lo, &cut_hi);
   const unsigned nb = (unsigned)cdiv(n, kBlock);
   if (c.dim == 2)
     SPH_LAUNCH(ws, "skin_idx_count", st, (k_skin_pairs<2, false><<<nb, kBlock, 0, st>>>(a)));

@@ -71,7 +71,7 @@ __device__ __forceinline__ void delta_L_run(const SweepArgs& a, const int blk) {
     const float4 Pj = a.R.P[j];
     const V3 dr = min_image<DIM>(Pi, Pj, a.geo);
     const float d2 = dist2<DIM>(dr.x, dr.y, dr.z);
-    if (!in_cutoff<DIM>(a, d2, Pi, Pj, s, j)) return;
+    if (!in_cutoff<DIM>(a, d2, Pi, Pj)) return;
     const float d = sqrt_fast(d2);
     const float vj = Pj.w * rcp_fast(a.R.U0[j].w);
     const V3 kg = (K.dw(d * p.one_over_h) * rcp_fast(d + kEps) * vj) * dr;  // grad W_ij V_j
@@ -127,7 +127,7 @@ __device__ __forceinline__ void delta_grad_run(const SweepArgs& a, const int blk
     const float4 Pj = a.R.P[j];
     const V3 dr = min_image<DIM>(Pi, Pj, a.geo);
     const float d2 = dist2<DIM>(dr.x, dr.y, dr.z);
-    if (!in_cutoff<DIM>(a, d2, Pi, Pj, s, j)) return;
+    if (!in_cutoff<DIM>(a, d2, Pi, Pj)) return;
     const float d = sqrt_fast(d2);
     const float rhoj = a.R.U0[j].w;
     const V3 kg = (K.dw(d * p.one_over_h) * rcp_fast(d + kEps)) * dr;
@@ -160,7 +160,7 @@ __device__ __forceinline__ void renorm_run(const SweepArgs& a, const int blk) {
     const float4 Pj = a.R.P[j];
     const V3 dr = min_image<DIM>(Pi, Pj, a.geo);
     const float d2 = dist2<DIM>(dr.x, dr.y, dr.z);
-    if (!in_cutoff<DIM>(a, d2, Pi, Pj, s, j)) return;
+    if (!in_cutoff<DIM>(a, d2, Pi, Pj)) return;
     const float w = K.w(sqrt_fast(d2) * p.one_over_h);
     nom += Pj.w * w;
     den += (Pj.w * rcp_fast(a.R.U1[j].w)) * w;
@@ -205,7 +205,7 @@ __device__ __forceinline__ void walls_run(const SweepArgs& a, const int blk) {
       const float4 Pj = a.R.P[j];
       const V3 dr = min_image<DIM>(Pi, Pj, a.geo);
       const float d2 = dist2<DIM>(dr.x, dr.y, dr.z);
-      if (!in_cutoff<DIM>(a, d2, Pi, Pj, s, j)) return;
+      if (!in_cutoff<DIM>(a, d2, Pi, Pj)) return;
       const float d = sqrt_fast(d2);
       const float w = K.w(d * p.one_over_h);
       ws += w;
@@ -303,7 +303,7 @@ __device__ __forceinline__ void force_run(const SweepArgs& a, const int blk) {
     const V3 dr = min_image<DIM>(Pi, Pj, a.geo);
     const float d2 = dist2<DIM>(dr.x, dr.y, dr.z);
     if (d2 == 0.0f) return;  // self pair / coincident: every term is exactly 0
-    if (!in_cutoff<DIM>(a, d2, Pi, Pj, s, j)) return;
+    if (!in_cutoff<DIM>(a, d2, Pi, Pj)) return;
     // 1/(d + EPS) is 1/d in float32 for every d above ~4e-9 (EPS = 2.2e-16)
     const float inv_d = rsq_fast(d2);
     const float d = d2 * inv_d;

@@ -67,40 +67,6 @@ __device__ __forceinline__ float disp_fast(float a, float b, float side, float i
   return __builtin_fmaf(-rintf(d * inv_side), side, d);
 }
 
-// Linear index of the reference-grid cell of a position: int32(fl32(x / cell))
-// per axis (partition.py:367), the float32 quotient correctly rounded through a
-// float64 one, clamped into the grid, then x + y cps_x (+ z cps_x cps_y).
-template <int DIM>
-__device__ __forceinline__ int ref_cell_index(const float4 p, const float* cs, const int* cps) {
-  const float x[3] = {p.x, p.y, p.z};
-  int h = 0, stride = 1;
-#pragma unroll
-  for (int d = 0; d < DIM; ++d) {
-    int q = (int)(float)((double)x[d] / (double)cs[d]);
-    q = q < 0 ? 0 : q;
-    q = q >= cps[d] ? cps[d] - 1 : q;
-    h += q * stride;
-    stride *= cps[d];
-  }
-  return h;
-}
-
-// Whether two reference cells are within one cell of each other along every
-// axis, periodically: the 3^d candidate cells the reference searches
-// (partition.py:832-856; every grid here has at least 3 cells per side).
-template <int DIM>
-__device__ __forceinline__ bool ref_cells_adjacent(int ha, int hb, const int* cps) {
-#pragma unroll
-  for (int d = 0; d < DIM; ++d) {
-    const int a = ha % cps[d], b = hb % cps[d];
-    ha /= cps[d];
-    hb /= cps[d];
-    const int dd = a > b ? a - b : b - a;
-    if (dd > 1 && dd != cps[d] - 1) return false;
-  }
-  return true;
-}
-
 // Hardware reciprocal, square root and reciprocal square root (v_rcp_f32,
 // v_sqrt_f32, v_rsq_f32, about 1 ulp) for
 // the pair sums, whose results are held to the reference at float32 rounding

@@ -80,10 +80,6 @@ struct sph_ws {
     int32_t* cell_count = nullptr;  // ncell, all zero between builds
     int32_t* flag = nullptr;        // device: 1 = rebuild the lists this step
     float4* posb = nullptr;         // N: position of each original id at the last build
-    // reference cells no wider than the cutoff: pairs in the band also need
-    // adjacent reference cells; rcell (N, sorted order) holds each particle's
-    bool adj_check = false;
-    int32_t* rcell = nullptr;
     sph::Bank b;                    // perm, iperm, cell_start, pos, ell, cnt of the lists
   } sk;
   // sorted particle records used by the physics sweeps

@@ -136,30 +136,18 @@ struct SweepArgs {
   // cutoff, at or above cut_hi outside, in between the exact reference metric
   // decides.  Exact lists pass every pair (cut_lo = cut_hi = +inf).
   float cut_lo, cut_hi, cut_sq;
-  // Skin lists on a reference grid whose cells are no wider than the cutoff:
-  // the reference cell of every sorted particle, written by the state gather
-  // (null otherwise), and that grid.  A pair in the band is then also required
-  // to lie in adjacent reference cells, as the reference's candidate search.
-  int32_t* rcell;
-  float rcs[3];
-  int rcps[3];
   int64_t* info;  // non-null: the density sweep counts pairs (per-workgroup partials in part)
   int64_t* part;  // [2 * gridDim.x] pairs and largest list of each workgroup
   int remap;      // XCD-contiguous block order (grid a multiple of 8)
 };
 
 // Pair (sweep centre s, listed j) of the neighbour set: the reference predicate
-// metric_sq(r_s, r_j) < cutoff^2 with s in the sender role, as in the build,
-// and, where reference cells are no wider than the cutoff, reference cells
-// that the 3^d candidate search visits (only a pair within rounding of the
-// cutoff can lie further apart, so only the band needs the test).
+// metric_sq(r_s, r_j) < cutoff^2 with s in the sender role, as in the build.
 template <int DIM>
-__device__ __forceinline__ bool in_cutoff(const SweepArgs& a, float d2, float4 Pi, float4 Pj,
-                                          int s, int j) {
+__device__ __forceinline__ bool in_cutoff(const SweepArgs& a, float d2, float4 Pi, float4 Pj) {
   if (d2 < a.cut_lo) return true;
   if (d2 >= a.cut_hi) return false;
-  if (!(metric_sq_exact<DIM>(Pi, Pj, a.geo.box, a.geo.half) < a.cut_sq)) return false;
-  return !a.rcell || ref_cells_adjacent<DIM>(a.rcell[s], a.rcell[j], a.rcps);
+  return metric_sq_exact<DIM>(Pi, Pj, a.geo.box, a.geo.half) < a.cut_sq;
 }
 
 // Visits the pairs of sorted particle s.  The next pair index (a
@@ -243,7 +231,6 @@ __device__ __forceinline__ void gather_state_run(const SweepArgs& a, const float
   const sph_state_in& in = a.in;
   V3 rr = load_vec<DIM>(r, i);
   a.R.P[s] = f4(rr, in.mass[i]);
-  if (a.rcell) a.rcell[s] = ref_cell_index<DIM>(f4(rr, 0.0f), a.rcs, a.rcps);
   if (nl_pos && (!gate || *gate)) {
     nl_pos[s] = f4(rr, 0.0f);
     if (posb) posb[i] = f4(rr, 0.0f);
@@ -342,7 +329,7 @@ __device__ __forceinline__ int density_of(const SweepArgs& a, const int s, const
     const float4 Pj = a.R.P[j];
     const V3 dr = min_image<DIM>(Pi, Pj, a.geo);
     const float d2 = dist2<DIM>(dr.x, dr.y, dr.z);
-    if (!in_cutoff<DIM>(a, d2, Pi, Pj, s, j)) return;
+    if (!in_cutoff<DIM>(a, d2, Pi, Pj)) return;
     ++npair;
     const float d = sqrt_fast(d2);
     const float q = d * p.one_over_h;
@@ -590,11 +577,6 @@ inline SweepArgs make_args(sph_ws* ws, int bank, const sph_solver_params* prm, c
     cut_band(ws->cfg, &a.cut_lo, &a.cut_hi);
   } else {
     a.cut_lo = a.cut_hi = INFINITY;
-  }
-  a.rcell = bank == kSkinBank && ws->sk.adj_check ? ws->sk.rcell : nullptr;
-  for (int d = 0; d < 3; ++d) {
-    a.rcs[d] = ws->cfg.cell_size[d];
-    a.rcps[d] = d < ws->cfg.dim ? ws->cfg.cps[d] : 1;
   }
   a.info = nullptr;
   a.part = ws->part_sweep;

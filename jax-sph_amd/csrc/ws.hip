@@ -5,6 +5,7 @@
 #include <cstring>
 #include <string>
 
+#include "skin_grid.h"
 #include "sph_internal.h"
 
 namespace sph {
@@ -171,21 +172,13 @@ extern "C" int sph_skin_enable(sph_ws* ws, float skin, int32_t width) {
   int32_t cps[3] = {1, 1, 1};
   float cs[3] = {1.0f, 1.0f, 1.0f};
   int64_t ncell = 1;
-  // Pairs inside the cutoff lie in adjacent reference cells when the cells are
-  // wider than the cutoff with room for the rounding of r / cell.  Where a cell
-  // is no wider than that (a box of a whole number of cutoffs), a pair within
-  // rounding of the cutoff can lie two cells apart, and the reference's 3^d
-  // candidate search drops it: the sweeps then also test the reference cells
-  // of the pairs in the band (every other pair is far enough inside the cutoff
-  // to lie in adjacent cells, provided the cell is at most 2e-6 box narrower
-  // than the cutoff; the band is 32 ulp(box) wide in distance).
-  bool adj = false;
+  // The skin lists filter by distance alone, so they need every pair inside
+  // the cutoff to lie in adjacent reference cells, as the reference's 3^d
+  // candidate search requires; on a box of a whole number of cutoffs a pair
+  // within rounding of the cutoff may not.
   for (int d = 0; d < c.dim; ++d) {
     if (c.cps[d] > 100000) return 0;
-    if ((double)c.cell_size[d] * (1.0 - 1e-4) < (double)c.cutoff) {
-      adj = true;
-      if ((double)c.cell_size[d] < (double)c.cutoff - 2e-6 * maxbox || c.cps[d] < 3) return 0;
-    }
+    if (!cutoff_pairs_in_adjacent_cells(c.box[d], c.cell_size[d], c.cps[d], c.cutoff_sq)) return 0;
   }
   for (int d = 0; d < c.dim; ++d) {
     // skin cells strictly wider than cutoff + skin, at least 3 per side
@@ -214,7 +207,6 @@ extern "C" int sph_skin_enable(sph_ws* ws, float skin, int32_t width) {
   A(S.b.pos, N);
   A(S.b.ell, ws->tpp * lw * npad);
   A(S.b.cnt, ws->tpp * npad);
-  if (adj) A(S.rcell, N);
 #undef A
   if (rc == SPH_OK) {
     const int32_t one = 1;  // the first step builds the lists
@@ -224,14 +216,13 @@ extern "C" int sph_skin_enable(sph_ws* ws, float skin, int32_t width) {
   }
   if (rc != SPH_OK) {
     void* ps[] = {S.cell_count, S.flag, S.posb, S.b.perm, S.b.iperm, S.b.cell_start, S.b.pos,
-                  S.b.ell, S.b.cnt, S.rcell};
+                  S.b.ell, S.b.cnt};
     hipDeviceSynchronize();
     for (void* p : ps)
       if (p) hipFree(p);
     S = sph_ws::Skin();
     return rc;
   }
-  S.adj_check = adj;
   for (int d = 0; d < 3; ++d) {
     S.cps[d] = cps[d];
     S.cell_size[d] = cs[d];
@@ -263,7 +254,7 @@ extern "C" int sph_ws_destroy(sph_ws* ws) {
   {
     auto& S = ws->sk;
     void* ps[] = {S.cell_count, S.flag, S.posb, S.b.perm, S.b.iperm, S.b.cell_start, S.b.pos,
-                  S.b.ell, S.b.cnt, S.rcell};
+                  S.b.ell, S.b.cnt};
     for (void* p : ps)
       if (p) hipFree(p);
   }
