@@ -62,7 +62,7 @@ def parse():
                     help="run only the ensemble leg and print its JSON")
     ap.add_argument("--cpu-sample-side", type=int, default=64)
     ap.add_argument("--skin", default="auto",
-                    help="Verlet skin of the fused step's pair lists ('auto' = 0.1 cutoff, "
+                    help="Verlet skin of the fused step's pair lists ('auto' = 0.08 cutoff, "
                          "0 = exact lists rebuilt every step)")
     ap.add_argument("--xcd-remap", type=int, default=1,
                     help="1: XCD-contiguous block order in the pair build and sweeps")

@@ -73,7 +73,7 @@ def neighbor_list(displacement_or_metric, box_size, r_cutoff: float,
                   skin="auto"):
     """partition.py:492-571: same arguments; ``num_particles_max``/``pbc`` only
     matter to the matscipy backend.  ``skin`` (not in the reference) sets the
-    Verlet skin of the fused step's lists for ``jaxmd_vmap`` ("auto" = 0.1 *
+    Verlet skin of the fused step's lists for ``jaxmd_vmap`` ("auto" = 0.08 *
     cutoff, 0 = rebuild exact lists every step); results are the same."""
     assert backend in BACKENDS, f"Unknown backend {backend}"
     kw = dict(num_particles_max=num_particles_max, num_partitions=num_partitions, pbc=pbc)

@@ -36,7 +36,7 @@ def test_valid_configurations_construct_without_a_device():
     fns = _make(backend="jaxmd_scan", num_partitions=4)
     assert fns.search.backend == "jaxmd_scan" and fns.search.num_partitions == 4
     assert _make(skin=0.0).search.skin == 0.0
-    assert _make(skin="auto").search.skin == pytest.approx(0.01, rel=1e-6)
+    assert _make(skin="auto").search.skin == pytest.approx(0.008, rel=1e-6)
 
 
 def test_unknown_backend():
