@@ -66,6 +66,9 @@ def parse():
                          "0 = exact lists rebuilt every step)")
     ap.add_argument("--xcd-remap", type=int, default=1,
                     help="1: XCD-contiguous block order in the pair build and sweeps")
+    ap.add_argument("--scatter-probe", action="store_true",
+                    help="only time the j-side scatter-adds that half-pair (Newton's third "
+                         "law) sweeps would need at --n-side, and print them")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (5 times), to "
                          "scale hardware byte counters; prints nothing else")
@@ -81,6 +84,47 @@ def copy_calibration(mb):
         dst.copy_(src)
     torch.cuda.synchronize()
     print(json.dumps({"copy_calibration_mib": mb, "launches": 5}))
+
+
+def scatter_probe(n_side, reps=5):
+    """Cost of the j-side updates of half-pair sweeps on the 3D TGV lattice:
+    each unordered pair inside the cutoff is evaluated once and its result is
+    added to both particles, so the pair's second particle receives a
+    scatter-add (6 floats for du/dt and dv/dt, 1 for a density sum).  The
+    pairs are the exact neighbour list at n_side^3 in its sender-major order,
+    the order a sweep visits them; torch's index_add_ is the scatter (float
+    atomics on the device)."""
+    import torch
+    from jax_sph_amd import case_setup as cs
+    from jax_sph_amd import partition
+    cfg = cs.load_config("tgv", {"case": {"dim": 3, "dx": 2 * np.pi / n_side}})
+    (cfg, box, state, g_ext_fn, bc_fn, nw_fn, eos, key, disp, shift) = \
+        cs.load_case("tgv")(cfg).initialize()
+    solver = cs.make_solver(cfg, disp, eos, g_ext_fn)
+    fn = partition.neighbor_list(disp, box, r_cutoff=solver._kernel_fn.cutoff,
+                                 capacity_multiplier=1.25, mask_self=True,
+                                 format=partition.Sparse, skin=0.0)
+    nb = fn.allocate(state["r"])
+    n = state["r"].shape[0]
+    recv, send = nb.idx[0].long(), nb.idx[1].long()
+    half = (recv < n) & (recv > send)
+    recv = recv[half].contiguous()
+    m = int(recv.numel())
+    out = {"n": n, "half_pairs": m}
+    for name, width in (("force", 6), ("density", 1)):
+        vals = torch.ones((m, width), dtype=torch.float32, device="cuda")
+        acc = torch.zeros((n, width), dtype=torch.float32, device="cuda")
+        acc.index_add_(0, recv, vals)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        torch.cuda.synchronize()
+        ev[0].record()
+        for _ in range(reps):
+            acc.index_add_(0, recv, vals)
+        ev[1].record()
+        torch.cuda.synchronize()
+        out[f"{name}_scatter_us"] = ev[0].elapsed_time(ev[1]) * 1e3 / reps
+        out[f"{name}_float_adds"] = m * width
+    print(json.dumps({"scatter_probe": out}), flush=True)
 
 
 def fwd_adjoint_leg(steps=100, every=10, seed=0):
@@ -406,6 +450,9 @@ def main():
     a = parse()
     if a.copy_calibration_mb:
         copy_calibration(a.copy_calibration_mb)
+        return
+    if a.scatter_probe:
+        scatter_probe(a.n_side)
         return
     if a.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not "
