@@ -115,22 +115,25 @@ __global__ __launch_bounds__(kAB) void k_adj_forces(AdjArgs aa, float4* __restri
     if (!in_cutoff<DIM>(a, d2, Pk, Pj)) return;
     const float4 Uj = a.R.U1[j], Qj = a.R.Q1[j], Lj4 = aa.la[j];
     const float mj = Pj.w, rhoj = Uj.w, pj = Qj.x, etaj = Qj.y;
-    const float d = sqrtf(d2);
+    // hardware rsqrt/rcp (about 1 ulp), as in the forward sweeps; 1/(d + eps)
+    // is 1/d in float32 for every d above ~4e-9 (eps = 2.2e-16)
+    const float inv = rsq_fast(d2);
+    const float d = d2 * inv;
     const float q = d * p.one_over_h;
     const float w1 = K.dw(q);
     const float w2 = K.d2w(q);
-    const float inv = 1.0f / (d + kEps);
     const float A0 = w1 * inv;                      // W'(d) / (d + eps)
-    const float Ad0 = w2 * inv - w1 * inv * inv;    // d/dd of the above
-    const float inv_mj = 1.0f / mj;
+    const float Ad0 = (w2 - w1 * inv) * inv;        // d/dd of the above
+    const float inv_mj = rcp_fast(mj);
     const float Akj = A0 * inv_mk, Ajk = A0 * inv_mj;
     const float Adkj = Ad0 * inv_mk, Adjk = Ad0 * inv_mj;
-    const float Vj2 = (mj / rhoj) * (mj / rhoj);
+    const float Vj = mj * rcp_fast(rhoj);
+    const float Vj2 = Vj * Vj;
     const float Vs = Vk2 + Vj2;
-    const float ckj = Vs * Akj, cjk = Vs * Ajk;
     const float rs = rhok + rhoj;
-    const float pkj = (rhoj * pk + rhok * pj) / rs;
-    const float etakj = 2.0f * etak * etaj / (etak + etaj + kEps);
+    const float irs = rcp_fast(rs);
+    const float pkj = (rhoj * pk + rhok * pj) * irs;
+    const float etakj = 2.0f * etak * etaj * rcp_fast(etak + etaj + kEps);
     const V3 uj = v3(Uj);
     V3 T = (-pkj) * dr + etakj * (uk - uj);
     V3 wj = {0.f, 0.f, 0.f};
@@ -138,14 +141,14 @@ __global__ __launch_bounds__(kAB) void k_adj_forces(AdjArgs aa, float4* __restri
       wj = v3(a.R.V0[j]) - uj;
       T = T + (0.5f * rhok * dot<DIM>(wk, dr)) * uk + (0.5f * rhoj * dot<DIM>(wj, dr)) * uj;
     }
-    const float Dp = rhoj * (dPk * rs + (pj - pk)) / (rs * rs);  // d p_kj / d rho_k (via p_k)
+    const float Dp = rhoj * (dPk * rs + (pj - pk)) * (irs * irs);  // d p_kj / d rho_k (via p_k)
     const V3 lj = v3(Lj4);
-    const V3 lc = ckj * lk - cjk * lj;   // (V_k^2 + V_j^2) W'/(d + eps) l_kj
-    const V3 lA = Akj * lk - Ajk * lj;
+    const V3 lA = Akj * lk - Ajk * lj;   // W'/(d + eps) l_kj
+    const V3 lc = Vs * lA;               // (V_k^2 + V_j^2) W'/(d + eps) l_kj
     const V3 lAd = Adkj * lk - Adjk * lj;
     rhob += (-2.0f * Vk2 / rhok) * dot<DIM>(lA, T) - Dp * dot<DIM>(lc, dr);
     ub = ub + etakj * lc;
-    rb = rb + Vs * ((dot<DIM>(lAd, T) / d) * dr - pkj * lA);
+    rb = rb + Vs * ((dot<DIM>(lAd, T) * inv) * dr - pkj * lA);
     if (TVF) {
       const V3 nj = v3(aa.lb[j]);
       const V3 nA = Akj * nk - Ajk * nj;     // W'/(d + eps) n_kj
@@ -153,7 +156,7 @@ __global__ __launch_bounds__(kAB) void k_adj_forces(AdjArgs aa, float4* __restri
       const float uk_lc = dot<DIM>(uk, lc), wk_r = dot<DIM>(wk, dr);
       // G: the volume factor and the pair geometry
       rhob += (-2.0f * Vk2 / rhok) * pbg * dot<DIM>(nA, dr);
-      rb = rb + (Vs * pbg) * ((dot<DIM>(nAd, dr) / d) * dr + nA);
+      rb = rb + (Vs * pbg) * ((dot<DIM>(nAd, dr) * inv) * dr + nA);
       // the stress A_k = rho_k u_k w_k^T of T: through rho_k, u_k, v_k and r
       rhob += 0.5f * wk_r * uk_lc;
       ub = ub + (0.5f * rhok) * (wk_r * lc - uk_lc * dr);
@@ -195,9 +198,9 @@ __global__ __launch_bounds__(kAB) void k_adj_density(AdjArgs aa, const float4* _
     const float d2 = dist2<DIM>(dr.x, dr.y, dr.z);
     if (d2 == 0.0f) return;
     if (!in_cutoff<DIM>(a, d2, Pk, Pj)) return;
-    const float d = sqrtf(d2);
-    const float w1 = K.dw(d * p.one_over_h);
-    rb = rb + ((Bk + ab0[j].w) * w1 / d) * dr;
+    const float inv_d = rsq_fast(d2);
+    const float w1 = K.dw(d2 * inv_d * p.one_over_h);
+    rb = rb + ((Bk + ab0[j].w) * w1 * inv_d) * dr;
   });
   rb = lane_sum<TPP>(rb);
   if (h != 0) return;
