@@ -296,6 +296,8 @@ def fwd_adjoint_3d_leg(n_side=64, steps=100, every=10, seed=0):
             "t_forward_skin_s": timings["t_forward_skin_s"],
             "t_forward_exact_s": timings["t_forward_exact_s"],
             "t_forward_plus_adjoint_s": t_fa, "graph_setup_s": setup_s,
+            "workspace_sets": gr.n_sets, "segments_kept_in_place": gr.keep_all,
+            "workspace_gb": sum(nb.workspace.bytes for st_ in gr.sets for nb in st_) / 1e9,
             "cost_ratio": t_fa / timings["t_forward_skin_s"],
             "cost_ratio_vs_exact_forward": t_fa / timings["t_forward_exact_s"],
             "particle_steps_per_s_fwd_adjoint": n * steps / t_fa}

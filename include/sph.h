@@ -129,7 +129,10 @@ int64_t sph_ws_bytes(const sph_ws *ws);
  * launches XLA decides).  "xcd_remap" (default 1): the pair build and the
  * sweeps hand each of the 8 XCDs a contiguous range of cell-sorted particles.
  * "skin_rebuild" (1): the next fused step rebuilds the skin lists, as after
- * sph_skin_enable (synchronous; outside graph capture). */
+ * sph_skin_enable (synchronous; outside graph capture).
+ * "ell_overflow" (1): for a workspace whose ell_width is below 3^dim x
+ * cell_capacity, a particle with more pairs than fit raises
+ * SPH_NEIGHBOR_LIST_OVERFLOW (default 0: such a width is not expected). */
 int sph_ws_tune(sph_ws *ws, const char *key, int64_t value);
 
 /* ---- neighbour search: jax_sph/partition.py:492-571 + jax_md/partition.py:723-1026 */

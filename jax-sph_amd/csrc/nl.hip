@@ -1029,7 +1029,9 @@ int build_pairs(sph_ws* ws, int bank, int64_t max_occupancy, int64_t* info, hipS
   a.lane_width = ws->lane_width;
   a.mask_self = c.mask_self;
   a.part = ws->part_build;
-  a.trunc_bits = 0;
+  // 3^d x cell capacity entries per particle never truncate; a narrower
+  // buffer reports a truncated sub-list as an overflow
+  a.trunc_bits = ws->ell_overflow ? SPH_NEIGHBOR_LIST_OVERFLOW : 0;
   a.gate = nullptr;
   a.remap = ws->xcd_remap;
   // The cell-adjacent image is the minimum image of every pair within the

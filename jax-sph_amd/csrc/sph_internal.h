@@ -49,6 +49,9 @@ struct sph_ws {
   int32_t tpp = 1;         // lanes per particle (tpp_of(dim))
   int32_t lane_width = 0;  // capacity of one sub-list
   int32_t xcd_remap = 1;   // XCD-contiguous block order of the pair build and sweeps
+  // exact lists narrower than 3^d x cell capacity (sph_ws_tune "ell_overflow"):
+  // a truncated sub-list raises SPH_NEIGHBOR_LIST_OVERFLOW
+  int32_t ell_overflow = 0;
   int64_t bytes = 0;
   // binning scratch (shared by both banks)
   int32_t* cell_count = nullptr;  // ncell

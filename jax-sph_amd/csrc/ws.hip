@@ -272,6 +272,12 @@ extern "C" int sph_ws_tune(sph_ws* ws, const char* key, int64_t value) {
     ws->xcd_remap = value ? 1 : 0;
     return SPH_OK;
   }
+  if (std::strcmp(key, "ell_overflow") == 0) {
+    // exact pair buffers sized below 3^d x cell capacity per particle: a
+    // sub-list that would be truncated raises SPH_NEIGHBOR_LIST_OVERFLOW
+    ws->ell_overflow = value ? 1 : 0;
+    return SPH_OK;
+  }
   if (std::strcmp(key, "skin_rebuild") == 0) {
     // the next fused step rebuilds the skin lists, as the first step after
     // sph_skin_enable does (synchronous; not for use under graph capture)

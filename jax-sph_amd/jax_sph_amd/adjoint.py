@@ -217,11 +217,21 @@ class GraphRollout:
     while the calling stream applies V, segment after segment.  A set is
     reused only after its V has run.  ``n_steps`` must be a multiple of
     ``k``.  The gradients equal :func:`rollout`'s bit for bit.
+
+    With as many sets as segments, the forward pass itself runs R[s] for
+    segment s, so every step's state and pair list stays in place, and the
+    backward pass of that forward only replays the V graphs: memory for the
+    recompute.  ``workspace_sets=None`` takes that route when all sets fit in
+    half of the free device memory, else 4 sets up to 100k particles and 2
+    above.  The workspaces hold ``list_margin`` times the largest pair list
+    of the start per particle; a list that outgrows that raises the overflow
+    error below (a larger margin, or :func:`rollout`, which re-allocates).
     """
 
     def __init__(self, state: Dict[str, torch.Tensor], solver: WCSPH, neighbor_fn, shift_fn,
                  dt: float, n_steps: int, tvf: float = 0.0, checkpoint_every: int = 10,
-                 workspace_sets: Optional[int] = None, side_streams: int = 3, bc_fn=None):
+                 workspace_sets: Optional[int] = None, side_streams: int = 3, bc_fn=None,
+                 list_margin: float = 1.5):
         _check(solver, tvf, bc_fn)
         if getattr(getattr(neighbor_fn, "search", None), "skin", 0.0):
             raise ValueError("the differentiable rollout needs a neighbor_list with skin=0")
@@ -239,11 +249,31 @@ class GraphRollout:
         carried = _carried(self.tvf)
         base = solver.state_in(state)
         n_seg = self.n_steps // self.k
-        if workspace_sets is None:   # concurrency pays at small sizes; memory rules large ones
-            workspace_sets = 4 if base["r"].shape[0] <= 100_000 else 2
+        # Each step keeps its list in a workspace of its own, so the pair
+        # buffers are sized from the largest list at the start (times
+        # list_margin) rather than for 3^d x cell capacity candidates per
+        # particle; a list that outgrows them raises the overflow bit.
+        probe = neighbor_fn.allocate(base["r"])
+        width = int(np.ceil(int(probe.info[3]) * float(list_margin))) + 16
+        del probe
+
+        def alloc():
+            return neighbor_fn.allocate(base["r"], ell_width=width)
+        self.sets = [[alloc() for _ in range(self.k)]]
+        if workspace_sets is None:
+            # one set: its workspaces, plus the k states its recompute keeps
+            # and the graphs' scratch, both counted as much again
+            per_set = sum(nb.workspace.bytes for nb in self.sets[0])
+            per_set += 2 * self.k * sum(v.numel() * v.element_size() for v in base.values())
+            free, _ = torch.cuda.mem_get_info()
+            if n_seg * per_set <= free // 2:
+                workspace_sets = n_seg
+            else:   # concurrency pays at small sizes; memory rules large ones
+                workspace_sets = 4 if base["r"].shape[0] <= 100_000 else 2
         self.n_sets = max(1, min(int(workspace_sets), n_seg))
-        self.sets = [[neighbor_fn.allocate(base["r"]) for _ in range(self.k)]
-                     for _ in range(self.n_sets)]
+        # every segment keeps its steps in a set of its own: no recompute
+        self.keep_all = self.n_sets == n_seg
+        self.sets += [[alloc() for _ in range(self.k)] for _ in range(self.n_sets - 1)]
         # carried fields live in one (3, N, d) buffer: a checkpoint is one copy
         self.base_buf = torch.stack([base[key] for key in carried]).contiguous()
         self.S_buf = self.base_buf.clone()
@@ -311,6 +341,8 @@ class GraphRollout:
 
     def forward(self, u0, r0):
         """Runs the rollout; returns (u_final, r_final, checkpoints)."""
+        if self.keep_all:
+            return self._forward_kept(u0, r0)
         self.S_buf.copy_(self.base_buf)
         self.S["u"].copy_(u0)
         self.S["v"].copy_(u0)
@@ -334,16 +366,48 @@ class GraphRollout:
         last = self._steps[s][-1][0]
         return last["u"].clone(), last["r"].clone(), ckpts, self.generation
 
+    def _forward_kept(self, u0, r0):
+        """forward() with a set per segment: segment s runs R[s] from the end
+        state of segment s - 1, and all of them stay in place."""
+        carried = _carried(self.tvf)
+        n = self.n_steps // self.k
+        self.bits.zero_()
+        ckpts = []
+        for seg in range(n):
+            c = self.C[seg]
+            if seg == 0:
+                self.C_bufs[0].copy_(self.base_buf)
+                c["u"].copy_(u0)
+                c["v"].copy_(u0)
+                c["r"].copy_(r0)
+            else:
+                last = self._steps[seg - 1][-1][0]
+                for key in carried:
+                    c[key].copy_(last[key])
+                c["v"].copy_(last["v"])
+            ckpts.append(self.C_bufs[seg].clone())   # for a backward after a later forward
+            self.gR[seg].replay()
+        if int(self.bits) & 3:
+            raise RuntimeError("neighbour-list overflow inside the graph rollout; use "
+                               "adjoint.rollout (re-allocates) or a larger capacity")
+        self.generation += 1
+        last = self._steps[n - 1][-1][0]
+        return last["u"].clone(), last["r"].clone(), ckpts, self.generation
+
     def backward(self, ckpts, ub_final, rb_final, generation=None):
         """Cotangents of (r0, u0).  When no other forward ran since the one
-        that made `ckpts` (same generation), its last segment is still in
-        place and is not recomputed."""
+        that made `ckpts` (same generation), its last segment (with a set per
+        segment: every segment) is still in place and is not recomputed."""
         main = torch.cuda.current_stream()
         self.bar[0].copy_(rb_final)
         self.bar[1].copy_(ub_final)
         for b in self.bar[2:]:
             b.zero_()
         n = len(ckpts)
+        if self.keep_all and generation == self.generation:
+            for seg in reversed(range(n)):
+                self.gV[seg].replay()
+            return self.bar[0].clone(), self.bar[1].clone()
         ev_r = [torch.cuda.Event() for _ in range(n)]
         ev_v = [torch.cuda.Event() for _ in range(n)]
         for side in self.sides:

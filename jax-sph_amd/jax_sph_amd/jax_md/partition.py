@@ -292,7 +292,13 @@ class _NeighborSearch:
         return c
 
     # ------------------------------------------------------------ builds
-    def allocate(self, position, extra_capacity: int = 0, **kwargs) -> NeighborList:
+    def allocate(self, position, extra_capacity: int = 0, ell_width: Optional[int] = None,
+                 **kwargs) -> NeighborList:
+        """partition.py:911-1021 (neighbor_list_fn with neighbors=None, then
+        allocate_fn).  ``ell_width`` (not in the reference) caps the
+        pair buffer at that many pairs per particle instead of the 3^d x cell
+        capacity candidates; a particle with more pairs then raises
+        NEIGHBOR_LIST_OVERFLOW (used by workspaces that hold many lists)."""
         dev = _lib.device()
         r = _lib.as_f32(position, dev)
         n, dim = r.shape
@@ -316,7 +322,11 @@ class _NeighborSearch:
             width = min(3 ** dim * max(cap, 1), n)
         else:
             width = n
+        if ell_width is not None:
+            width = max(1, min(width, int(ell_width)))
         ws = _lib.Workspace(self._config(n, dim, cap, width))
+        if ell_width is not None:
+            ws.tune("ell_overflow", 1)
         ws.n, ws.dim, ws.cell_capacity = n, dim, cap
         ws.candidate_width = n * (3 ** dim) * cap if self.use_cells else n * n
         bank = 0

@@ -332,20 +332,27 @@ def test_c4_gradient_matches_float64_finite_differences():
     assert torch.equal(u2.grad, g_mse)
 
 
-def test_graph_rollout_equals_eager_rollout():
+@pytest.mark.parametrize("sets", [None, 1, 2])
+def test_graph_rollout_equals_eager_rollout(sets):
     """GraphRollout (both passes as HIP graphs, one workspace per step of a
     segment) gives the eager rollout's values and gradients bit for bit, also
-    when called twice."""
+    when called twice: with a workspace set per segment (the forward keeps
+    every step, the backward only applies the transposes; None picks that at
+    this size) and with fewer sets than segments (the backward recomputes).
+    Two forwards before the first backward make the kept sets stale, and the
+    backward of the first then recomputes its segments."""
     cfg, box, st, solver, fn, shift, gen = _setup()
     n = st["r"].shape[0]
     w = torch.randn(n, 2, generator=gen).cuda()
     u0 = st["u"].clone().requires_grad_(True)
     s = dict(st)
     s["u"] = u0
-    u, r, _ = adjoint.rollout(s, solver, fn, shift, cfg.solver.dt, 8, checkpoint_every=4)
+    u, r, _ = adjoint.rollout(s, solver, fn, shift, cfg.solver.dt, 12, checkpoint_every=4)
     ((w * u).sum() + (w * r).sum()).backward()
     g_eager = u0.grad.clone()
-    gr = adjoint.GraphRollout(st, solver, fn, shift, cfg.solver.dt, 8, checkpoint_every=4)
+    gr = adjoint.GraphRollout(st, solver, fn, shift, cfg.solver.dt, 12, checkpoint_every=4,
+                              workspace_sets=sets)
+    assert gr.keep_all == (sets is None)
     for _ in range(2):
         uu = st["u"].clone().requires_grad_(True)
         ug, rg = gr(uu, st["r"])
@@ -353,3 +360,11 @@ def test_graph_rollout_equals_eager_rollout():
         torch.cuda.synchronize()
         assert torch.equal(ug, u) and torch.equal(rg, r)
         assert torch.equal(uu.grad, g_eager)
+    # a second forward (from another start) between the first and its backward
+    u1 = st["u"].clone().requires_grad_(True)
+    ug1, rg1 = gr(u1, st["r"])
+    u2 = (st["u"] * 0.5).requires_grad_(True)
+    gr(u2, st["r"])
+    ((w * ug1).sum() + (w * rg1).sum()).backward()
+    torch.cuda.synchronize()
+    assert torch.equal(u1.grad, g_eager)
