@@ -221,8 +221,9 @@ typedef struct {
 } sph_cotangent_out;
 /* Vector-Jacobian product of one si_euler step (integrator.py:22-56), the
  * transpose that jax.grad builds for the reference's step.  Standard SPH,
- * density summation, no wall boundary treatment (bc_trick)/heat/artificial
- * viscosity, QSK or WC2K; with prm->tvf_stress the transport-velocity step
+ * density summation, no heat/artificial viscosity, the generalized wall
+ * boundary condition (bc_trick, solver.py:431-530) for no-slip walls only,
+ * QSK or WC2K; with prm->tvf_stress the transport-velocity step
  * (solver.py:199-256: the A = rho u (v - u) stress and the background-pressure
  * dvdt), whose v' = u' + tvf_coef * dvdt (tvf_coef = tvf dt / 2, 0 without
  * tvf).  `bc` (nullable) is the step's case boundary hook as in sph_step:

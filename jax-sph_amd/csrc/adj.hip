@@ -2,9 +2,10 @@
 //
 // The reference differentiates its step with jax.grad (reverse mode through
 // XLA).  Here the transpose of the step is written out.  Scope: the standard
-// SPH solver with density summation, no walls, no heat and no artificial
-// viscosity, with or without the transport-velocity formulation (tvf).  State
-// n -> n+1 (integrator.py:26-54, solver.py:176-178, 199-256, 801-802):
+// SPH solver with density summation, no heat and no artificial viscosity,
+// with or without the transport-velocity formulation (tvf), and with or
+// without the generalized wall boundary condition for no-slip walls (below).
+// State n -> n+1 (integrator.py:26-54, solver.py:176-178, 199-256, 801-802):
 //
 //   u' = u + dt a,   v' = u' + c b  (c = tvf dt / 2),   r' = wrap(r + dt v')
 //   a' = F(r', u', v'),   b' = G(r')
@@ -29,6 +30,22 @@
 // ubar = ubar', abar = dt ubar', bbar = c vbar'.  Both Jacobian products are
 // gathers over the pair list of r' (two sweeps: the second needs the density
 // cotangent of the neighbours), with no atomics, like the forward sweeps.
+//
+// Generalized wall boundary condition (solver.py:431-530, no-slip, no heat):
+// before the force, every wall particle i takes Shepard averages over its
+// fluid neighbours j (w_ij = W(d_ij), S_i = sum_j w_ij + eps):
+//   u^_i = 2 u_i - sum_j w_ij u_j / S_i   (v^ likewise)
+//   p^_i = (sum_j w_ij p_j + g . sum_j rho_j w_ij r_ij) / S_i,  rho^_i = R(p^_i)
+// (R the inverse EOS; fluid particles keep rho, as R(P(rho)) = rho).  The force
+// transpose gives the cotangents of u^, v^ and rho^ (through p^ = P(rho^)); with
+// alpha_i = ubar^_i / S_i, beta_i = vbar^_i / S_i, gamma_i = pbar^_i / S_i,
+// pbar^_i = rhobar^_i / P'(rho^_i), the walls pass to their fluid neighbours
+//   ubar_j -= w_ij alpha_i,  vbar_j -= w_ij beta_i,
+//   pbar_j += w_ij gamma_i,  rhobar_j += w_ij (g . r_ij) gamma_i,
+// keep ubar_i = 2 ubar^_i, and both sides take the position derivative of
+// L_ij = -alpha_i . (u_j - uw_i) - beta_i . (v_j - vw_i)
+//        + gamma_i (p_j + rho_j g . r_ij - p^_i)        (times dw_ij)
+// plus the explicit r_ij of the pressure sum, gamma_i rho_j w_ij g.
 #include <hip/hip_runtime.h>
 
 #include "sph_device.h"
@@ -82,7 +99,9 @@ __global__ void k_adj_gather(AdjArgs a) {
 
 // Sweep A: density cotangent (times m, fluid only), the u' and v' cotangents
 // of F and the direct r' cotangent of F and G, per sorted particle k.
-template <int DIM, int KIND, bool TVF>
+// WALLS: the records after the generalized wall BC (U2, V2, Q2), and for wall
+// particles the raw cotangent of rho^ in ab0.w (the walls sweeps consume it).
+template <int DIM, int KIND, bool TVF, bool WALLS>
 __global__ __launch_bounds__(kAB) void k_adj_forces(AdjArgs aa, float4* __restrict__ ab0,
                                                     float4* __restrict__ ab1,
                                                     float4* __restrict__ ab2) {
@@ -94,7 +113,10 @@ __global__ __launch_bounds__(kAB) void k_adj_forces(AdjArgs aa, float4* __restri
   const sph_solver_params& p = a.prm;
   const KernelEval<KIND, DIM> K{p.kernel, p.sigma, p.one_over_h};
   const Eos eos = make_eos(p);
-  const float4 Pk = a.R.P[k], Uk = a.R.U1[k], Qk = a.R.Q1[k], Lk4 = aa.la[k];
+  const float4* __restrict__ RU = WALLS ? a.R.U2 : a.R.U1;
+  const float4* __restrict__ RQ = WALLS ? a.R.Q2 : a.R.Q1;
+  const float4* __restrict__ RV = WALLS ? a.R.V2 : a.R.V0;
+  const float4 Pk = a.R.P[k], Uk = RU[k], Qk = RQ[k], Lk4 = aa.la[k];
   const float mk = Pk.w, rhok = Uk.w, pk = Qk.x, etak = Qk.y;
   const float Vk2 = (mk / rhok) * (mk / rhok);
   const float dPk = eos.dp_drho(rhok);
@@ -102,7 +124,7 @@ __global__ __launch_bounds__(kAB) void k_adj_forces(AdjArgs aa, float4* __restri
   const V3 uk = v3(Uk), lk = v3(Lk4);
   // transport-velocity terms: w = v - u, the background pressure of G and the
   // sorted cotangent of b'
-  const V3 wk = TVF ? v3(a.R.V0[k]) - uk : V3{0.f, 0.f, 0.f};
+  const V3 wk = TVF ? v3(RV[k]) - uk : V3{0.f, 0.f, 0.f};
   const V3 nk = TVF ? v3(aa.lb[k]) : V3{0.f, 0.f, 0.f};
   const float pbg = TVF ? eos.p_fn(0.0f) : 0.0f;
   float rhob = 0.0f;
@@ -113,7 +135,7 @@ __global__ __launch_bounds__(kAB) void k_adj_forces(AdjArgs aa, float4* __restri
     const float d2 = dist2<DIM>(dr.x, dr.y, dr.z);
     if (d2 == 0.0f) return;  // self pair: every term and derivative vanishes
     if (!in_cutoff<DIM>(a, d2, Pk, Pj)) return;
-    const float4 Uj = a.R.U1[j], Qj = a.R.Q1[j], Lj4 = aa.la[j];
+    const float4 Uj = RU[j], Qj = RQ[j], Lj4 = aa.la[j];
     const float mj = Pj.w, rhoj = Uj.w, pj = Qj.x, etaj = Qj.y;
     // hardware rsqrt/rcp (about 1 ulp), as in the forward sweeps; 1/(d + eps)
     // is 1/d in float32 for every d above ~4e-9 (eps = 2.2e-16)
@@ -138,7 +160,7 @@ __global__ __launch_bounds__(kAB) void k_adj_forces(AdjArgs aa, float4* __restri
     V3 T = (-pkj) * dr + etakj * (uk - uj);
     V3 wj = {0.f, 0.f, 0.f};
     if (TVF) {
-      wj = v3(a.R.V0[j]) - uj;
+      wj = v3(RV[j]) - uj;
       T = T + (0.5f * rhok * dot<DIM>(wk, dr)) * uk + (0.5f * rhoj * dot<DIM>(wj, dr)) * uj;
     }
     const float Dp = rhoj * (dPk * rs + (pj - pk)) * (irs * irs);  // d p_kj / d rho_k (via p_k)
@@ -170,9 +192,157 @@ __global__ __launch_bounds__(kAB) void k_adj_forces(AdjArgs aa, float4* __restri
   if (TVF) vb = lane_sum<TPP>(vb);
   if (h != 0) return;
   const bool fluid = (int)Qk.z == kFluid;
-  ab0[k] = f4(ub, fluid ? mk * rhob : 0.0f);  // rho = m sum W only for fluid (solver.py:796)
+  // rho = m sum W only for fluid (solver.py:796); a wall's rho^ comes from its p^
+  ab0[k] = f4(ub, fluid ? mk * rhob : (WALLS ? rhob : 0.0f));
   ab1[k] = f4(rb, 0.0f);
   if (TVF) ab2[k] = f4(vb, 0.0f);
+}
+
+// Sweep W1, wall particles: their Shepard sums recomputed as the forward
+// walls sweep takes them (fluid neighbours in the cutoff), and their
+// cotangents turned into the factors the fluid side gathers:
+// wf0 = (alpha, gamma), wf1 = (beta, p^), wf2 = uw, wf3 = vw.  The wall's own
+// u (and v) then carry 2 ubar^ (2 vbar^), and its density cotangent is spent.
+template <int DIM, int KIND, bool TVF>
+__global__ __launch_bounds__(kAB) void k_adj_walls_prep(AdjArgs aa, float4* __restrict__ ab0,
+                                                        float4* __restrict__ ab2,
+                                                        float4* __restrict__ wf0,
+                                                        float4* __restrict__ wf1,
+                                                        float4* __restrict__ wf2,
+                                                        float4* __restrict__ wf3) {
+  constexpr int TPP = tpp_of(DIM);
+  const SweepArgs& a = aa.s;
+  const int tid = logical_block(a.remap) * blockDim.x + threadIdx.x;
+  const int k = tid / TPP, h = tid % TPP;
+  if (k >= a.n) return;
+  if (!is_wall((int)a.R.Q1[k].z)) return;  // a particle's lanes leave together
+  const sph_solver_params& p = a.prm;
+  const KernelEval<KIND, DIM> K{p.kernel, p.sigma, p.one_over_h};
+  const Eos eos = make_eos(p);
+  const float4 Pk = a.R.P[k];
+  float S = 0.f, Cp = 0.f;
+  V3 A = {0.f, 0.f, 0.f}, B = {0.f, 0.f, 0.f}, Cr = {0.f, 0.f, 0.f};
+  for_each_pair(a, k, h, [&](const int j) {
+    const float4 Qj = a.R.Q1[j];
+    if ((int)Qj.z != kFluid) return;
+    const float4 Pj = a.R.P[j];
+    const V3 dr = min_image<DIM>(Pk, Pj, a.geo);
+    const float d2 = dist2<DIM>(dr.x, dr.y, dr.z);
+    if (!in_cutoff<DIM>(a, d2, Pk, Pj)) return;
+    const float w = K.w(sqrt_fast(d2) * p.one_over_h);
+    const float4 Uj = a.R.U1[j];
+    S += w;
+    A = A + w * v3(Uj);
+    if (TVF) B = B + w * v3(a.R.V0[j]);
+    Cp += w * Qj.x;
+    Cr = Cr + (Uj.w * w) * dr;
+  });
+  S = lane_sum<TPP>(S);
+  A = lane_sum<TPP>(A);
+  if (TVF) B = lane_sum<TPP>(B);
+  Cp = lane_sum<TPP>(Cp);
+  Cr = lane_sum<TPP>(Cr);
+  if (h != 0) return;
+  const float inv = 1.0f / (S + kEps);
+  const V3 g = g_ext_of<DIM>(a, a.perm[k]);
+  const float phat = (Cp + dot<DIM>(g, Cr)) * inv;
+  const float4 A0 = ab0[k];
+  const float pbar = A0.w / eos.dp_drho(a.R.U2[k].w);  // rho^ = R(p^): R' = 1 / P'(rho^)
+  const V3 vbh = TVF ? v3(ab2[k]) : V3{0.f, 0.f, 0.f};
+  wf0[k] = f4(inv * v3(A0), inv * pbar);
+  wf1[k] = f4(inv * vbh, phat);
+  wf2[k] = f4(inv * A, 0.0f);
+  wf3[k] = f4(inv * B, 0.0f);
+  ab0[k] = f4(2.0f * v3(A0), 0.0f);
+  if (TVF) ab2[k] = f4(2.0f * vbh, 0.0f);
+}
+
+// Sweep W2: fluid particles gather the cotangents their wall neighbours pass
+// them (u, v, p -> rho, rho, and r); wall particles gather the derivative of
+// their averages with respect to their own position.  Each particle updates
+// only its own ab0/ab1/ab2 entries.
+template <int DIM, int KIND, bool TVF>
+__global__ __launch_bounds__(kAB) void k_adj_walls(AdjArgs aa, float4* __restrict__ ab0,
+                                                   float4* __restrict__ ab1,
+                                                   float4* __restrict__ ab2,
+                                                   const float4* __restrict__ wf0,
+                                                   const float4* __restrict__ wf1,
+                                                   const float4* __restrict__ wf2,
+                                                   const float4* __restrict__ wf3) {
+  constexpr int TPP = tpp_of(DIM);
+  const SweepArgs& a = aa.s;
+  const int tid = logical_block(a.remap) * blockDim.x + threadIdx.x;
+  const int k = tid / TPP, h = tid % TPP;
+  if (k >= a.n) return;
+  const sph_solver_params& p = a.prm;
+  const KernelEval<KIND, DIM> K{p.kernel, p.sigma, p.one_over_h};
+  const Eos eos = make_eos(p);
+  const int tag_k = (int)a.R.Q1[k].z;
+  const bool fluid_k = tag_k == kFluid;
+  if (!fluid_k && !is_wall(tag_k)) return;
+  const float4 Pk = a.R.P[k], Uk = a.R.U1[k];
+  const V3 g = g_ext_of<DIM>(a, a.perm[k]);
+  const V3 uk = v3(Uk), vk = TVF ? v3(a.R.V0[k]) : V3{0.f, 0.f, 0.f};
+  const float pk = a.R.Q1[k].x, rhok = Uk.w;
+  // a wall's own factors (fluid particles use their neighbours')
+  const float4 F0 = fluid_k ? make_float4(0.f, 0.f, 0.f, 0.f) : wf0[k];
+  const float4 F1 = fluid_k ? make_float4(0.f, 0.f, 0.f, 0.f) : wf1[k];
+  const V3 uwk = fluid_k ? V3{0.f, 0.f, 0.f} : v3(wf2[k]);
+  const V3 vwk = fluid_k || !TVF ? V3{0.f, 0.f, 0.f} : v3(wf3[k]);
+  V3 ub = {0.f, 0.f, 0.f}, vb = {0.f, 0.f, 0.f}, rb = {0.f, 0.f, 0.f};
+  float pacc = 0.f, racc = 0.f;
+  for_each_pair(a, k, h, [&](const int j) {
+    const float4 Qj = a.R.Q1[j];
+    const int tag_j = (int)Qj.z;
+    // fluid-wall pairs only, seen from either side
+    if (fluid_k ? !is_wall(tag_j) : tag_j != kFluid) return;
+    const float4 Pj = a.R.P[j];
+    const V3 dr = min_image<DIM>(Pk, Pj, a.geo);  // r_k - r_j
+    const float d2 = dist2<DIM>(dr.x, dr.y, dr.z);
+    if (d2 == 0.0f) return;
+    if (!in_cutoff<DIM>(a, d2, Pk, Pj)) return;
+    const float inv_d = rsq_fast(d2);
+    const float q = d2 * inv_d * p.one_over_h;
+    const float w = K.w(q), wd = K.dw(q);
+    if (fluid_k) {
+      // wall neighbour j: r_jk = -dr
+      const float4 G0 = wf0[j], G1 = wf1[j];
+      const V3 al = v3(G0), be = v3(G1);
+      const float ga = G0.w, gr = -dot<DIM>(g, dr);
+      ub = ub - w * al;
+      if (TVF) vb = vb - w * be;
+      pacc += w * ga;
+      racc += w * gr * ga;
+      float L = -dot<DIM>(al, uk - v3(wf2[j])) + ga * (pk + rhok * gr - G1.w);
+      if (TVF) L -= dot<DIM>(be, vk - v3(wf3[j]));
+      rb = rb + (L * wd * inv_d) * dr - (ga * rhok * w) * g;
+    } else {
+      // fluid neighbour j: r_kj = dr
+      const float4 Uj = a.R.U1[j];
+      float L = -dot<DIM>(v3(F0), v3(Uj) - uwk) + F0.w * (Qj.x + Uj.w * dot<DIM>(g, dr) - F1.w);
+      if (TVF) L -= dot<DIM>(v3(F1), v3(a.R.V0[j]) - vwk);
+      rb = rb + (L * wd * inv_d) * dr + (F0.w * Uj.w * w) * g;
+    }
+  });
+  rb = lane_sum<TPP>(rb);
+  if (fluid_k) {
+    ub = lane_sum<TPP>(ub);
+    if (TVF) vb = lane_sum<TPP>(vb);
+    pacc = lane_sum<TPP>(pacc);
+    racc = lane_sum<TPP>(racc);
+  }
+  if (h != 0) return;
+  float4 B1 = ab1[k];
+  ab1[k] = f4(v3(B1) + rb, B1.w);
+  if (fluid_k) {
+    const float4 B0 = ab0[k];
+    const float mk = a.R.P[k].w;
+    ab0[k] = f4(v3(B0) + ub, B0.w + mk * (eos.dp_drho(rhok) * pacc + racc));
+    if (TVF) {
+      const float4 B2 = ab2[k];
+      ab2[k] = f4(v3(B2) + vb, B2.w);
+    }
+  }
 }
 
 // Sweep B: the r' cotangent through the density sums, then the kinematic
@@ -223,25 +393,41 @@ __global__ __launch_bounds__(kAB) void k_adj_density(AdjArgs aa, const float4* _
   if (aa.bb_out) store_vec<DIM>(aa.bb_out, i, aa.c_tvf * vbt);
 }
 
-template <int DIM, int KIND, bool TVF>
+template <int DIM, int KIND, bool TVF, bool WALLS>
 int launch_adjoint(sph_ws* ws, AdjArgs& a, bool records_ready, hipStream_t st) {
   const int n = a.s.n;
   dim3 grid(xcd_grid(cdivl((int64_t)n * tpp_of(DIM), kAB), a.s.remap));
-  if (!records_ready)
+  if (!records_ready) {
     SPH_LAUNCH(ws, "density", st, (k_density<DIM, KIND, 0><<<grid, kAB, 0, st>>>(a.s)));
+    if (WALLS) SPH_LAUNCH(ws, "walls", st, (k_walls<DIM, KIND, 0><<<grid, kAB, 0, st>>>(a.s)));
+  }
   SPH_LAUNCH(ws, "adj_gather", st,
              (k_adj_gather<DIM, TVF><<<(unsigned)cdivl(n, kAB), kAB, 0, st>>>(a)));
   SPH_LAUNCH(ws, "adj_forces", st,
-             (k_adj_forces<DIM, KIND, TVF><<<grid, kAB, 0, st>>>(a, ws->AB0, ws->AB1, ws->AB2)));
+             (k_adj_forces<DIM, KIND, TVF, WALLS><<<grid, kAB, 0, st>>>(a, ws->AB0, ws->AB1,
+                                                                        ws->AB2)));
+  if (WALLS) {
+    // the delta-SPH records hold the per-wall factors (no delta-SPH here)
+    SPH_LAUNCH(ws, "adj_walls_prep", st,
+               (k_adj_walls_prep<DIM, KIND, TVF><<<grid, kAB, 0, st>>>(
+                   a, ws->AB0, ws->AB2, ws->DL0, ws->DL1, ws->DL2, ws->DG0)));
+    SPH_LAUNCH(ws, "adj_walls", st,
+               (k_adj_walls<DIM, KIND, TVF><<<grid, kAB, 0, st>>>(
+                   a, ws->AB0, ws->AB1, ws->AB2, ws->DL0, ws->DL1, ws->DL2, ws->DG0)));
+  }
   SPH_LAUNCH(ws, "adj_density", st,
              (k_adj_density<DIM, KIND, TVF><<<grid, kAB, 0, st>>>(a, ws->AB0, ws->AB1, ws->AB2)));
   return SPH_OK;
 }
 
 template <int DIM, int KIND>
-int launch_adjoint_tvf(sph_ws* ws, AdjArgs& a, bool tvf, bool records_ready, hipStream_t st) {
-  return tvf ? launch_adjoint<DIM, KIND, true>(ws, a, records_ready, st)
-             : launch_adjoint<DIM, KIND, false>(ws, a, records_ready, st);
+int launch_adjoint_tvf(sph_ws* ws, AdjArgs& a, bool tvf, bool walls, bool records_ready,
+                       hipStream_t st) {
+  if (walls)
+    return tvf ? launch_adjoint<DIM, KIND, true, true>(ws, a, records_ready, st)
+               : launch_adjoint<DIM, KIND, false, true>(ws, a, records_ready, st);
+  return tvf ? launch_adjoint<DIM, KIND, true, false>(ws, a, records_ready, st)
+             : launch_adjoint<DIM, KIND, false, false>(ws, a, records_ready, st);
 }
 
 }  // namespace sph
@@ -263,10 +449,11 @@ extern "C" int sph_step_vjp(sph_ws* ws, int bank, const sph_solver_params* prm, 
     set_error("sph_step_vjp: the r, u and a cotangent arrays are required");
     return SPH_ERR_ARG;
   }
-  if (prm->solver != 0 || prm->density_evolution || prm->bc_trick || prm->heat_conduction ||
-      prm->artificial_alpha != 0.0f) {
-    set_error("sph_step_vjp: only the standard SPH solver with density summation, no walls, "
-              "no heat and no artificial viscosity is differentiated");
+  if (prm->solver != 0 || prm->density_evolution || prm->heat_conduction ||
+      prm->artificial_alpha != 0.0f || (prm->bc_trick && prm->free_slip)) {
+    set_error("sph_step_vjp: only the standard SPH solver with density summation, no heat, "
+              "no artificial viscosity and (with the wall boundary condition) no-slip walls "
+              "is differentiated");
     return SPH_ERR_ARG;
   }
   if (prm->kernel != SPH_KERNEL_QSK && prm->kernel != SPH_KERNEL_WC2K) {
@@ -313,11 +500,12 @@ extern "C" int sph_step_vjp(sph_ws* ws, int bank, const sph_solver_params* prm, 
                  k_gather_state<3><<<g1, kAB, 0, st>>>(a.s, in.r, in.u, in.v, nullptr, nullptr,
                                                        nullptr));
   }
+  const bool walls = prm->bc_trick != 0;
   if (prm->dim == 2)
     return prm->kernel == SPH_KERNEL_QSK
-               ? launch_adjoint_tvf<2, SPH_KERNEL_QSK>(ws, a, tvf, ready, st)
-               : launch_adjoint_tvf<2, SPH_KERNEL_WC2K>(ws, a, tvf, ready, st);
+               ? launch_adjoint_tvf<2, SPH_KERNEL_QSK>(ws, a, tvf, walls, ready, st)
+               : launch_adjoint_tvf<2, SPH_KERNEL_WC2K>(ws, a, tvf, walls, ready, st);
   return prm->kernel == SPH_KERNEL_QSK
-             ? launch_adjoint_tvf<3, SPH_KERNEL_QSK>(ws, a, tvf, ready, st)
-             : launch_adjoint_tvf<3, SPH_KERNEL_WC2K>(ws, a, tvf, ready, st);
+             ? launch_adjoint_tvf<3, SPH_KERNEL_QSK>(ws, a, tvf, walls, ready, st)
+             : launch_adjoint_tvf<3, SPH_KERNEL_WC2K>(ws, a, tvf, walls, ready, st);
 }

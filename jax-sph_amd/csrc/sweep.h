@@ -471,6 +471,79 @@ __device__ __forceinline__ void density_run(const SweepArgs& a, const int blk) {
 
 SPH_SWEEP_KERNELS(k_density, density_run, (int DIM, int KIND, int MODE), (DIM, KIND, MODE), 8)
 
+// ------------------------------------------------------------- walls
+// MODE 0: generalized wall BC of the SPH solver (solver.py:431-530)
+// MODE 1: heat BC of the Riemann solver (solver.py:554-565)
+// followed by T += dt * dTdt when heat conduction is on (solver.py:834).
+template <int DIM, int KIND, int MODE>
+__device__ __forceinline__ void walls_run(const SweepArgs& a, const int blk) {
+  constexpr int TPP = tpp_of(DIM);
+  const int tid = blk * blockDim.x + threadIdx.x;
+  const int s = tid / TPP, h = tid % TPP;
+  if (s >= a.n) return;  // whole lane groups leave together
+  const sph_solver_params& p = a.prm;
+  const KernelEval<KIND, DIM> K{p.kernel, p.sigma, p.one_over_h};
+  const Eos eos = make_eos(p);
+  const float4 Pi = a.R.P[s], Ui = a.Ud[s], Vi = a.R.V0[s], Qi = a.Qd[s];
+  const int tag_i = (int)Qi.z;
+  const int i = a.perm[s];
+  V3 u = v3(Ui), v = v3(Vi);
+  float pr = Qi.x, T = Qi.w;
+  if (is_wall(tag_i)) {
+    float ws = 0.f, sp = 0.f, sT = 0.f;
+    V3 su = {0.f, 0.f, 0.f}, sv = {0.f, 0.f, 0.f}, srr = {0.f, 0.f, 0.f};
+    for_each_pair(a, s, h, [&](const int j) {
+      const float4 Qj = a.Qd[j];
+      if ((int)Qj.z != kFluid) return;
+      const float4 Pj = a.R.P[j];
+      const V3 dr = min_image<DIM>(Pi, Pj, a.geo);
+      const float d2 = dist2<DIM>(dr.x, dr.y, dr.z);
+      if (!in_cutoff<DIM>(a, d2, Pi, Pj)) return;
+      const float d = sqrt_fast(d2);
+      const float w = K.w(d * p.one_over_h);
+      ws += w;
+      if (MODE == 0) {
+        const float4 Uj = a.Ud[j], Vj = a.R.V0[j];
+        su = su + w * v3(Uj);
+        sv = sv + w * v3(Vj);
+        sp += w * Qj.x;
+        srr = srr + (Uj.w * w) * dr;
+      }
+      if (p.heat_conduction) sT += w * Qj.w;
+    });
+    ws = lane_sum<TPP>(ws);
+    if (MODE == 0) {
+      su = lane_sum<TPP>(su);
+      sv = lane_sum<TPP>(sv);
+      sp = lane_sum<TPP>(sp);
+      srr = lane_sum<TPP>(srr);
+    }
+    if (p.heat_conduction) sT = lane_sum<TPP>(sT);
+    const float inv = 1.0f / (ws + kEps);
+    if (MODE == 0) {
+      V3 uw = inv * su, vw = inv * sv;
+      if (p.free_slip) {
+        const V3 n = (-1.0f) * v3(a.R.NW[s]);
+        uw = dot<DIM>(uw, n) * n;
+        vw = dot<DIM>(vw, n) * n;
+      }
+      u = 2.0f * u - uw;
+      v = 2.0f * v - vw;
+      const V3 g = g_ext_of<DIM>(a, i);
+      pr = (sp + dot<DIM>(g, srr)) * inv;
+    }
+    if (p.heat_conduction && (tag_i == kSolidWall || tag_i == kMovingWall)) T = sT * inv;
+  }
+  if (h != 0) return;
+  const float rho = MODE == 0 ? eos.rho_fn(pr) : Ui.w;
+  if (p.heat_conduction) T = T + p.dt * a.in.dTdt[i];
+  a.R.U2[s] = f4(u, rho);
+  a.R.V2[s] = f4(v, Vi.w);
+  a.R.Q2[s] = make_float4(pr, Qi.y, Qi.z, T);
+  a.R.W2[s] = f4(v - u, pr);
+}
+SPH_SWEEP_KERNELS(k_walls, walls_run, (int DIM, int KIND, int MODE), (DIM, KIND, MODE), 8)
+
 
 // ------------------------------------------------------------- sweep plans
 // The sweeps of one forward evaluation as fixed by the solver parameters

@@ -16,15 +16,16 @@ positions.  :func:`rollout` wraps an ``n_steps`` rollout in a
 Memory is O(n_steps / k + k) states instead of O(n_steps).
 
 Differentiated configuration (other settings raise ``NotImplementedError``):
-standard SPH (``solver="SPH"``), density summation, no wall boundary
-treatment (``is_bc_trick``), no heat conduction, no artificial viscosity,
-Quintic or Wendland-C2 kernel, a constant body force, with or without the
-transport-velocity formulation (any ``tvf``: the stress ``A = rho u (v - u)``
-and the background-pressure acceleration, solver.py:199-256).  Wall particles
-may be present, held by the case's boundary hook (``bc_fn``, a
-``case_setup.CaseBC``: the lid-driven cavity of the reference's gradient
-check, notebooks/iclr24_grads.ipynb).  Gradients flow to the initial velocity
-and positions.
+standard SPH (``solver="SPH"``), density summation, no heat conduction, no
+artificial viscosity, Quintic or Wendland-C2 kernel, a constant body force,
+with or without the transport-velocity formulation (any ``tvf``: the stress
+``A = rho u (v - u)`` and the background-pressure acceleration,
+solver.py:199-256), and with or without the generalized wall boundary
+condition for no-slip walls (``is_bc_trick``, solver.py:431-530).  Wall
+particles may be held by the case's boundary hook (``bc_fn``, a
+``case_setup.CaseBC``: the lid-driven cavity as the reference's case file
+runs it, and as its gradient check runs it, notebooks/iclr24_grads.ipynb).
+Gradients flow to the initial velocity and positions.
 """
 from __future__ import annotations
 
@@ -50,8 +51,8 @@ def _check(solver: WCSPH, tvf: float, bc_fn=None):
         why.append("solver must be 'SPH'")
     if solver.is_rho_evol:
         why.append("density evolution must be summation")
-    if solver.is_bc_trick:
-        why.append("walls are not differentiated")
+    if solver.is_bc_trick and solver.is_free_slip:
+        why.append("free-slip walls are not differentiated")
     if solver.is_heat_conduction:
         why.append("heat conduction is not differentiated")
     if solver.artificial_alpha != 0.0:

@@ -233,6 +233,86 @@ def test_reference_gradient_check_configuration(case):
         assert abs(ad - fd) <= 1e-3 * scale, (case, trial, ad, fd, scale)
 
 
+@pytest.mark.parametrize("tvf,n_steps,k", [(0.0, 1, 1), (0.0, 4, 2), (1.0, 3, 3)])
+def test_wall_bc_rollout_gradient_matches_oracle_fd(tvf, n_steps, k):
+    """The lid-driven cavity as its case file runs it: the generalized wall
+    boundary condition (is_bc_trick, solver.py:431-530, no-slip walls) sets
+    the walls' velocity, pressure and density from Shepard averages of their
+    fluid neighbours before every force evaluation, and the case hook holds
+    the walls.  dx = 0.05, viscosity 0.1, 10 warm-up steps, then the gradient
+    of 0.5 sum(u^2) + <w, r> after n steps in the fluid's initial velocity and
+    in all positions, against float64 oracle central differences (eps = 1e-6):
+    along the gradient to 1e-3 relative, along random directions to 1e-3 of
+    sum |g_i d_i|.  The same differences taken without the wall boundary
+    condition differ by more than 1%, ten times the tolerance (from two steps
+    on: after one step u' and r' depend on the forces of the state before), so
+    the wall terms of the transpose are exercised.  (Walls' initial velocities are the hook's
+    constants: their cotangent is zero, and they are not perturbed.)"""
+    ov = {"case": {"dx": 0.05, "viscosity": 0.1}, "solver": {"t_end": 1.0, "tvf": tvf}}
+    cfg, box, state, g, bc, eos, disp, shift, solver = G.setup("ldc", ov)
+    assert solver.is_bc_trick and not solver.is_free_slip and bc.kind == 1
+    dt = cfg.solver.dt
+    cut = solver._kernel_fn.cutoff
+    warm = partition.neighbor_list(disp, box, r_cutoff=cut, mask_self=False,
+                                   format=partition.Sparse)
+    nb = warm.allocate(state["r"])
+    adv = integrator.si_euler(tvf, solver.forward_wrapper(), shift, bc, None)
+    st = state
+    for _ in range(10):
+        st, nb = adv(dt, st, nb)
+    fn = partition.neighbor_list(disp, box, r_cutoff=cut, capacity_multiplier=1.25,
+                                 mask_self=False, format=partition.Sparse, skin=0.0)
+    n = st["r"].shape[0]
+    gen = torch.Generator().manual_seed(5)
+    w = torch.randn(n, 2, generator=gen).cuda().double()
+    u0 = st["u"].clone().requires_grad_(True)
+    r0 = st["r"].clone().requires_grad_(True)
+    s_in = dict(st)
+    s_in["u"], s_in["r"] = u0, r0
+    u, r, _ = adjoint.rollout(s_in, solver, fn, shift, dt, n_steps, tvf=tvf,
+                              checkpoint_every=k, bc_fn=bc)
+    (0.5 * (u.double() ** 2).sum() + (w * r.double()).sum()).backward()
+    # the graph-replayed rollout (its transposes read the records its forward
+    # left, after the wall boundary condition) gives the same bits
+    gro = adjoint.GraphRollout(st, solver, fn, shift, dt, n_steps, tvf=tvf,
+                               checkpoint_every=k, bc_fn=bc)
+    u2 = st["u"].clone().requires_grad_(True)
+    r2 = st["r"].clone().requires_grad_(True)
+    ug, rg = gro(u2, r2)
+    (0.5 * (ug.double() ** 2).sum() + (w * rg.double()).sum()).backward()
+    assert torch.equal(ug, u) and torch.equal(rg, r)
+    assert torch.equal(u2.grad, u0.grad) and torch.equal(r2.grad, r0.grad)
+    fluid = (st["tag"] == 0).cpu().numpy()[:, None]
+    gu = u0.grad.double().cpu().numpy() * fluid
+    gr = r0.grad.double().cpu().numpy()
+    assert np.isfinite(gu).all() and np.isfinite(gr).all()
+    base = G.to_np(st)
+    wn = w.cpu().numpy()
+    nobc = G.setup("ldc", {**ov, "solver": {**ov["solver"], "is_bc_trick": False}})[-1]
+
+    def L(eps, du, dr, sv=solver):
+        s = G.to_f64(dict(base))
+        s["u"] = s["u"] + eps * du
+        s["v"] = s["u"].copy()   # the rollout starts from v = u (v' is recomputed)
+        s["r"] = np.mod(s["r"] + eps * dr, box)
+        out = _oracle_rollout(s, sv, box, dt, n_steps, tvf=tvf, bc_fn=bc)
+        return 0.5 * float((out["u"] ** 2).sum()) + float((wn * out["r"]).sum())
+
+    rng = np.random.default_rng(3)
+    eps = 1e-6
+    norm = np.sqrt((gu ** 2).sum() + (gr ** 2).sum())
+    dirs = [(gu / norm, gr / norm)] + [(rng.standard_normal(gu.shape) * fluid,
+                                        rng.standard_normal(gr.shape)) for _ in range(2)]
+    for trial, (du, dr) in enumerate(dirs):
+        fd = (L(eps, du, dr) - L(-eps, du, dr)) / (2 * eps)
+        ad = float((gu * du).sum() + (gr * dr).sum())
+        scale = abs(fd) if trial == 0 else float(np.abs(gu * du).sum() + np.abs(gr * dr).sum())
+        assert abs(ad - fd) <= 1e-3 * scale, (trial, ad, fd, scale)
+        if trial == 0 and n_steps >= 2:   # one step: u', r' do not see the new forces
+            fd_nobc = (L(eps, du, dr, nobc) - L(-eps, du, dr, nobc)) / (2 * eps)
+            assert abs(fd_nobc - fd) > 0.01 * abs(fd), (fd, fd_nobc)
+
+
 def test_checkpoint_interval_does_not_change_the_gradient():
     cfg, box, st, solver, fn, shift, gen = _setup()
     n = st["r"].shape[0]
