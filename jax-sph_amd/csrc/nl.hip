@@ -568,7 +568,11 @@ struct BuildArgs {
 // build's registers allow.
 constexpr int kStage = 24;
 
-template <int DIM, int TPP, bool ORDERED>
+// BAND = false (skin lists): pairs are kept by the fast squared distance
+// alone.  A skin list must hold every pair within cutoff + 0.98 skin (the
+// most two particles can close before the next rebuild), so rounding at its
+// outer edge, cutoff + skin, cannot drop a pair that matters.
+template <int DIM, int TPP, bool ORDERED, bool BAND>
 __device__ __forceinline__ void test_range(const BuildArgs& a, int s, int h, const float4 ps,
                                            int sid, int lo, int hi, const float* img,
                                            int32_t* __restrict__ out0, int32_t* stage0,
@@ -592,7 +596,13 @@ __device__ __forceinline__ void test_range(const BuildArgs& a, int s, int h, con
         pg[u] = a.pos[bc];
       }
       unsigned got = 0u;  // bit u: candidate t0 + u is a pair
-      if (a.fast) {
+      if (!BAND) {
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+          const float4 pb = pg[u];
+          got |= dist2<DIM>(sx - pb.x, sy - pb.y, sz - pb.z) < a.lo ? 1u << u : 0u;
+        }
+      } else if (a.fast) {
         bool near = false;  // some candidate lies in the band around the cutoff
 #pragma unroll
         for (int u = 0; u < G; ++u) {
@@ -663,7 +673,7 @@ __device__ __forceinline__ void test_range(const BuildArgs& a, int s, int h, con
 // the receivers with d^2 < cutoff^2 and stores them in its own sub-list.
 // blk: the workgroup's block within this list's grid; it also indexes the
 // workgroup's partials.
-template <int DIM, int TPP, bool ORDERED>
+template <int DIM, int TPP, bool ORDERED, bool BAND>
 __device__ __forceinline__ void nl_build_body(const BuildArgs& a, const int blk) {
   __shared__ int32_t stage_buf[kStage][kBlock];
   if (a.gate && *a.gate == 0) return;  // uniform over the list's grid
@@ -677,7 +687,7 @@ __device__ __forceinline__ void nl_build_body(const BuildArgs& a, const int blk)
     int32_t* stage = &stage_buf[0][threadIdx.x - h];    // stage column of lane 0 of s
     float img[3] = {0.0f, 0.0f, 0.0f};
     if (!a.g.use_cells) {
-      test_range<DIM, TPP, ORDERED>(a, s, h, ps, sid, 0, a.n, img, out, stage, count);
+      test_range<DIM, TPP, ORDERED, BAND>(a, s, h, ps, sid, 0, a.n, img, out, stage, count);
     } else {
       int c[3];
       cell_coords<DIM>(ps, a.g, c);
@@ -697,7 +707,7 @@ __device__ __forceinline__ void nl_build_body(const BuildArgs& a, const int blk)
           const int row = (cz * ny + cy) * nx;
           if (c[0] >= 1 && c[0] + 1 < nx) {
             img[0] = 0.0f;
-            test_range<DIM, TPP, ORDERED>(a, s, h, ps, sid, a.cell_start[row + c[0] - 1],
+            test_range<DIM, TPP, ORDERED, BAND>(a, s, h, ps, sid, a.cell_start[row + c[0] - 1],
                                           a.cell_start[row + c[0] + 2], img, out, stage, count);
           } else {
             for (int dx = -1; dx <= 1; ++dx) {
@@ -705,7 +715,7 @@ __device__ __forceinline__ void nl_build_body(const BuildArgs& a, const int blk)
               img[0] = 0.0f;
               if (cx < 0) { cx += nx; img[0] = -a.box[0]; }
               else if (cx >= nx) { cx -= nx; img[0] = a.box[0]; }
-              test_range<DIM, TPP, ORDERED>(a, s, h, ps, sid, a.cell_start[row + cx],
+              test_range<DIM, TPP, ORDERED, BAND>(a, s, h, ps, sid, a.cell_start[row + cx],
                                             a.cell_start[row + cx + 1], img, out, stage, count);
             }
           }
@@ -753,9 +763,9 @@ __device__ __forceinline__ void nl_build_body(const BuildArgs& a, const int blk)
   }
 }
 
-template <int DIM, int TPP, bool ORDERED>
+template <int DIM, int TPP, bool ORDERED, bool BAND>
 __global__ __launch_bounds__(kBlock, ORDERED ? 1 : 6) void k_nl_build(BuildArgs a) {
-  nl_build_body<DIM, TPP, ORDERED>(a, logical_block(a.remap));
+  nl_build_body<DIM, TPP, ORDERED, BAND>(a, logical_block(a.remap));
 }
 
 __global__ void k_finalize(int64_t* __restrict__ info, int64_t max_occ) {
@@ -1046,14 +1056,14 @@ int build_pairs(sph_ws* ws, int bank, int64_t max_occupancy, int64_t* info, hipS
   const dim3 grid(xcd_grid(cdiv((int64_t)c.n * (c.dim == 2 ? T2 : T3), kBlock), a.remap));
   if (c.dim == 2) {
     if (c.ordered)
-      SPH_LAUNCH(ws, "nl_build", st, (k_nl_build<2, T2, true><<<grid, kBlock, 0, st>>>(a)));
+      SPH_LAUNCH(ws, "nl_build", st, (k_nl_build<2, T2, true, true><<<grid, kBlock, 0, st>>>(a)));
     else
-      SPH_LAUNCH(ws, "nl_build", st, (k_nl_build<2, T2, false><<<grid, kBlock, 0, st>>>(a)));
+      SPH_LAUNCH(ws, "nl_build", st, (k_nl_build<2, T2, false, true><<<grid, kBlock, 0, st>>>(a)));
   } else {
     if (c.ordered)
-      SPH_LAUNCH(ws, "nl_build", st, (k_nl_build<3, T3, true><<<grid, kBlock, 0, st>>>(a)));
+      SPH_LAUNCH(ws, "nl_build", st, (k_nl_build<3, T3, true, true><<<grid, kBlock, 0, st>>>(a)));
     else
-      SPH_LAUNCH(ws, "nl_build", st, (k_nl_build<3, T3, false><<<grid, kBlock, 0, st>>>(a)));
+      SPH_LAUNCH(ws, "nl_build", st, (k_nl_build<3, T3, false, true><<<grid, kBlock, 0, st>>>(a)));
   }
   // occupancy info[1], largest list info[3], and the overflow bit when max_occupancy >= 0
   SPH_LAUNCH(ws, "reduce_pairs", st,
@@ -1179,9 +1189,9 @@ int skin_build(sph_ws* ws, int64_t* info, hipStream_t st) {
   constexpr int T2 = tpp_of(2), T3 = tpp_of(3);
   const dim3 grid(xcd_grid(cdiv((int64_t)c.n * (c.dim == 2 ? T2 : T3), kBlock), a.remap));
   if (c.dim == 2)
-    SPH_LAUNCH(ws, "skin_build", st, (k_nl_build<2, T2, false><<<grid, kBlock, 0, st>>>(a)));
+    SPH_LAUNCH(ws, "skin_build", st, (k_nl_build<2, T2, false, false><<<grid, kBlock, 0, st>>>(a)));
   else
-    SPH_LAUNCH(ws, "skin_build", st, (k_nl_build<3, T3, false><<<grid, kBlock, 0, st>>>(a)));
+    SPH_LAUNCH(ws, "skin_build", st, (k_nl_build<3, T3, false, false><<<grid, kBlock, 0, st>>>(a)));
   // largest skin list of one particle -> info[7] (only on a rebuild)
   SPH_LAUNCH(ws, "reduce_skin", st,
              k_reduce_parts<<<1, kReduceThreads, 0, st>>>(ws->part_build, (int)grid.x, nullptr,
@@ -1320,7 +1330,7 @@ __global__ __launch_bounds__(kBlock, 6) void k_nl_build_ens(const BuildArgs* __r
                                                          const EnsBlock* __restrict__ map) {
   const EnsBlock m = map[blockIdx.x];
   if (m.sim < 0) return;
-  nl_build_body<DIM, TPP, false>(t[m.sim], m.blk);
+  nl_build_body<DIM, TPP, false, false>(t[m.sim], m.blk);
 }
 
 // one workgroup per member
