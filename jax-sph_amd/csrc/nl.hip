@@ -588,12 +588,22 @@ __device__ __forceinline__ void test_range(const BuildArgs& a, int s, int h, con
     unsigned mask = 0u;
     for (int t0 = 0; t0 < nb; t0 += G) {
       float4 pg[G];
+      if (BAND) {
 #pragma unroll
-      for (int u = 0; u < G; ++u) {
-        const int b = c0 + (t0 + u) * TPP;
-        const int bc = b < hi ? b : hi - 1;
-        __builtin_assume(bc >= 0);  // sorted slots: no sign extension in the address
-        pg[u] = a.pos[bc];
+        for (int u = 0; u < G; ++u) {
+          const int b = c0 + (t0 + u) * TPP;
+          const int bc = b < hi ? b : hi - 1;
+          __builtin_assume(bc >= 0);  // sorted slots: no sign extension in the address
+          pg[u] = a.pos[bc];
+        }
+      } else {
+        // skin builds read past the range into the padded tail (kSkinPosPad),
+        // from one address with constant offsets; those candidates are masked
+        // off below
+        __builtin_assume(c0 >= 0);
+        const float4* __restrict__ pb = a.pos + (c0 + t0 * TPP);
+#pragma unroll
+        for (int u = 0; u < G; ++u) pg[u] = pb[u * TPP];
       }
       unsigned got = 0u;  // bit u: candidate t0 + u is a pair
       if (!BAND) {

@@ -29,6 +29,11 @@ struct Bank {
 // scheduler enough independent gathers to hide memory latency.
 __host__ __device__ constexpr int tpp_of(int dim) { return dim == 3 ? 4 : 2; }
 
+// Spare entries after the skin lists' sorted build positions: the skin build
+// reads each batch of candidates from one address with constant offsets, up
+// to 31 TPP entries past a cell range, without clamping each one.
+constexpr int kSkinPosPad = 128;
+
 struct ProfEntry {
   const char* name;
   hipEvent_t a, b;

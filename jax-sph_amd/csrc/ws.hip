@@ -204,9 +204,9 @@ extern "C" int sph_skin_enable(sph_ws* ws, float skin, int32_t width) {
   A(S.b.perm, N);
   A(S.b.iperm, N);
   A(S.b.cell_start, ncell + 1);
-  A(S.b.pos, N);
   A(S.b.ell, ws->tpp * lw * npad);
   A(S.b.cnt, ws->tpp * npad);
+  A(S.b.pos, N + kSkinPosPad);  // last, so that its padding moves no other buffer
 #undef A
   if (rc == SPH_OK) {
     const int32_t one = 1;  // the first step builds the lists
