@@ -211,19 +211,22 @@ int sph_step(sph_ws *ws, int bank, const sph_solver_params *prm, float dt, float
 /* ---- adjoint ------------------------------------------------------------- */
 /* Cotangents of the step's state, (N, dim), orig order: positions r, velocity
  * u, acceleration a = dudt, and for the transport-velocity step the transport
- * velocity v and its acceleration b = dvdt.  v and b of bar_next may be NULL
- * (zero); b of bar may be NULL (not written). */
+ * velocity v and its acceleration b = dvdt; with density continuity also the
+ * density rho, (N,).  v, b and rho of bar_next may be NULL (zero); b and rho
+ * of bar may be NULL (not written). */
 typedef struct {
-  const float *r, *u, *a, *v, *b;
+  const float *r, *u, *a, *v, *b, *rho;
 } sph_cotangent_in;
 typedef struct {
-  float *r, *u, *a, *b;
+  float *r, *u, *a, *b, *rho;
 } sph_cotangent_out;
 /* Vector-Jacobian product of one si_euler step (integrator.py:22-56), the
  * transpose that jax.grad builds for the reference's step.  Standard SPH,
- * density summation, no heat/artificial viscosity, the generalized wall
- * boundary condition (bc_trick, solver.py:431-530) for no-slip walls only,
- * QSK or WC2K; with prm->tvf_stress the transport-velocity step
+ * density summation, or continuity (solver.py:24-30) without the wall
+ * boundary condition and renormalisation, no heat/artificial viscosity, the
+ * generalized wall boundary condition (bc_trick, solver.py:431-530) for
+ * no-slip walls with summation, QSK or WC2K; with prm->tvf_stress the
+ * transport-velocity step
  * (solver.py:199-256: the A = rho u (v - u) stress and the background-pressure
  * dvdt), whose v' = u' + tvf_coef * dvdt (tvf_coef = tvf dt / 2, 0 without
  * tvf).  `bc` (nullable) is the step's case boundary hook as in sph_step:
@@ -231,7 +234,9 @@ typedef struct {
  * particles it holds carry only a position cotangent.
  * `next` is state n+1 (r, u, v with tvf, rho, p, mass, eta, tag) and `bank`
  * must hold the pairs of next->r; given the cotangents of state n+1 writes
- * those of state n (bar may alias nothing in bar_next).  records_ready != 0
+ * those of state n (bar may alias nothing in bar_next).  With density
+ * continuity records_ready must be set (the density the step advanced is not
+ * in `next`).  records_ready != 0
  * promises that the last kernels on this workspace were the sph_step that
  * produced `next` into `bank` (its sorted records are reused instead of
  * gathered again). */

@@ -64,10 +64,12 @@ struct AdjArgs {
   const float* ab_in;        // (N, d) cotangent of a' = dudt'
   const float* vb_in;        // (N, d) cotangent of v', or null (zero)
   const float* bb_in;        // (N, d) cotangent of b' = dvdt', or null (zero)
+  const float* rhob_in;      // (N) continuity: cotangent of rho', or null (zero)
   float* rb_out;             // (N, d) cotangents of state n
   float* ub_out;
   float* ab_out;
   float* bb_out;             // with tvf: cotangent of b = dvdt (null: not written)
+  float* rhob_out;           // (N) continuity: cotangent of rho (null: not written)
   float4* la;                // sorted cotangent of a'
   float4* lb;                // sorted cotangent of b' (tvf)
   float dt;
@@ -101,7 +103,9 @@ __global__ void k_adj_gather(AdjArgs a) {
 // of F and the direct r' cotangent of F and G, per sorted particle k.
 // WALLS: the records after the generalized wall BC (U2, V2, Q2), and for wall
 // particles the raw cotangent of rho^ in ab0.w (the walls sweeps consume it).
-template <int DIM, int KIND, bool TVF, bool WALLS>
+// CONT (density continuity): ab0.w = B, the whole cotangent of rho', the next
+// step's included.
+template <int DIM, int KIND, bool TVF, bool WALLS, bool CONT>
 __global__ __launch_bounds__(kAB) void k_adj_forces(AdjArgs aa, float4* __restrict__ ab0,
                                                     float4* __restrict__ ab1,
                                                     float4* __restrict__ ab2) {
@@ -192,8 +196,12 @@ __global__ __launch_bounds__(kAB) void k_adj_forces(AdjArgs aa, float4* __restri
   if (TVF) vb = lane_sum<TPP>(vb);
   if (h != 0) return;
   const bool fluid = (int)Qk.z == kFluid;
-  // rho = m sum W only for fluid (solver.py:796); a wall's rho^ comes from its p^
-  ab0[k] = f4(ub, fluid ? mk * rhob : (WALLS ? rhob : 0.0f));
+  if (CONT) {
+    ab0[k] = f4(ub, rhob + (aa.rhob_in ? aa.rhob_in[a.perm[k]] : 0.0f));
+  } else {
+    // rho = m sum W only for fluid (solver.py:796); a wall's rho^ comes from its p^
+    ab0[k] = f4(ub, fluid ? mk * rhob : (WALLS ? rhob : 0.0f));
+  }
   ab1[k] = f4(rb, 0.0f);
   if (TVF) ab2[k] = f4(vb, 0.0f);
 }
@@ -345,6 +353,82 @@ __global__ __launch_bounds__(kAB) void k_adj_walls(AdjArgs aa, float4* __restric
   }
 }
 
+// Sweep C (density continuity, solver.py:24-30, 750-756): the transpose of
+//   rho'_k = rho_k + dt rho_k sum_j V_j (u'_k - u'_j) . g_kj,
+// V_j = m_j / rho_j, g_kj = W'(d) r_kj / (d + eps), with rho the step's input
+// density and u' the kicked velocity (the U0 records).  With B = ab0.w the
+// cotangent of rho', c_kj = B_k rho_k V_j + B_j rho_j V_k is symmetric, so
+//   ubar'_k += dt sum_j c_kj g_kj,
+//   rbar'_k += dt sum_j c_kj H_kj (u'_k - u'_j),  H = d g / d r_k,
+//   rhobar_k = B_k (1 + dt sum_j V_j (u'_k - u'_j) . g_kj)
+//              - dt m_k / rho_k^2 sum_j B_j rho_j (u'_k - u'_j) . g_kj,
+// then the kinematic transpose as in sweep B.
+template <int DIM, int KIND, bool TVF>
+__global__ __launch_bounds__(kAB) void k_adj_continuity(AdjArgs aa,
+                                                        const float4* __restrict__ ab0,
+                                                        const float4* __restrict__ ab1,
+                                                        const float4* __restrict__ ab2) {
+  constexpr int TPP = tpp_of(DIM);
+  const SweepArgs& a = aa.s;
+  const int tid = logical_block(a.remap) * blockDim.x + threadIdx.x;
+  const int k = tid / TPP, h = tid % TPP;
+  if (k >= a.n) return;
+  const sph_solver_params& p = a.prm;
+  const KernelEval<KIND, DIM> K{p.kernel, p.sigma, p.one_over_h};
+  const float4 Pk = a.R.P[k], Uk = a.R.U0[k];
+  const float mk = Pk.w, rhok = Uk.w, Vk = mk / rhok;
+  const V3 uk = v3(Uk);
+  const float Bk = ab0[k].w;
+  float accD = 0.f, accR = 0.f;
+  V3 ub = {0.f, 0.f, 0.f}, rb = {0.f, 0.f, 0.f};
+  for_each_pair(a, k, h, [&](const int j) {
+    const float4 Pj = a.R.P[j];
+    const V3 dr = min_image<DIM>(Pk, Pj, a.geo);
+    const float d2 = dist2<DIM>(dr.x, dr.y, dr.z);
+    if (d2 == 0.0f) return;
+    if (!in_cutoff<DIM>(a, d2, Pk, Pj)) return;
+    const float4 Uj = a.R.U0[j];
+    const float inv = rsq_fast(d2);
+    const float d = d2 * inv;
+    const float q = d * p.one_over_h;
+    const float w1 = K.dw(q), w2 = K.d2w(q);
+    const float A0 = w1 * inv;                // W'(d) / (d + eps)
+    const float Ad0 = (w2 - w1 * inv) * inv;  // d/dd of the above
+    const V3 du = uk - v3(Uj);
+    const float Vj = Pj.w * rcp_fast(Uj.w);
+    const float Bj = ab0[j].w;
+    const float rdu = dot<DIM>(dr, du);
+    const float dg = A0 * rdu;                // (u'_k - u'_j) . g_kj
+    const float c = Bk * rhok * Vj + Bj * Uj.w * Vk;
+    accD += Vj * dg;
+    accR += Bj * Uj.w * dg;
+    ub = ub + (c * A0) * dr;
+    rb = rb + c * (A0 * du + (Ad0 * rdu * inv) * dr);
+  });
+  accD = lane_sum<TPP>(accD);
+  accR = lane_sum<TPP>(accR);
+  ub = lane_sum<TPP>(ub);
+  rb = lane_sum<TPP>(rb);
+  if (h != 0) return;
+  const float dt = aa.dt;
+  const int i = a.perm[k];
+  const float4 A0r = ab0[k], A1 = ab1[k];
+  const V3 rbt = load_vec<DIM>(aa.rb_in, i) + v3(A1) + dt * rb;
+  V3 vbt = dt * rbt;  // r' = r + dt v'
+  if (aa.vb_in) vbt = vbt + load_vec<DIM>(aa.vb_in, i);
+  if (TVF) vbt = vbt + v3(ab2[k]);
+  V3 ubt = load_vec<DIM>(aa.ub_in, i) + v3(A0r) + dt * ub + vbt;  // v' = u' + c b
+  if (bc_holds(aa.bc_kind, a.in.tag[i])) {
+    ubt = V3{0.f, 0.f, 0.f};
+    vbt = ubt;
+  }
+  store_vec<DIM>(aa.rb_out, i, rbt);
+  store_vec<DIM>(aa.ub_out, i, ubt);
+  store_vec<DIM>(aa.ab_out, i, aa.dt * ubt);  // u' = u + dt a
+  if (aa.bb_out) store_vec<DIM>(aa.bb_out, i, aa.c_tvf * vbt);
+  if (aa.rhob_out) aa.rhob_out[i] = Bk * (1.0f + dt * accD) - dt * (mk / (rhok * rhok)) * accR;
+}
+
 // Sweep B: the r' cotangent through the density sums, then the kinematic
 // transpose of the kick, transport velocity and drift, written in original
 // order.
@@ -393,19 +477,26 @@ __global__ __launch_bounds__(kAB) void k_adj_density(AdjArgs aa, const float4* _
   if (aa.bb_out) store_vec<DIM>(aa.bb_out, i, aa.c_tvf * vbt);
 }
 
-template <int DIM, int KIND, bool TVF, bool WALLS>
+template <int DIM, int KIND, bool TVF, bool WALLS, bool CONT>
 int launch_adjoint(sph_ws* ws, AdjArgs& a, bool records_ready, hipStream_t st) {
   const int n = a.s.n;
   dim3 grid(xcd_grid(cdivl((int64_t)n * tpp_of(DIM), kAB), a.s.remap));
   if (!records_ready) {
-    SPH_LAUNCH(ws, "density", st, (k_density<DIM, KIND, 0><<<grid, kAB, 0, st>>>(a.s)));
+    SPH_LAUNCH(ws, "density", st,
+               (k_density<DIM, KIND, CONT ? 1 : 0><<<grid, kAB, 0, st>>>(a.s)));
     if (WALLS) SPH_LAUNCH(ws, "walls", st, (k_walls<DIM, KIND, 0><<<grid, kAB, 0, st>>>(a.s)));
   }
   SPH_LAUNCH(ws, "adj_gather", st,
              (k_adj_gather<DIM, TVF><<<(unsigned)cdivl(n, kAB), kAB, 0, st>>>(a)));
   SPH_LAUNCH(ws, "adj_forces", st,
-             (k_adj_forces<DIM, KIND, TVF, WALLS><<<grid, kAB, 0, st>>>(a, ws->AB0, ws->AB1,
+             (k_adj_forces<DIM, KIND, TVF, WALLS, CONT><<<grid, kAB, 0, st>>>(
+                 a, ws->AB0, ws->AB1, ws->AB2)));
+  if (CONT) {
+    SPH_LAUNCH(ws, "adj_continuity", st,
+               (k_adj_continuity<DIM, KIND, TVF><<<grid, kAB, 0, st>>>(a, ws->AB0, ws->AB1,
                                                                         ws->AB2)));
+    return SPH_OK;
+  }
   if (WALLS) {
     // the delta-SPH records hold the per-wall factors (no delta-SPH here)
     SPH_LAUNCH(ws, "adj_walls_prep", st,
@@ -421,13 +512,16 @@ int launch_adjoint(sph_ws* ws, AdjArgs& a, bool records_ready, hipStream_t st) {
 }
 
 template <int DIM, int KIND>
-int launch_adjoint_tvf(sph_ws* ws, AdjArgs& a, bool tvf, bool walls, bool records_ready,
-                       hipStream_t st) {
+int launch_adjoint_tvf(sph_ws* ws, AdjArgs& a, bool tvf, bool walls, bool cont,
+                       bool records_ready, hipStream_t st) {
+  if (cont)
+    return tvf ? launch_adjoint<DIM, KIND, true, false, true>(ws, a, records_ready, st)
+               : launch_adjoint<DIM, KIND, false, false, true>(ws, a, records_ready, st);
   if (walls)
-    return tvf ? launch_adjoint<DIM, KIND, true, true>(ws, a, records_ready, st)
-               : launch_adjoint<DIM, KIND, false, true>(ws, a, records_ready, st);
-  return tvf ? launch_adjoint<DIM, KIND, true, false>(ws, a, records_ready, st)
-             : launch_adjoint<DIM, KIND, false, false>(ws, a, records_ready, st);
+    return tvf ? launch_adjoint<DIM, KIND, true, true, false>(ws, a, records_ready, st)
+               : launch_adjoint<DIM, KIND, false, true, false>(ws, a, records_ready, st);
+  return tvf ? launch_adjoint<DIM, KIND, true, false, false>(ws, a, records_ready, st)
+             : launch_adjoint<DIM, KIND, false, false, false>(ws, a, records_ready, st);
 }
 
 }  // namespace sph
@@ -449,11 +543,13 @@ extern "C" int sph_step_vjp(sph_ws* ws, int bank, const sph_solver_params* prm, 
     set_error("sph_step_vjp: the r, u and a cotangent arrays are required");
     return SPH_ERR_ARG;
   }
-  if (prm->solver != 0 || prm->density_evolution || prm->heat_conduction ||
-      prm->artificial_alpha != 0.0f || (prm->bc_trick && prm->free_slip)) {
-    set_error("sph_step_vjp: only the standard SPH solver with density summation, no heat, "
-              "no artificial viscosity and (with the wall boundary condition) no-slip walls "
-              "is differentiated");
+  if (prm->solver != 0 || prm->heat_conduction || prm->artificial_alpha != 0.0f ||
+      (prm->bc_trick && (prm->free_slip || prm->density_evolution)) ||
+      (prm->density_evolution && prm->rho_renorm)) {
+    set_error("sph_step_vjp: only the standard SPH solver with no heat and no artificial "
+              "viscosity is differentiated: density summation, with or without the wall "
+              "boundary condition for no-slip walls, or density continuity without that "
+              "condition and without renormalisation");
     return SPH_ERR_ARG;
   }
   if (prm->kernel != SPH_KERNEL_QSK && prm->kernel != SPH_KERNEL_WC2K) {
@@ -476,8 +572,9 @@ extern "C" int sph_step_vjp(sph_ws* ws, int bank, const sph_solver_params* prm, 
   AdjArgs a;
   a.s = make_args(ws, bank, prm, &in, &none);
   a.rb_in = bar_next->r; a.ub_in = bar_next->u; a.ab_in = bar_next->a;
-  a.vb_in = bar_next->v; a.bb_in = bar_next->b;
+  a.vb_in = bar_next->v; a.bb_in = bar_next->b; a.rhob_in = bar_next->rho;
   a.rb_out = bar->r; a.ub_out = bar->u; a.ab_out = bar->a; a.bb_out = bar->b;
+  a.rhob_out = bar->rho;
   a.la = ws->LA;
   a.lb = ws->LB;
   a.dt = dt;
@@ -489,6 +586,12 @@ extern "C" int sph_step_vjp(sph_ws* ws, int bank, const sph_solver_params* prm, 
   }
   const int n = a.s.n;
   const bool ready = records_ready != 0;
+  if (prm->density_evolution && !ready) {
+    // continuity advances the density of state n, which `next` does not hold
+    set_error("sph_step_vjp: density continuity needs the step's own records "
+              "(records_ready: call right after the sph_step that produced `next`)");
+    return SPH_ERR_ARG;
+  }
   dim3 g1((unsigned)cdivl(n, kAB));
   if (!ready) {
     if (prm->dim == 2)
@@ -500,12 +603,12 @@ extern "C" int sph_step_vjp(sph_ws* ws, int bank, const sph_solver_params* prm, 
                  k_gather_state<3><<<g1, kAB, 0, st>>>(a.s, in.r, in.u, in.v, nullptr, nullptr,
                                                        nullptr));
   }
-  const bool walls = prm->bc_trick != 0;
+  const bool walls = prm->bc_trick != 0, cont = prm->density_evolution != 0;
   if (prm->dim == 2)
     return prm->kernel == SPH_KERNEL_QSK
-               ? launch_adjoint_tvf<2, SPH_KERNEL_QSK>(ws, a, tvf, walls, ready, st)
-               : launch_adjoint_tvf<2, SPH_KERNEL_WC2K>(ws, a, tvf, walls, ready, st);
+               ? launch_adjoint_tvf<2, SPH_KERNEL_QSK>(ws, a, tvf, walls, cont, ready, st)
+               : launch_adjoint_tvf<2, SPH_KERNEL_WC2K>(ws, a, tvf, walls, cont, ready, st);
   return prm->kernel == SPH_KERNEL_QSK
-             ? launch_adjoint_tvf<3, SPH_KERNEL_QSK>(ws, a, tvf, walls, ready, st)
-             : launch_adjoint_tvf<3, SPH_KERNEL_WC2K>(ws, a, tvf, walls, ready, st);
+             ? launch_adjoint_tvf<3, SPH_KERNEL_QSK>(ws, a, tvf, walls, cont, ready, st)
+             : launch_adjoint_tvf<3, SPH_KERNEL_WC2K>(ws, a, tvf, walls, cont, ready, st);
 }

@@ -69,11 +69,12 @@ class BCSpec(Structure):
 
 class CotangentIn(Structure):
     _fields_ = [("r", c_void_p), ("u", c_void_p), ("a", c_void_p), ("v", c_void_p),
-                ("b", c_void_p)]
+                ("b", c_void_p), ("rho", c_void_p)]
 
 
 class CotangentOut(Structure):
-    _fields_ = [("r", c_void_p), ("u", c_void_p), ("a", c_void_p), ("b", c_void_p)]
+    _fields_ = [("r", c_void_p), ("u", c_void_p), ("a", c_void_p), ("b", c_void_p),
+                ("rho", c_void_p)]
 
 
 class EnsembleMember(Structure):

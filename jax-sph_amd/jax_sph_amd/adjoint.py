@@ -16,12 +16,14 @@ positions.  :func:`rollout` wraps an ``n_steps`` rollout in a
 Memory is O(n_steps / k + k) states instead of O(n_steps).
 
 Differentiated configuration (other settings raise ``NotImplementedError``):
-standard SPH (``solver="SPH"``), density summation, no heat conduction, no
-artificial viscosity, Quintic or Wendland-C2 kernel, a constant body force,
-with or without the transport-velocity formulation (any ``tvf``: the stress
-``A = rho u (v - u)`` and the background-pressure acceleration,
-solver.py:199-256), and with or without the generalized wall boundary
-condition for no-slip walls (``is_bc_trick``, solver.py:431-530).  Wall
+standard SPH (``solver="SPH"``), no heat conduction, no artificial viscosity,
+Quintic or Wendland-C2 kernel, a constant body force, with or without the
+transport-velocity formulation (any ``tvf``: the stress ``A = rho u (v - u)``
+and the background-pressure acceleration, solver.py:199-256); density
+summation with or without the generalized wall boundary condition for
+no-slip walls (``is_bc_trick``, solver.py:431-530), or density continuity
+(``density_evolution``, solver.py:24-30; the density is then carried from
+step to step, and checkpointed) without that condition.  Wall
 particles may be held by the case's boundary hook (``bc_fn``, a
 ``case_setup.CaseBC``: the lid-driven cavity as the reference's case file
 runs it, and as its gradient check runs it, notebooks/iclr24_grads.ipynb).
@@ -49,8 +51,10 @@ def _check(solver: WCSPH, tvf: float, bc_fn=None):
         why.append("bc_fn must be a case boundary hook (case_setup.CaseBC)")
     if solver.solver != "SPH":
         why.append("solver must be 'SPH'")
-    if solver.is_rho_evol:
-        why.append("density evolution must be summation")
+    if solver.is_rho_evol and solver.is_bc_trick:
+        why.append("density continuity is differentiated without the wall boundary condition")
+    if solver.is_rho_evol and solver.is_rho_renorm:
+        why.append("density renormalisation is not differentiated")
     if solver.is_bc_trick and solver.is_free_slip:
         why.append("free-slip walls are not differentiated")
     if solver.is_heat_conduction:
@@ -70,16 +74,20 @@ def _tvf_coef(tvf: float, dt: float) -> np.float32:
     return _f(_f(tvf * 0.5) * _f(dt))
 
 
-def _cot_in(rb, ub, ab, bb=None):
-    """sph_cotangent_in of (r, u, a[, b]); the cotangent of v' is zero (the
-    next step recomputes v from u and dvdt)."""
-    return _lib.CotangentIn(rb.data_ptr(), ub.data_ptr(), ab.data_ptr(), None,
-                            bb.data_ptr() if bb is not None else None)
+def _ptr(t):
+    return t.data_ptr() if t is not None else None
 
 
-def _cot_out(rb, ub, ab, bb=None):
-    return _lib.CotangentOut(rb.data_ptr(), ub.data_ptr(), ab.data_ptr(),
-                             bb.data_ptr() if bb is not None else None)
+def _cot_in(c: Dict[str, torch.Tensor]):
+    """sph_cotangent_in of the carried cotangents c (keys of _carried); the
+    cotangent of v' is zero (the next step recomputes v from u and dvdt)."""
+    return _lib.CotangentIn(_ptr(c["r"]), _ptr(c["u"]), _ptr(c["dudt"]), None,
+                            _ptr(c.get("dvdt")), _ptr(c.get("rho")))
+
+
+def _cot_out(c: Dict[str, torch.Tensor]):
+    return _lib.CotangentOut(_ptr(c["r"]), _ptr(c["u"]), _ptr(c["dudt"]),
+                             _ptr(c.get("dvdt")), _ptr(c.get("rho")))
 
 
 class _Runner:
@@ -96,6 +104,7 @@ class _Runner:
         self.k = max(1, int(checkpoint_every))
         self.base = solver.state_in(base_state)
         self.tvf = float(tvf)
+        self.carried = _carried(self.tvf, solver.is_rho_evol)
         self.advance = si_euler(self.tvf, solver.forward_wrapper(), shift_fn, bc_fn, None)
         self.prm = solver.params(tvf_stress=self.tvf != 0.0)
         self.ctvf = float(_tvf_coef(self.tvf, self.dt))
@@ -141,10 +150,10 @@ class _Runner:
         return st, ckpts
 
     def backward(self, ckpts, rb, ub):
-        ab = torch.zeros_like(ub)
-        tvf = self.tvf != 0.0
-        bb = torch.zeros_like(ub) if tvf else None  # cotangent of dvdt
-        bufs = [torch.empty_like(ub) for _ in range(4 if tvf else 3)]
+        # cotangents of the carried state; the rollout's outputs are u and r
+        cur = {key: torch.zeros_like(ckpts[0][key]) for key in self.carried}
+        cur["r"], cur["u"] = rb, ub
+        spare = {key: torch.empty_like(v) for key, v in cur.items()}
         L = _lib.lib()
         for seg in reversed(range(len(ckpts))):
             start = seg * self.k
@@ -153,23 +162,28 @@ class _Runner:
             nb = self._list_at(st0["r"])
             _, _, states = self._steps(st0, nb, count, keep=True)
             for n in reversed(range(count)):
-                nxt = states[n]
-                # the exact pairs of the step's new positions
-                nbn = self._list_at(nxt["r"])
+                if self.solver.is_rho_evol:
+                    # the transpose needs the step's records: the continuity
+                    # update starts from the density of the state before, which
+                    # state n+1 does not hold, so the step runs again from there
+                    prev = states[n - 1] if n > 0 else st0
+                    nxt, nbn = self.advance(self.dt, prev, self._list_at(prev["r"]))
+                    ready = 1
+                else:
+                    nxt = states[n]
+                    # the exact pairs of the step's new positions
+                    nbn = self._list_at(nxt["r"])
+                    ready = 0
                 ws = nbn.workspace
                 cin = self.solver.c_state_in(nxt)
-                bar_in = _cot_in(rb, ub, ab, bb)
-                bar_out = _cot_out(*bufs)
+                bar_in = _cot_in(cur)
+                bar_out = _cot_out(spare)
                 _lib.check(L.sph_step_vjp(ws.handle, nbn.bank, ctypes.byref(self.prm), self.dt,
                                           self.ctvf, ctypes.byref(self.bc), ctypes.byref(cin),
-                                          ctypes.byref(bar_in), ctypes.byref(bar_out), 0,
+                                          ctypes.byref(bar_in), ctypes.byref(bar_out), ready,
                                           _lib.stream()), "sph_step_vjp")
-                old = [rb, ub, ab] + ([bb] if tvf else [])
-                rb, ub, ab = bufs[:3]
-                if tvf:
-                    bb = bufs[3]
-                bufs = old
-        return rb, ub
+                cur, spare = spare, cur
+        return cur["r"], cur["u"]
 
 
 class _Rollout(torch.autograd.Function):
@@ -191,12 +205,34 @@ class _Rollout(torch.autograd.Function):
 
 
 EVOLVING = ("r", "u", "v", "dudt", "dvdt", "rho", "p", "drhodt")
-# what one step of the differentiated configuration reads from the state it
+
+
+# What one step of the differentiated configuration reads from the state it
 # starts from: positions, velocity and the accelerations (v = u + c dvdt is
-# recomputed; so are the fluid density and pressure, by the summation and the
-# EOS).  Without tvf, c = 0 and dvdt is not read.
-def _carried(tvf: float) -> Tuple[str, ...]:
-    return ("r", "u", "dudt", "dvdt") if tvf != 0.0 else ("r", "u", "dudt")
+# recomputed; with density summation so are the fluid density and pressure,
+# by the summation and the EOS).  Without tvf, c = 0 and dvdt is not read;
+# with density continuity the step advances the density it is given.
+def _carried(tvf: float, continuity: bool = False) -> Tuple[str, ...]:
+    keys = ("r", "u", "dudt", "dvdt") if tvf != 0.0 else ("r", "u", "dudt")
+    return keys + ("rho",) if continuity else keys
+
+
+class _Flat:
+    """The carried fields of a state in one flat buffer, so that a checkpoint
+    is one copy; `views` maps each field to its (N, d) or (N,) part."""
+
+    def __init__(self, fields: Dict[str, torch.Tensor], keys: Tuple[str, ...]):
+        self.keys = keys
+        self.shapes = [tuple(fields[k].shape) for k in keys]
+        self.buf = torch.cat([fields[k].reshape(-1) for k in keys]).contiguous()
+
+    def views(self, buf: torch.Tensor) -> Dict[str, torch.Tensor]:
+        out, off = {}, 0
+        for key, shape in zip(self.keys, self.shapes):
+            n = int(np.prod(shape))
+            out[key] = buf[off:off + n].view(shape)
+            off += n
+        return out
 
 
 class GraphRollout:
@@ -247,7 +283,8 @@ class GraphRollout:
         self.prm = solver.params(tvf_stress=self.tvf != 0.0)
         self.advance = si_euler(self.tvf, solver.forward_wrapper(), shift_fn, bc_fn, None)
         self.bc = _bc_spec(bc_fn)
-        carried = _carried(self.tvf)
+        carried = _carried(self.tvf, solver.is_rho_evol)
+        self.carried = carried
         base = solver.state_in(state)
         n_seg = self.n_steps // self.k
         # Each step keeps its list in a workspace of its own, so the pair
@@ -275,22 +312,21 @@ class GraphRollout:
         # every segment keeps its steps in a set of its own: no recompute
         self.keep_all = self.n_sets == n_seg
         self.sets += [[alloc() for _ in range(self.k)] for _ in range(self.n_sets - 1)]
-        # carried fields live in one (3, N, d) buffer: a checkpoint is one copy
-        self.base_buf = torch.stack([base[key] for key in carried]).contiguous()
+        # carried fields live in one flat buffer: a checkpoint is one copy
+        self.flat = _Flat(base, carried)
+        self.base_buf = self.flat.buf
         self.S_buf = self.base_buf.clone()
         self.S = {key: v.clone() for key, v in base.items()}   # forward segment state
-        for j, key in enumerate(carried):
-            self.S[key] = self.S_buf[j]
+        self.S.update(self.flat.views(self.S_buf))
         self.C_bufs, self.C = [], []
         for _ in range(self.n_sets):
             cb = self.base_buf.clone()
             c = {key: v.clone() for key, v in base.items()}
-            for j, key in enumerate(carried):
-                c[key] = cb[j]
+            c.update(self.flat.views(cb))
             self.C_bufs.append(cb)
             self.C.append(c)
-        # r, u, a (and dvdt) cotangents
-        self.bar = [torch.zeros_like(base["u"]) for _ in range(len(carried))]
+        # cotangents of the carried fields
+        self.bar = {key: torch.zeros_like(base[key]) for key in carried}
         self.bits = torch.zeros((), dtype=torch.int64, device=base["u"].device)
         self.sides = [torch.cuda.Stream() for _ in range(max(1, int(side_streams)))]
         self.generation = 0
@@ -321,11 +357,11 @@ class GraphRollout:
                 self.bits.copy_(torch.bitwise_or(self.bits, bits))
             gV = torch.cuda.CUDAGraph()
             with torch.cuda.graph(gV):
-                cur = list(self.bar)
+                cur = dict(self.bar)
                 for st_j, nb_j in reversed(steps):
-                    out = [torch.empty_like(cur[0]) for _ in range(len(cur))]
-                    bar_in = _cot_in(*cur)
-                    bar_out = _cot_out(*out)
+                    out = {key: torch.empty_like(v) for key, v in cur.items()}
+                    bar_in = _cot_in(cur)
+                    bar_out = _cot_out(out)
                     _lib.check(L.sph_step_vjp(nb_j.workspace.handle, nb_j.bank,
                                               ctypes.byref(self.prm), self.dt, self.ctvf,
                                               ctypes.byref(self.bc),
@@ -334,8 +370,8 @@ class GraphRollout:
                                               1,  # records left by this step's own R pass
                                               _lib.stream()), "sph_step_vjp")
                     cur = out
-                for dst, src in zip(self.bar, cur):
-                    dst.copy_(src)
+                for key, dst in self.bar.items():
+                    dst.copy_(cur[key])
             self.gR.append(gR)
             self.gV.append(gV)
             self._steps.append(steps)   # keeps the recomputed states alive
@@ -370,7 +406,7 @@ class GraphRollout:
     def _forward_kept(self, u0, r0):
         """forward() with a set per segment: segment s runs R[s] from the end
         state of segment s - 1, and all of them stay in place."""
-        carried = _carried(self.tvf)
+        carried = self.carried
         n = self.n_steps // self.k
         self.bits.zero_()
         ckpts = []
@@ -400,15 +436,18 @@ class GraphRollout:
         that made `ckpts` (same generation), its last segment (with a set per
         segment: every segment) is still in place and is not recomputed."""
         main = torch.cuda.current_stream()
-        self.bar[0].copy_(rb_final)
-        self.bar[1].copy_(ub_final)
-        for b in self.bar[2:]:
-            b.zero_()
+        for key, b in self.bar.items():
+            if key == "r":
+                b.copy_(rb_final)
+            elif key == "u":
+                b.copy_(ub_final)
+            else:
+                b.zero_()
         n = len(ckpts)
         if self.keep_all and generation == self.generation:
             for seg in reversed(range(n)):
                 self.gV[seg].replay()
-            return self.bar[0].clone(), self.bar[1].clone()
+            return self.bar["r"].clone(), self.bar["u"].clone()
         ev_r = [torch.cuda.Event() for _ in range(n)]
         ev_v = [torch.cuda.Event() for _ in range(n)]
         for side in self.sides:
@@ -438,7 +477,7 @@ class GraphRollout:
             ev_v[seg].record(main)
             if seg - ns >= 0:
                 recompute(seg - ns)     # reuses the set V(seg) just released
-        return self.bar[0].clone(), self.bar[1].clone()
+        return self.bar["r"].clone(), self.bar["u"].clone()
 
     def __call__(self, u0: torch.Tensor, r0: torch.Tensor):
         """Differentiable (u_final, r_final) of the rollout from (u0, r0)."""

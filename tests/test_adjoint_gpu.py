@@ -313,6 +313,88 @@ def test_wall_bc_rollout_gradient_matches_oracle_fd(tvf, n_steps, k):
             assert abs(fd_nobc - fd) > 0.01 * abs(fd), (fd, fd_nobc)
 
 
+@pytest.mark.parametrize("case,tvf,n_steps,k", [("tgv", 0.0, 1, 1), ("tgv", 0.0, 4, 2),
+                                                 ("tgv", 1.0, 3, 3), ("ldc", 0.0, 12, 4)])
+def test_continuity_rollout_gradient_matches_oracle_fd(case, tvf, n_steps, k):
+    """Density continuity (density_evolution, solver.py:24-30): the density
+    is advanced from the one the step is given, so it is carried, and its
+    cotangent crosses the steps.  2D TGV (dx = 0.05) and the lid-driven cavity
+    with plain walls held by the hook (dx = 0.05, 10 warm-up steps); the
+    gradient of 0.5 sum(u^2) + <w, r> after n steps in the initial velocity
+    (fluid) and positions, against float64 oracle central differences, to
+    1e-3 as above.  From two steps on, the differences of the same rollouts
+    with density summation differ by more than 1% (the cavity, whose density
+    moves slowly, over 12 steps)."""
+    ov = {"case": {"dx": 0.05, "viscosity": 0.1}, "solver": {"t_end": 1.0, "tvf": tvf,
+                                                             "density_evolution": True}}
+    if case == "tgv":
+        ov["case"]["dim"] = 2
+    else:
+        ov["solver"]["is_bc_trick"] = False
+    cfg, box, state, g, bc, eos, disp, shift, solver = G.setup(case, ov)
+    assert solver.is_rho_evol and not solver.is_bc_trick
+    dt = cfg.solver.dt
+    cut = solver._kernel_fn.cutoff
+    st = state
+    if case == "ldc":
+        warm = partition.neighbor_list(disp, box, r_cutoff=cut, mask_self=False,
+                                       format=partition.Sparse)
+        nb = warm.allocate(state["r"])
+        adv = integrator.si_euler(tvf, solver.forward_wrapper(), shift, bc, None)
+        for _ in range(10):
+            st, nb = adv(dt, st, nb)
+    fn = partition.neighbor_list(disp, box, r_cutoff=cut, capacity_multiplier=1.25,
+                                 mask_self=False, format=partition.Sparse, skin=0.0)
+    n = st["r"].shape[0]
+    gen = torch.Generator().manual_seed(7)
+    w = torch.randn(n, 2, generator=gen).cuda().double()
+    u0 = st["u"].clone().requires_grad_(True)
+    r0 = st["r"].clone().requires_grad_(True)
+    s_in = dict(st)
+    s_in["u"], s_in["r"] = u0, r0
+    bc_fn = bc if case == "ldc" else None
+    u, r, _ = adjoint.rollout(s_in, solver, fn, shift, dt, n_steps, tvf=tvf,
+                              checkpoint_every=k, bc_fn=bc_fn)
+    (0.5 * (u.double() ** 2).sum() + (w * r.double()).sum()).backward()
+    gro = adjoint.GraphRollout(st, solver, fn, shift, dt, n_steps, tvf=tvf,
+                               checkpoint_every=k, bc_fn=bc_fn)
+    u2 = st["u"].clone().requires_grad_(True)
+    r2 = st["r"].clone().requires_grad_(True)
+    ug, rg = gro(u2, r2)
+    (0.5 * (ug.double() ** 2).sum() + (w * rg.double()).sum()).backward()
+    assert torch.equal(ug, u) and torch.equal(rg, r)
+    assert torch.equal(u2.grad, u0.grad) and torch.equal(r2.grad, r0.grad)
+    fluid = (st["tag"] == 0).cpu().numpy()[:, None]
+    gu = u0.grad.double().cpu().numpy() * fluid
+    gr = r0.grad.double().cpu().numpy()
+    assert np.isfinite(gu).all() and np.isfinite(gr).all()
+    base = G.to_np(st)
+    wn = w.cpu().numpy()
+    summ = G.setup(case, {**ov, "solver": {**ov["solver"], "density_evolution": False}})[-1]
+
+    def L(eps, du, dr, sv=solver):
+        s = G.to_f64(dict(base))
+        s["u"] = s["u"] + eps * du
+        s["v"] = s["u"].copy()
+        s["r"] = np.mod(s["r"] + eps * dr, box)
+        out = _oracle_rollout(s, sv, box, dt, n_steps, tvf=tvf, bc_fn=bc_fn)
+        return 0.5 * float((out["u"] ** 2).sum()) + float((wn * out["r"]).sum())
+
+    rng = np.random.default_rng(4)
+    eps = 1e-6
+    norm = np.sqrt((gu ** 2).sum() + (gr ** 2).sum())
+    dirs = [(gu / norm, gr / norm)] + [(rng.standard_normal(gu.shape) * fluid,
+                                        rng.standard_normal(gr.shape)) for _ in range(2)]
+    for trial, (du, dr) in enumerate(dirs):
+        fd = (L(eps, du, dr) - L(-eps, du, dr)) / (2 * eps)
+        ad = float((gu * du).sum() + (gr * dr).sum())
+        scale = abs(fd) if trial == 0 else float(np.abs(gu * du).sum() + np.abs(gr * dr).sum())
+        assert abs(ad - fd) <= 1e-3 * scale, (trial, ad, fd, scale)
+        if trial == 0 and n_steps >= 2:
+            fd_s = (L(eps, du, dr, summ) - L(-eps, du, dr, summ)) / (2 * eps)
+            assert abs(fd_s - fd) > 0.01 * abs(fd), (fd, fd_s)
+
+
 def test_checkpoint_interval_does_not_change_the_gradient():
     cfg, box, st, solver, fn, shift, gen = _setup()
     n = st["r"].shape[0]
