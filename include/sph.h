@@ -26,6 +26,7 @@
 #ifndef SPH_H
 #define SPH_H
 
+#include <stddef.h>
 #include <stdint.h>
 #include <hip/hip_runtime_api.h>
 
@@ -152,6 +153,28 @@ int sph_nl_fill_max(int32_t dim, int32_t n, const float *r, const float *cell_si
 #define SPH_FILL_MAX_STATIC_CELLS (1 << 21)
 int sph_nl_fill_max_static(int32_t dim, int32_t n, const float *r, const float *cell_size,
                            const int32_t *cps, int64_t *info, hipStream_t stream);
+/* Dense cell list of CellListFns.allocate/update (jax_md/partition.py:243-428,
+ * cell_list_fn at :294-413).  Buffers are cell-major over the flat cell hash
+ * x + nx*(y + ny*z), `capacity` slots per cell: position_buffer
+ * [cells*capacity*dim] f32, id_buffer [cells*capacity] i32, and n_named side
+ * arrays (per particle named_width[b] 32-bit words, read from named[b]) into
+ * named_buffer[b] [cells*capacity*named_width[b]].  Empty slots hold position
+ * 0, id n and named_fill[b] in every word.  The particle at cell-sorted
+ * position s (cells ascending, ids ascending inside a cell: the stable argsort
+ * of :367-373) goes to slot cell*capacity + s mod capacity (:379-385).  In a
+ * cell holding more than `capacity` particles the later one of two that share
+ * a slot keeps it, and info[0] |= SPH_CELL_LIST_OVERFLOW (info[0] is not
+ * cleared, so overflow is sticky as in update at :396).  info[2] = the
+ * largest cell occupancy.  Cells are int32(r / cell_size) clamped into the
+ * grid (the reference hashes r == side into a wrong cell).  `work` is caller
+ * scratch of sph_cell_list_work_bytes(n, cells) bytes, 4-byte aligned. */
+size_t sph_cell_list_work_bytes(int64_t n, int64_t cell_count);
+int sph_cell_list(int32_t dim, int32_t n, const float *r, const float *cell_size,
+                  const int32_t *cps, int64_t capacity, float *position_buffer,
+                  int32_t *id_buffer, int32_t n_named, const uint32_t *const *named,
+                  const int32_t *named_width, const uint32_t *named_fill,
+                  uint32_t *const *named_buffer, void *work, size_t work_bytes, int64_t *info,
+                  hipStream_t stream);
 /* Rebuild test of update (partition.py:1010-1018): info[4] = any(|r - r_ref|^2 > thr_sq). */
 int sph_nl_needs_rebuild(sph_ws *ws, const float *r, const float *r_ref, float threshold_sq,
                          int64_t *info, hipStream_t stream);

@@ -1544,6 +1544,54 @@ static int scratch_of_current_device(int32_t** out) {
   *out = g_scratch_ptr[dev];
   return SPH_OK;
 }
+
+// ----------------------------------------------------------- dense cell list
+// Slot of the particle at sorted position s (cell-major, ascending id inside a
+// cell): cell * capacity + s mod capacity, the scatter index of
+// partition.py:368-385.  A cell's particles take consecutive slots from its
+// first sorted position mod capacity, wrapping inside the cell, so up to
+// `capacity` of them never collide.  In an overflowing cell the particle
+// capacity places later in the cell takes the slot (the later scatter update
+// wins), so every slot has exactly one writer.
+__device__ __forceinline__ int64_t cell_list_slot(const int32_t* __restrict__ perm,
+                                                  const int32_t* __restrict__ pcell,
+                                                  const int32_t* __restrict__ cell_start, int s,
+                                                  int64_t cap, int* id) {
+  *id = perm[s];
+  const int h = pcell[*id];
+  if ((int64_t)s + cap < (int64_t)cell_start[h + 1]) return -1;  // overwritten later
+  return (int64_t)h * cap + s % cap;
+}
+
+template <int DIM>
+__global__ void k_cell_list_fill(const float* __restrict__ r, const int32_t* __restrict__ perm,
+                                 const int32_t* __restrict__ pcell,
+                                 const int32_t* __restrict__ cell_start, int n, int64_t cap,
+                                 float* __restrict__ pos_buf, int32_t* __restrict__ id_buf) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  int id;
+  const int64_t slot = cell_list_slot(perm, pcell, cell_start, s, cap, &id);
+  if (slot < 0) return;
+  id_buf[slot] = id;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) pos_buf[slot * DIM + d] = r[(int64_t)id * DIM + d];
+}
+
+// One 32-bit word of a per-particle side array (`width` words per particle).
+__global__ void k_cell_list_named(const uint32_t* __restrict__ src, int width,
+                                  const int32_t* __restrict__ perm,
+                                  const int32_t* __restrict__ pcell,
+                                  const int32_t* __restrict__ cell_start, int n, int64_t cap,
+                                  uint32_t* __restrict__ dst) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)n * width) return;
+  const int s = (int)(t / width), w = (int)(t - (int64_t)s * width);
+  int id;
+  const int64_t slot = cell_list_slot(perm, pcell, cell_start, s, cap, &id);
+  if (slot < 0) return;
+  dst[slot * width + w] = src[(int64_t)id * width + w];
+}
 }  // namespace sph
 
 extern "C" int sph_nl_fill_max(int32_t dim, int32_t n, const float* r, const float* cell_size,
@@ -1573,6 +1621,98 @@ extern "C" int sph_nl_fill_max_static(int32_t dim, int32_t n, const float* r,
   const int rc = scratch_of_current_device(&counts);
   if (rc != SPH_OK) return rc;
   return fill_max_run(dim, n, r, g, nc, counts, info, st);
+}
+
+extern "C" size_t sph_cell_list_work_bytes(int64_t n, int64_t cell_count) {
+  if (n < 0 || cell_count < 1) return 0;
+  // counts, cell starts, cell and arrival slot per particle, arrivals, sorted ids, inverse
+  return (size_t)(2 * cell_count + 1 + 5 * n) * sizeof(int32_t);
+}
+
+extern "C" int sph_cell_list(int32_t dim, int32_t n, const float* r, const float* cell_size,
+                             const int32_t* cps, int64_t capacity, float* position_buffer,
+                             int32_t* id_buffer, int32_t n_named, const uint32_t* const* named,
+                             const int32_t* named_width, const uint32_t* named_fill,
+                             uint32_t* const* named_buffer, void* work, size_t work_bytes,
+                             int64_t* info, hipStream_t st) {
+  Grid g;
+  int64_t nc = 0;
+  if (!fill_grid("sph_cell_list", dim, n, r, cell_size, cps, info, &g, &nc)) return SPH_ERR_ARG;
+  if (!position_buffer || !id_buffer || !work || n_named < 0 ||
+      (n_named > 0 && (!named || !named_width || !named_fill || !named_buffer))) {
+    set_error("sph_cell_list: bad argument");
+    return SPH_ERR_ARG;
+  }
+  if (nc > INT32_MAX) { set_error("sph_cell_list: more than 2^31 - 1 cells"); return SPH_ERR_SIZE; }
+  if (capacity < 1 || nc * capacity > INT64_MAX / 4 / dim) {
+    set_error("sph_cell_list: capacity must be >= 1 and the buffers addressable");
+    return SPH_ERR_ARG;
+  }
+  if (work_bytes < sph_cell_list_work_bytes(n, nc)) {
+    set_error("sph_cell_list: work buffer smaller than sph_cell_list_work_bytes");
+    return SPH_ERR_SIZE;
+  }
+  for (int b = 0; b < n_named; ++b)
+    if (!named[b] || !named_buffer[b] || named_width[b] < 1) {
+      set_error("sph_cell_list: bad side array " + std::to_string(b));
+      return SPH_ERR_ARG;
+    }
+  int32_t* counts = static_cast<int32_t*>(work);
+  int32_t* start = counts + nc;
+  int32_t* pcell = start + nc + 1;
+  int32_t* pslot = pcell + n;
+  int32_t* unsorted = pslot + n;
+  int32_t* perm = unsorted + n;
+  int32_t* iperm = perm + n;
+  const int64_t slots = nc * capacity;
+  sph_ws* ws = nullptr;  // no profiling without a workspace
+  // empty slots: position 0, id N, side arrays their fill word (partition.py:334-358)
+  SPH_LAUNCH(ws, "fill", st,
+             k_fill<float><<<cdiv(slots * dim, kBlock), kBlock, 0, st>>>(position_buffer,
+                                                                        slots * dim, 0.0f));
+  SPH_LAUNCH(ws, "fill", st,
+             k_fill<int32_t><<<cdiv(slots, kBlock), kBlock, 0, st>>>(id_buffer, slots, n));
+  for (int b = 0; b < n_named; ++b) {
+    const int64_t words = slots * named_width[b];
+    SPH_LAUNCH(ws, "fill", st,
+               k_fill<uint32_t><<<cdiv(words, kBlock), kBlock, 0, st>>>(named_buffer[b], words,
+                                                                       named_fill[b]));
+  }
+  SPH_LAUNCH(ws, "fill", st, k_fill<int32_t><<<cdiv(nc, kBlock), kBlock, 0, st>>>(counts, nc, 0));
+  if (n > 0) {
+    if (dim == 2)
+      SPH_LAUNCH(ws, "hash_count", st,
+                 k_hash_count<2><<<cdiv(n, kBlock), kBlock, 0, st>>>(r, n, g, counts, pcell,
+                                                                   pslot, nullptr));
+    else
+      SPH_LAUNCH(ws, "hash_count", st,
+                 k_hash_count<3><<<cdiv(n, kBlock), kBlock, 0, st>>>(r, n, g, counts, pcell,
+                                                                   pslot, nullptr));
+  }
+  // cell starts, the largest occupancy into info[2], CELL_LIST_OVERFLOW into info[0]
+  SPH_LAUNCH(ws, "scan_cells", st,
+             (k_scan_single<int32_t, int32_t><<<1, kScanThreads, 0, st>>>(
+                 counts, nc, start, &info[2], info, capacity)));
+  if (n == 0) return SPH_OK;
+  SPH_LAUNCH(ws, "scatter", st,
+             k_scatter<<<cdiv(n, kBlock), kBlock, 0, st>>>(pcell, pslot, start, unsorted, n));
+  // ascending id inside each cell: the order of the reference's stable argsort
+  SPH_LAUNCH(ws, "cell_sort", st,
+             k_cell_sort<3><<<cdiv(nc * 64, 256), 256, 0, st>>>(start, unsorted, perm, iperm, nc,
+                                                               nullptr, nullptr, Grid{}));
+  if (dim == 2)
+    SPH_LAUNCH(ws, "cell_list_fill", st,
+               k_cell_list_fill<2><<<cdiv(n, kBlock), kBlock, 0, st>>>(
+                   r, perm, pcell, start, n, capacity, position_buffer, id_buffer));
+  else
+    SPH_LAUNCH(ws, "cell_list_fill", st,
+               k_cell_list_fill<3><<<cdiv(n, kBlock), kBlock, 0, st>>>(
+                   r, perm, pcell, start, n, capacity, position_buffer, id_buffer));
+  for (int b = 0; b < n_named; ++b)
+    SPH_LAUNCH(ws, "cell_list_named", st,
+               k_cell_list_named<<<cdiv((int64_t)n * named_width[b], kBlock), kBlock, 0, st>>>(
+                   named[b], named_width[b], perm, pcell, start, n, capacity, named_buffer[b]));
+  return SPH_OK;
 }
 
 extern "C" int sph_nl_needs_rebuild(sph_ws* ws, const float* r, const float* r_ref,

@@ -114,6 +114,10 @@ def lib():
                                     vp, vp]),
         "sph_nl_fill_max_static": (c_int, [c_int32, c_int32, vp, POINTER(c_float),
                                            POINTER(c_int32), vp, vp]),
+        "sph_cell_list_work_bytes": (ctypes.c_size_t, [c_int64, c_int64]),
+        "sph_cell_list": (c_int, [c_int32, c_int32, vp, POINTER(c_float), POINTER(c_int32),
+                                  c_int64, vp, vp, c_int32, vp, vp, vp, vp, vp,
+                                  ctypes.c_size_t, vp, vp]),
         "sph_nl_needs_rebuild": (c_int, [vp, vp, vp, c_float, vp, vp]),
         "sph_nl_build": (c_int, [vp, c_int, vp, vp, c_int64, vp, vp]),
         "sph_nl_finalize": (c_int, [vp, c_int64, vp, vp]),

@@ -485,3 +485,215 @@ def neighbor_list(displacement_or_metric, box, r_cutoff: float, dr_threshold: fl
     fns = NeighborListFns(allocate_fn, update_fn)
     fns.search = s
     return fns
+
+
+# ------------------------------------------------------------------ cell lists
+class CellList:
+    """partition.py:53-101.  Buffers have the common leading shape
+    ``S = (nz, ny, nx, cell_capacity)`` (``(ny, nx, cell_capacity)`` in 2D):
+    ``position_buffer`` S + (dim,) float32, ``id_buffer`` S + (1,) int32 with
+    empty slots = N, ``named_buffer[k]`` S + the side array's trailing shape
+    (``(1,)`` for a 1-d array) with empty slots = 10**5.
+    ``did_buffer_overflow`` is a 0-d bool device tensor."""
+
+    def __init__(self, position_buffer, id_buffer, named_buffer, did_buffer_overflow,
+                 cell_capacity: int, cell_size, update_fn: Callable):
+        self.position_buffer = position_buffer
+        self.id_buffer = id_buffer
+        self.named_buffer = named_buffer
+        self.did_buffer_overflow = did_buffer_overflow
+        self.cell_capacity = cell_capacity
+        self.cell_size = cell_size
+        self.update_fn = update_fn
+
+    def update(self, position, **kwargs) -> "CellList":
+        cl_data = (self.cell_capacity, self.did_buffer_overflow, self.update_fn)
+        return self.update_fn(position, cl_data, **kwargs)
+
+    @property
+    def kwarg_buffers(self):
+        return self.named_buffer
+
+
+class CellListFns:
+    """partition.py:104-111."""
+
+    def __init__(self, allocate: Callable, update: Callable):
+        self.allocate = allocate
+        self.update = update
+
+    def __iter__(self):
+        return iter((self.allocate, self.update))
+
+
+class _CellGrid:
+    """_cell_dimensions (partition.py:112-156).  A Python-scalar box is
+    divided in float64 and gives ``(cps,) * dim`` cells with no size check, as
+    in the reference; an array box is divided in float32 (this package's
+    precision) and must hold at least 3 cells per side."""
+
+    def __init__(self, dim: int, box_size, minimum_cell_size):
+        if isinstance(box_size, torch.Tensor):
+            box_size = box_size.detach().cpu().numpy()
+        self.scalar = isinstance(box_size, (int, float, np.integer, np.floating))
+        if self.scalar:
+            box = float(box_size)
+            cps = np.floor(box / float(minimum_cell_size))
+            cell_size = box / cps
+            self.cps = np.full(dim, int(cps), np.int64)
+            self.cell_size = cell_size                     # float, as the reference stores it
+            self.cell_size32 = np.full(dim, np.float32(cell_size), np.float32)
+            self.shape = (int(cps),) * dim
+        else:
+            b = np.asarray(box_size)
+            box = b.astype(np.float32)
+            if box.ndim == 0:
+                cps = np.floor(box / np.float32(minimum_cell_size))
+                self.cell_size = (box / cps).astype(np.float32)
+                self.cps = np.full(dim, int(cps), np.int64)
+                self.cell_size32 = np.full(dim, self.cell_size, np.float32)
+                self.shape = (int(cps),) * dim
+            else:
+                box = box.reshape(-1)
+                assert box.size == dim
+                mcs = np.asarray(minimum_cell_size, np.float32)
+                cps = np.floor(box / mcs)
+                self.cell_size = (box / cps).astype(np.float32)
+                self.cps = cps.astype(np.int64)
+                if np.any(self.cps < 3):
+                    raise ValueError("Box must be at least 3x the size of the grid spacing in "
+                                     "each dimension.")
+                self.cell_size32 = self.cell_size.reshape(-1)
+                self.shape = tuple(int(x) for x in self.cps[::-1])
+        if np.any(self.cps < 1):
+            raise ValueError(f"Box {box_size} holds no cell of size {minimum_cell_size}.")
+        self.box_size = box_size
+        self.count = int(np.prod(self.cps))
+
+    def c_args(self, dim):
+        cs = (ctypes.c_float * 3)(*([float(x) for x in self.cell_size32] + [1.0] * (3 - dim)))
+        cp = (ctypes.c_int32 * 3)(*([int(x) for x in self.cps] + [1] * (3 - dim)))
+        return cs, cp
+
+
+def _cell_counts(position: torch.Tensor, grid: _CellGrid, info: torch.Tensor) -> torch.Tensor:
+    n, dim = position.shape
+    counts = torch.empty(grid.count, dtype=torch.int32, device=position.device)
+    cs, cp = grid.c_args(dim)
+    r = position if n else torch.zeros(1, dtype=torch.float32, device=position.device)
+    _lib.check(_lib.lib().sph_nl_fill_max(dim, n, _lib.ptr(r), cs, cp, _lib.ptr(counts),
+                                          _lib.ptr(info), _lib.stream()), "sph_nl_fill_max")
+    return counts
+
+
+def count_cell_filling(position, box_size, minimum_cell_size) -> torch.Tensor:
+    """partition.py:159-174: particles per cell, flat over the cell hash
+    (int32 on the device).  A particle on the upper box face counts in the
+    last cell (the reference hashes it into a wrong one)."""
+    r = _lib.as_f32(position)
+    grid = _CellGrid(int(r.shape[1]), box_size, minimum_cell_size)
+    info = torch.zeros(8, dtype=torch.int64, device=r.device)
+    return _cell_counts(r, grid, info)
+
+
+_EMPTY_KWARG_VALUE = 10 ** 5  # partition.py:334
+
+
+def _side_array(k, v, n, dev):
+    """A per-particle side array as 32-bit words (partition.py:336-357):
+    floating types in float32, integer types in int32 (this package's
+    precision), bool through int32."""
+    if not isinstance(v, (torch.Tensor, np.ndarray)):
+        raise ValueError(f'Data must be specified as an ndarray. Found "{k}" with type {type(v)}.')
+    if v.shape[0] != n:
+        raise ValueError("Data must be specified per-particle (an ndarray with shape "
+                         f'({n}, ...)). Found "{k}" with shape {tuple(v.shape)}.')
+    t = torch.as_tensor(v)
+    if t.dtype == torch.bool:
+        out_dtype, t = torch.bool, t.to(torch.int32)
+    elif t.is_floating_point():
+        t = t.to(torch.float32)
+        out_dtype = torch.float32
+    else:
+        t = t.to(torch.int32)
+        out_dtype = torch.int32
+    tail = tuple(v.shape[1:]) if len(v.shape) > 1 else (1,)
+    t = t.to(dev).contiguous()
+    fill = (np.array([_EMPTY_KWARG_VALUE], np.float32).view(np.uint32)[0]
+            if t.dtype == torch.float32 else _EMPTY_KWARG_VALUE)
+    return t, tail, out_dtype, int(fill)
+
+
+def cell_list(box_size, minimum_cell_size: float,
+              buffer_size_multiplier: float = 1.25) -> CellListFns:
+    """partition.py:243-428 on the HIP library (``sph_cell_list``): hashing,
+    the stable cell sort, the slot scatter and the occupancy all run on the
+    device; ``allocate`` reads the largest cell occupancy back to size the
+    capacity, ``update`` never synchronises."""
+
+    def cell_list_fn(position, capacity_overflow_update=None, extra_capacity: int = 0,
+                     **kwargs) -> CellList:
+        dev = _lib.device()
+        r = _lib.as_f32(position, dev)
+        if r.dim() != 2 or r.shape[1] not in (2, 3):
+            raise ValueError(f"Cell list spatial dimension must be 2 or 3. Found "
+                             f"{r.shape[-1] if r.dim() else 0}.")
+        n, dim = r.shape
+        grid = _CellGrid(dim, box_size, minimum_cell_size)
+        info = torch.zeros(8, dtype=torch.int64, device=dev)
+        if capacity_overflow_update is None:
+            # _estimate_cell_capacity (partition.py:195-199) on the grid
+            # re-derived from the rounded cell size
+            grid2 = _CellGrid(dim, grid.box_size, grid.cell_size)
+            _cell_counts(r, grid2, info)
+            cell_capacity = int(int(info[2]) * buffer_size_multiplier) + int(extra_capacity)
+            overflow = torch.zeros((), dtype=torch.bool, device=dev)
+            update_fn = cell_list_fn
+        else:
+            cell_capacity, overflow, update_fn = capacity_overflow_update
+            overflow = torch.as_tensor(overflow, device=dev).to(torch.bool)
+        cell_capacity = int(cell_capacity)
+        side = [(k,) + _side_array(k, v, n, dev) for k, v in kwargs.items()]
+        slots = grid.count * cell_capacity
+        pos_buf = torch.empty(slots * dim, dtype=torch.float32, device=dev)
+        id_buf = torch.empty(slots, dtype=torch.int32, device=dev)
+        named = {}
+        for k, t, tail, _, _ in side:
+            named[k] = torch.empty(slots * int(np.prod(tail)), dtype=t.dtype, device=dev)
+        if slots == 0:
+            pass  # no slot to write: only an empty particle set sizes a capacity of 0
+        else:
+            L = _lib.lib()
+            work = torch.empty(max(1, L.sph_cell_list_work_bytes(n, grid.count)),
+                               dtype=torch.uint8, device=dev)
+            cs, cp = grid.c_args(dim)
+            k = len(side)
+            src = (ctypes.c_void_p * max(k, 1))(*[t.data_ptr() for _, t, *_ in side])
+            dst = (ctypes.c_void_p * max(k, 1))(*[named[s[0]].data_ptr() for s in side])
+            width = (ctypes.c_int32 * max(k, 1))(*[int(np.prod(s[2])) for s in side])
+            fill = (ctypes.c_uint32 * max(k, 1))(*[s[4] for s in side])
+            rr = r if n else torch.zeros(1, dtype=torch.float32, device=dev)
+            _lib.check(L.sph_cell_list(dim, n, _lib.ptr(rr), cs, cp, cell_capacity,
+                                       _lib.ptr(pos_buf), _lib.ptr(id_buf), k, src, width, fill,
+                                       dst, _lib.ptr(work), work.numel(), _lib.ptr(info),
+                                       _lib.stream()), "sph_cell_list")
+            overflow = overflow | ((info[0] & int(PEC.CELL_LIST_OVERFLOW)) != 0)
+        shape = grid.shape + (cell_capacity,)
+        named_out = {}
+        for k, t, tail, out_dtype, _ in side:
+            b = named[k].view(shape + tail)
+            named_out[k] = b != 0 if out_dtype == torch.bool else b
+        return CellList(pos_buf.view(shape + (dim,)), id_buf.view(shape + (1,)), named_out,
+                        overflow, cell_capacity, grid.cell_size, update_fn)
+
+    def allocate_fn(position, extra_capacity: int = 0, **kwargs) -> CellList:
+        return cell_list_fn(position, extra_capacity=extra_capacity, **kwargs)
+
+    def update_fn(position, cl_or_capacity, **kwargs) -> CellList:
+        if isinstance(cl_or_capacity, int):
+            return cell_list_fn(position, (int(cl_or_capacity), False, cell_list_fn), **kwargs)
+        cl = cl_or_capacity
+        return cell_list_fn(position, (cl.cell_capacity, cl.did_buffer_overflow, cl.update_fn),
+                            **kwargs)
+
+    return CellListFns(allocate_fn, update_fn)

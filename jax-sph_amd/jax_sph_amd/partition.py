@@ -11,6 +11,8 @@ from __future__ import annotations
 
 from .jax_md import space  # noqa: F401
 from .jax_md.partition import (  # noqa: F401
+    CellList,
+    CellListFns,
     Dense,
     NeighborList,
     NeighborListFns,
@@ -20,6 +22,8 @@ from .jax_md.partition import (  # noqa: F401
     PartitionErrorCode,
     Sparse,
     StaleNeighborListError,
+    cell_list,
+    count_cell_filling,
     is_sparse,
 )
 from .jax_md.partition import neighbor_list as vmap_neighbor_list

@@ -15,14 +15,15 @@ LIB = os.path.join(ROOT, "jax-sph_amd", "jax_sph_amd", "libsph.so")
 
 def declared_symbols():
     text = open(HEADER).read()
-    return sorted(set(re.findall(r"^\s*(?:int|int64_t|const char \*)\s*\*?\s*(sph_\w+)\s*\(",
+    return sorted(set(re.findall(r"^\s*(?:int|int64_t|size_t|const char \*)\s*\*?\s*(sph_\w+)\s*\(",
                                  text, re.M)))
 
 
 def test_header_declares_entry_points():
     syms = declared_symbols()
     for must in ("sph_ws_create", "sph_nl_build", "sph_nl_write_idx", "sph_forward",
-                 "sph_step", "sph_kernel_eval", "sph_integrate"):
+                 "sph_step", "sph_kernel_eval", "sph_integrate", "sph_cell_list",
+                 "sph_cell_list_work_bytes"):
         assert must in syms
 
 
