@@ -255,14 +255,17 @@ typedef struct {
  * tvf).  `bc` (nullable) is the step's case boundary hook as in sph_step:
  * the velocities and accelerations it sets are constants of the step, so the
  * particles it holds carry only a position cotangent.
- * `next` is state n+1 (r, u, v with tvf, rho, p, mass, eta, tag) and `bank`
+ * `next` is state n+1 (r, u, v with tvf, rho, p, mass, eta, tag; g_ext, the
+ * body force at next->r, or NULL for prm->g_const) and `bank`
  * must hold the pairs of next->r; given the cotangents of state n+1 writes
  * those of state n (bar may alias nothing in bar_next).  With density
  * continuity records_ready must be set (the density the step advanced is not
  * in `next`).  records_ready != 0
  * promises that the last kernels on this workspace were the sph_step that
  * produced `next` into `bank` (its sorted records are reused instead of
- * gathered again). */
+ * gathered again).  A position-dependent body force's own derivative,
+ * (dg/dr')^T applied to the cotangent of dudt', is not included: the caller
+ * adds it to bar_next->r (it is zero for piecewise-constant forces). */
 int sph_step_vjp(sph_ws *ws, int bank, const sph_solver_params *prm, float dt, float tvf_coef,
                  const sph_bc_spec *bc, const sph_state_in *next,
                  const sph_cotangent_in *bar_next, sph_cotangent_out *bar,

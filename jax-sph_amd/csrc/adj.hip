@@ -567,7 +567,8 @@ extern "C" int sph_step_vjp(sph_ws* ws, int bank, const sph_solver_params* prm, 
   }
   sph_state_in in = *next;
   if (!tvf) in.v = next->u;   // tvf = 0: v == u
-  in.g_ext = nullptr;
+  // a position-dependent body force (next->g_ext, at next->r) enters only the
+  // walls' pressure extrapolation; its own derivative is the caller's
   sph_state_out none{};
   AdjArgs a;
   a.s = make_args(ws, bank, prm, &in, &none);

@@ -17,7 +17,8 @@ Memory is O(n_steps / k + k) states instead of O(n_steps).
 
 Differentiated configuration (other settings raise ``NotImplementedError``):
 standard SPH (``solver="SPH"``), no heat conduction, no artificial viscosity,
-Quintic or Wendland-C2 kernel, a constant body force, with or without the
+Quintic or Wendland-C2 kernel, a constant body force or (with density
+summation, through :func:`rollout`) a position-dependent one, with or without the
 transport-velocity formulation (any ``tvf``: the stress ``A = rho u (v - u)``
 and the background-pressure acceleration, solver.py:199-256); density
 summation with or without the generalized wall boundary condition for
@@ -45,7 +46,7 @@ from .solver import WCSPH
 _f = np.float32
 
 
-def _check(solver: WCSPH, tvf: float, bc_fn=None):
+def _check(solver: WCSPH, tvf: float, bc_fn=None, graph: bool = False):
     why = []
     if bc_fn is not None and _bc_spec(bc_fn) is None:
         why.append("bc_fn must be a case boundary hook (case_setup.CaseBC)")
@@ -63,8 +64,11 @@ def _check(solver: WCSPH, tvf: float, bc_fn=None):
         why.append("artificial viscosity is not differentiated")
     if solver._kernel_fn.name not in ("QSK", "WC2K"):
         why.append("kernel must be QSK or WC2K")
-    if solver.g_const is None:
-        why.append("the body force must be constant")
+    if solver.g_const is None and graph:
+        why.append("the graph-replayed rollout needs a constant body force (rollout() "
+                   "differentiates a position-dependent one)")
+    if solver.g_const is None and solver.is_rho_evol:
+        why.append("a position-dependent body force is differentiated with density summation")
     if why:
         raise NotImplementedError("differentiable rollout: " + "; ".join(why))
 
@@ -109,8 +113,34 @@ class _Runner:
         self.prm = solver.params(tvf_stress=self.tvf != 0.0)
         self.ctvf = float(_tvf_coef(self.tvf, self.dt))
         self.bc = _bc_spec(bc_fn)
+        self.bc_fn = bc_fn
         self.nb: Optional[NeighborList] = None
         self.reallocations = 0
+        self._kept: Optional[torch.Tensor] = None
+
+    def _body_force_pullback(self, st, cur):
+        """A position-dependent body force adds g(r') to the step's dudt'
+        (solver.py:936, before the case hook), so r' also receives
+        (dg/dr')^T of the cotangent of dudt' where the hook keeps dudt'.  The
+        derivative is taken by autograd through the caller's g_ext_fn (a torch
+        function of r); a force built from comparisons (torch.where, as the
+        Poiseuille cases' are) has none, as under jax.grad."""
+        r = st["r"].detach().clone().requires_grad_(True)
+        with torch.enable_grad():
+            gv = self.solver.g_ext_fn(r)
+        if not (isinstance(gv, torch.Tensor) and gv.requires_grad):
+            return
+        a = cur["dudt"]
+        if self.bc_fn is not None:
+            if self._kept is None:   # 1 where the hook leaves dudt alone (tags are fixed)
+                probe = dict(st)
+                probe["dudt"] = torch.ones_like(st["dudt"])
+                probe["dvdt"] = torch.ones_like(st["dvdt"])
+                self._kept = self.bc_fn(probe)["dudt"].detach().clone()
+            a = a * self._kept
+        (jr,) = torch.autograd.grad(gv, r, grad_outputs=a.to(gv.dtype), allow_unused=True)
+        if jr is not None:
+            cur["r"] += jr.to(cur["r"].dtype)
 
     def _list_at(self, r):
         if self.nb is None:
@@ -175,7 +205,10 @@ class _Runner:
                     nbn = self._list_at(nxt["r"])
                     ready = 0
                 ws = nbn.workspace
-                cin = self.solver.c_state_in(nxt)
+                g = self.solver.g_ext_tensor(nxt["r"])   # None for a constant force
+                if g is not None:
+                    self._body_force_pullback(nxt, cur)
+                cin = self.solver.c_state_in(nxt, g)
                 bar_in = _cot_in(cur)
                 bar_out = _cot_out(spare)
                 _lib.check(L.sph_step_vjp(ws.handle, nbn.bank, ctypes.byref(self.prm), self.dt,
@@ -269,7 +302,7 @@ class GraphRollout:
                  dt: float, n_steps: int, tvf: float = 0.0, checkpoint_every: int = 10,
                  workspace_sets: Optional[int] = None, side_streams: int = 3, bc_fn=None,
                  list_margin: float = 1.5):
-        _check(solver, tvf, bc_fn)
+        _check(solver, tvf, bc_fn, graph=True)
         if getattr(getattr(neighbor_fn, "search", None), "skin", 0.0):
             raise ValueError("the differentiable rollout needs a neighbor_list with skin=0")
         self.k = int(checkpoint_every)

@@ -395,6 +395,109 @@ def test_continuity_rollout_gradient_matches_oracle_fd(case, tvf, n_steps, k):
             assert abs(fd_s - fd) > 0.01 * abs(fd), (fd, fd_s)
 
 
+class _SmoothForce:
+    """g(r) = a (sin 2 pi y, cos 2 pi x): a body force whose derivative in r
+    the adjoint must carry; ``np`` is its float64 twin for the oracle."""
+
+    def __init__(self, a):
+        self.a = float(a)
+
+    def __call__(self, r):
+        t = 2 * np.pi
+        return self.a * torch.stack([torch.sin(t * r[:, 1]), torch.cos(t * r[:, 0])], 1)
+
+    def np(self, r):
+        t = 2 * np.pi
+        return self.a * np.stack([np.sin(t * r[:, 1]), np.cos(t * r[:, 0])], 1)
+
+
+@pytest.mark.parametrize("case,n_steps,k", [("pf", 3, 2), ("smooth", 1, 1), ("smooth", 4, 2)])
+def test_body_force_rollout_gradient_matches_oracle_fd(case, n_steps, k, monkeypatch):
+    """A position-dependent body force g(r) (g_ext_fn, solver.py:936).  The
+    Poiseuille channel as its case file runs it (pf.py:134-143: the force
+    switched on between the walls by comparisons, so dg/dr = 0 as under
+    jax.grad; no-slip walls with the wall boundary condition, held by the
+    hook; dx = 0.05, fluid velocities perturbed), and a 2D TGV (dx = 0.05)
+    driven by the smooth force a (sin 2 pi y, cos 2 pi x), a = 200, whose
+    dg/dr the rollout takes by autograd through the force.  Gradient of
+    0.5 sum(u^2) + <w, r> after n steps in the initial velocity (fluid) and
+    positions against float64 oracle central differences, to 1e-3 as above.
+    For the smooth force, dropping the dg/dr term moves the directional
+    derivative by more than the tolerance (from two steps on)."""
+    if case == "pf":
+        ov = {"case": {"dx": 0.05}}
+        cfg, box, state, g, bc, eos, disp, shift, solver = G.setup("pf", ov)
+        assert solver.g_const is None and solver.is_bc_trick and bc.kind == 1
+        bc_fn = bc
+    else:
+        ov = {"case": {"dim": 2, "dx": 0.05, "viscosity": 0.1}, "solver": {"t_end": 1.0}}
+        cfg, box, state, g, bc, eos, disp, shift, solver = G.setup("tgv", ov)
+        from jax_sph_amd import case_setup as cs
+        solver = cs.make_solver(cfg, disp, eos, _SmoothForce(200.0))
+        bc_fn = None
+    dt = cfg.solver.dt
+    n = state["r"].shape[0]
+    gen = torch.Generator().manual_seed(11)
+    fluid_t = (state["tag"] == 0)[:, None]
+    st = dict(state)
+    st["u"] = torch.where(fluid_t, state["u"] + 0.05 * torch.randn(n, 2, generator=gen).cuda(),
+                          state["u"])
+    st["v"] = st["u"].clone()
+    fn = partition.neighbor_list(disp, box, r_cutoff=solver._kernel_fn.cutoff,
+                                 capacity_multiplier=1.25, mask_self=False,
+                                 format=partition.Sparse, skin=0.0)
+    w = torch.randn(n, 2, generator=gen).cuda().double()
+
+    def grads():
+        u0 = st["u"].clone().requires_grad_(True)
+        r0 = st["r"].clone().requires_grad_(True)
+        s_in = dict(st)
+        s_in["u"], s_in["r"] = u0, r0
+        u, r, _ = adjoint.rollout(s_in, solver, fn, shift, dt, n_steps, checkpoint_every=k,
+                                  bc_fn=bc_fn)
+        (0.5 * (u.double() ** 2).sum() + (w * r.double()).sum()).backward()
+        return u0.grad.double().cpu().numpy(), r0.grad.double().cpu().numpy()
+
+    fluid = fluid_t.cpu().numpy()
+    gu, gr = grads()
+    gu = gu * fluid
+    assert np.isfinite(gu).all() and np.isfinite(gr).all()
+    if case == "smooth":
+        with monkeypatch.context() as m:
+            m.setattr(adjoint._Runner, "_body_force_pullback", lambda self, s_, c_: None)
+            gu_no, gr_no = grads()
+        gu_no = gu_no * fluid
+    base = G.to_np(st)
+    wn = w.cpu().numpy()
+
+    def L(eps, du, dr):
+        s = G.to_f64(dict(base))
+        s["u"] = s["u"] + eps * du
+        s["v"] = s["u"].copy()
+        s["r"] = np.mod(s["r"] + eps * dr, box)
+        out = _oracle_rollout(s, solver, box, dt, n_steps, bc_fn=bc_fn)
+        return 0.5 * float((out["u"] ** 2).sum()) + float((wn * out["r"]).sum())
+
+    rng = np.random.default_rng(6)
+    eps = 1e-6
+    norm = np.sqrt((gu ** 2).sum() + (gr ** 2).sum())
+    dirs = [(gu / norm, gr / norm)] + [(rng.standard_normal(gu.shape) * fluid,
+                                        rng.standard_normal(gr.shape)) for _ in range(2)]
+    for trial, (du, dr) in enumerate(dirs):
+        fd = (L(eps, du, dr) - L(-eps, du, dr)) / (2 * eps)
+        ad = float((gu * du).sum() + (gr * dr).sum())
+        scale = abs(fd) if trial == 0 else float(np.abs(gu * du).sum() + np.abs(gr * dr).sum())
+        msg = (case, trial, ad, fd, scale)
+        if case == "smooth":
+            ad_no = float((gu_no * du).sum() + (gr_no * dr).sum())
+            msg += (ad_no,)
+            print(f"body force {case} n={n_steps} trial {trial}: ad {ad:.9g} fd {fd:.9g} "
+                  f"without dg/dr {ad_no:.9g} scale {scale:.6g}", flush=True)
+        assert abs(ad - fd) <= 1e-3 * scale, msg
+        if case == "smooth" and n_steps >= 2 and trial == 0:
+            assert abs(ad_no - fd) > 2e-3 * scale, msg
+
+
 def test_checkpoint_interval_does_not_change_the_gradient():
     cfg, box, st, solver, fn, shift, gen = _setup()
     n = st["r"].shape[0]
@@ -420,6 +523,17 @@ def test_unsupported_configurations_raise():
                                       format=partition.Sparse)
     with pytest.raises(ValueError):
         adjoint.rollout(st, solver, skin_fn, shift, cfg.solver.dt, 2)
+    # a position-dependent body force: not under graph replay, not with continuity
+    from jax_sph_amd import case_setup as cs
+    ov = {"case": {"dim": 2, "dx": 0.05}}
+    c2, _, _, _, _, eos, disp, _, _ = G.setup("tgv", ov)
+    forced = cs.make_solver(c2, disp, eos, _SmoothForce(1.0))
+    with pytest.raises(NotImplementedError, match="constant body force"):
+        adjoint.GraphRollout(st, forced, fn, shift, cfg.solver.dt, 2, checkpoint_every=1)
+    c3 = G.setup("tgv", {**ov, "solver": {"density_evolution": True}})[0]
+    forced_c = cs.make_solver(c3, disp, eos, _SmoothForce(1.0))
+    with pytest.raises(NotImplementedError, match="summation"):
+        adjoint.rollout(st, forced_c, fn, shift, cfg.solver.dt, 2)
 
 
 def _c4_fixture():
