@@ -244,11 +244,11 @@ typedef struct {
   float *r, *u, *a, *b, *rho;
 } sph_cotangent_out;
 /* Vector-Jacobian product of one si_euler step (integrator.py:22-56), the
- * transpose that jax.grad builds for the reference's step.  Standard SPH,
- * density summation, or continuity (solver.py:24-30) without the wall
- * boundary condition and renormalisation, no heat/artificial viscosity, the
- * generalized wall boundary condition (bc_trick, solver.py:431-530) for
- * no-slip walls with summation, QSK or WC2K; with prm->tvf_stress the
+ * transpose that jax.grad builds for the reference's step.  Standard SPH
+ * without heat; density summation or continuity (solver.py:24-30, without
+ * renormalisation); with or without artificial viscosity (solver.py:404-428)
+ * and the generalized wall boundary condition (bc_trick, solver.py:431-530)
+ * for no-slip walls; QSK or WC2K; with prm->tvf_stress the
  * transport-velocity step
  * (solver.py:199-256: the A = rho u (v - u) stress and the background-pressure
  * dvdt), whose v' = u' + tvf_coef * dvdt (tvf_coef = tvf dt / 2, 0 without

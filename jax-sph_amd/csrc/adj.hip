@@ -2,9 +2,10 @@
 //
 // The reference differentiates its step with jax.grad (reverse mode through
 // XLA).  Here the transpose of the step is written out.  Scope: the standard
-// SPH solver with density summation, no heat and no artificial viscosity,
-// with or without the transport-velocity formulation (tvf), and with or
-// without the generalized wall boundary condition for no-slip walls (below).
+// SPH solver without heat, density summation or continuity (sweep C), with or
+// without the transport-velocity formulation (tvf), artificial viscosity
+// (k_adj_forces) and the generalized wall boundary condition for no-slip
+// walls (below).
 // State n -> n+1 (integrator.py:26-54, solver.py:176-178, 199-256, 801-802):
 //
 //   u' = u + dt a,   v' = u' + c b  (c = tvf dt / 2),   r' = wrap(r + dt v')
@@ -104,8 +105,13 @@ __global__ void k_adj_gather(AdjArgs a) {
 // WALLS: the records after the generalized wall BC (U2, V2, Q2), and for wall
 // particles the raw cotangent of rho^ in ab0.w (the walls sweeps consume it).
 // CONT (density continuity): ab0.w = B, the whole cotangent of rho', the next
-// step's included.
-template <int DIM, int KIND, bool TVF, bool WALLS, bool CONT>
+// step's included (for a wall with WALLS, that of rho^).
+// AV: the artificial viscosity of fluid-fluid pairs (solver.py:404-428),
+//   a_k += K m_j phi_kj (du . r_kj) r_kj,  phi = W'(d) / (d rho_ab (d^2 + 0.01 h^2)),
+// K = alpha h c_ab, rho_ab = (rho_k + rho_j) / 2, du = u_k - u_j; its
+// cotangent-weighted sum is 1/2 sum_kj K phi (du . r)(r . l^AV_kj) with
+// l^AV_kj = m_j abar'_k - m_k abar'_j, symmetric in (k, j) like S_kj.
+template <int DIM, int KIND, bool TVF, bool WALLS, bool CONT, bool AV>
 __global__ __launch_bounds__(kAB) void k_adj_forces(AdjArgs aa, float4* __restrict__ ab0,
                                                     float4* __restrict__ ab1,
                                                     float4* __restrict__ ab2) {
@@ -131,6 +137,9 @@ __global__ __launch_bounds__(kAB) void k_adj_forces(AdjArgs aa, float4* __restri
   const V3 wk = TVF ? v3(RV[k]) - uk : V3{0.f, 0.f, 0.f};
   const V3 nk = TVF ? v3(aa.lb[k]) : V3{0.f, 0.f, 0.f};
   const float pbg = TVF ? eos.p_fn(0.0f) : 0.0f;
+  const bool fluid = (int)Qk.z == kFluid;
+  const float avK = AV ? p.artificial_alpha * p.av_h * 10.0f : 0.0f;  // alpha h c_ab
+  const float av_h2 = AV ? 0.01f * p.av_h * p.av_h : 0.0f;
   float rhob = 0.0f;
   V3 ub = {0.f, 0.f, 0.f}, rb = {0.f, 0.f, 0.f}, vb = {0.f, 0.f, 0.f};
   for_each_pair(a, k, h, [&](const int j) {
@@ -189,13 +198,27 @@ __global__ __launch_bounds__(kAB) void k_adj_forces(AdjArgs aa, float4* __restri
       vb = vb + (0.5f * rhok * uk_lc) * dr;
       rb = rb + (0.5f * rhok * uk_lc) * wk + (0.5f * rhoj * dot<DIM>(uj, lc)) * wj;
     }
+    if (AV && fluid && (int)Qj.z == kFluid) {
+      const float D = d2 + av_h2;
+      const float iD = rcp_fast(D);
+      const float irab = 2.0f * irs;              // 1 / rho_ab
+      const float Kphi = avK * A0 * irab * iD;    // K phi
+      const V3 du = uk - uj;
+      const float e = dot<DIM>(du, dr);
+      const V3 l = mj * lk - mk * lj;
+      const float sl = dot<DIM>(dr, l);
+      // d phi / dd, over d
+      const float dphi = (Ad0 * D - 2.0f * A0 * d) * irab * iD * iD * inv;
+      ub = ub + (Kphi * sl) * dr;
+      rhob -= Kphi * e * sl * irs;                // d phi / d rho_k = -phi / (rho_k + rho_j)
+      rb = rb + (avK * dphi * e * sl) * dr + Kphi * (sl * du + e * l);
+    }
   });
   rhob = lane_sum<TPP>(rhob);
   ub = lane_sum<TPP>(ub);
   rb = lane_sum<TPP>(rb);
   if (TVF) vb = lane_sum<TPP>(vb);
   if (h != 0) return;
-  const bool fluid = (int)Qk.z == kFluid;
   if (CONT) {
     ab0[k] = f4(ub, rhob + (aa.rhob_in ? aa.rhob_in[a.perm[k]] : 0.0f));
   } else {
@@ -269,7 +292,9 @@ __global__ __launch_bounds__(kAB) void k_adj_walls_prep(AdjArgs aa, float4* __re
 // them (u, v, p -> rho, rho, and r); wall particles gather the derivative of
 // their averages with respect to their own position.  Each particle updates
 // only its own ab0/ab1/ab2 entries.
-template <int DIM, int KIND, bool TVF>
+// CONT: the fluid's density cotangent is that of rho' itself (no m_k of the
+// summation's chain).
+template <int DIM, int KIND, bool TVF, bool CONT>
 __global__ __launch_bounds__(kAB) void k_adj_walls(AdjArgs aa, float4* __restrict__ ab0,
                                                    float4* __restrict__ ab1,
                                                    float4* __restrict__ ab2,
@@ -344,7 +369,7 @@ __global__ __launch_bounds__(kAB) void k_adj_walls(AdjArgs aa, float4* __restric
   ab1[k] = f4(v3(B1) + rb, B1.w);
   if (fluid_k) {
     const float4 B0 = ab0[k];
-    const float mk = a.R.P[k].w;
+    const float mk = CONT ? 1.0f : a.R.P[k].w;
     ab0[k] = f4(v3(B0) + ub, B0.w + mk * (eos.dp_drho(rhok) * pacc + racc));
     if (TVF) {
       const float4 B2 = ab2[k];
@@ -477,7 +502,7 @@ __global__ __launch_bounds__(kAB) void k_adj_density(AdjArgs aa, const float4* _
   if (aa.bb_out) store_vec<DIM>(aa.bb_out, i, aa.c_tvf * vbt);
 }
 
-template <int DIM, int KIND, bool TVF, bool WALLS, bool CONT>
+template <int DIM, int KIND, bool TVF, bool WALLS, bool CONT, bool AV>
 int launch_adjoint(sph_ws* ws, AdjArgs& a, bool records_ready, hipStream_t st) {
   const int n = a.s.n;
   dim3 grid(xcd_grid(cdivl((int64_t)n * tpp_of(DIM), kAB), a.s.remap));
@@ -489,39 +514,49 @@ int launch_adjoint(sph_ws* ws, AdjArgs& a, bool records_ready, hipStream_t st) {
   SPH_LAUNCH(ws, "adj_gather", st,
              (k_adj_gather<DIM, TVF><<<(unsigned)cdivl(n, kAB), kAB, 0, st>>>(a)));
   SPH_LAUNCH(ws, "adj_forces", st,
-             (k_adj_forces<DIM, KIND, TVF, WALLS, CONT><<<grid, kAB, 0, st>>>(
+             (k_adj_forces<DIM, KIND, TVF, WALLS, CONT, AV><<<grid, kAB, 0, st>>>(
                  a, ws->AB0, ws->AB1, ws->AB2)));
-  if (CONT) {
-    SPH_LAUNCH(ws, "adj_continuity", st,
-               (k_adj_continuity<DIM, KIND, TVF><<<grid, kAB, 0, st>>>(a, ws->AB0, ws->AB1,
-                                                                        ws->AB2)));
-    return SPH_OK;
-  }
   if (WALLS) {
     // the delta-SPH records hold the per-wall factors (no delta-SPH here)
     SPH_LAUNCH(ws, "adj_walls_prep", st,
                (k_adj_walls_prep<DIM, KIND, TVF><<<grid, kAB, 0, st>>>(
                    a, ws->AB0, ws->AB2, ws->DL0, ws->DL1, ws->DL2, ws->DG0)));
     SPH_LAUNCH(ws, "adj_walls", st,
-               (k_adj_walls<DIM, KIND, TVF><<<grid, kAB, 0, st>>>(
+               (k_adj_walls<DIM, KIND, TVF, CONT><<<grid, kAB, 0, st>>>(
                    a, ws->AB0, ws->AB1, ws->AB2, ws->DL0, ws->DL1, ws->DL2, ws->DG0)));
+  }
+  if (CONT) {
+    // the walls' rho^ cotangent is spent above (their rho' is overwritten by rho^)
+    SPH_LAUNCH(ws, "adj_continuity", st,
+               (k_adj_continuity<DIM, KIND, TVF><<<grid, kAB, 0, st>>>(a, ws->AB0, ws->AB1,
+                                                                        ws->AB2)));
+    return SPH_OK;
   }
   SPH_LAUNCH(ws, "adj_density", st,
              (k_adj_density<DIM, KIND, TVF><<<grid, kAB, 0, st>>>(a, ws->AB0, ws->AB1, ws->AB2)));
   return SPH_OK;
 }
 
-template <int DIM, int KIND>
-int launch_adjoint_tvf(sph_ws* ws, AdjArgs& a, bool tvf, bool walls, bool cont,
-                       bool records_ready, hipStream_t st) {
+template <int DIM, int KIND, bool TVF, bool AV>
+int launch_adjoint_kind(sph_ws* ws, AdjArgs& a, bool walls, bool cont, bool records_ready,
+                        hipStream_t st) {
   if (cont)
-    return tvf ? launch_adjoint<DIM, KIND, true, false, true>(ws, a, records_ready, st)
-               : launch_adjoint<DIM, KIND, false, false, true>(ws, a, records_ready, st);
-  if (walls)
-    return tvf ? launch_adjoint<DIM, KIND, true, true, false>(ws, a, records_ready, st)
-               : launch_adjoint<DIM, KIND, false, true, false>(ws, a, records_ready, st);
-  return tvf ? launch_adjoint<DIM, KIND, true, false, false>(ws, a, records_ready, st)
-             : launch_adjoint<DIM, KIND, false, false, false>(ws, a, records_ready, st);
+    return walls ? launch_adjoint<DIM, KIND, TVF, true, true, AV>(ws, a, records_ready, st)
+                 : launch_adjoint<DIM, KIND, TVF, false, true, AV>(ws, a, records_ready, st);
+  return walls ? launch_adjoint<DIM, KIND, TVF, true, false, AV>(ws, a, records_ready, st)
+               : launch_adjoint<DIM, KIND, TVF, false, false, AV>(ws, a, records_ready, st);
+}
+
+template <int DIM, int KIND>
+int launch_adjoint_tvf(sph_ws* ws, AdjArgs& a, bool tvf, bool walls, bool cont, bool av,
+                       bool records_ready, hipStream_t st) {
+  if (av)
+    return tvf ? launch_adjoint_kind<DIM, KIND, true, true>(ws, a, walls, cont, records_ready, st)
+               : launch_adjoint_kind<DIM, KIND, false, true>(ws, a, walls, cont, records_ready,
+                                                             st);
+  return tvf ? launch_adjoint_kind<DIM, KIND, true, false>(ws, a, walls, cont, records_ready, st)
+             : launch_adjoint_kind<DIM, KIND, false, false>(ws, a, walls, cont, records_ready,
+                                                            st);
 }
 
 }  // namespace sph
@@ -543,13 +578,11 @@ extern "C" int sph_step_vjp(sph_ws* ws, int bank, const sph_solver_params* prm, 
     set_error("sph_step_vjp: the r, u and a cotangent arrays are required");
     return SPH_ERR_ARG;
   }
-  if (prm->solver != 0 || prm->heat_conduction || prm->artificial_alpha != 0.0f ||
-      (prm->bc_trick && (prm->free_slip || prm->density_evolution)) ||
+  if (prm->solver != 0 || prm->heat_conduction || (prm->bc_trick && prm->free_slip) ||
       (prm->density_evolution && prm->rho_renorm)) {
-    set_error("sph_step_vjp: only the standard SPH solver with no heat and no artificial "
-              "viscosity is differentiated: density summation, with or without the wall "
-              "boundary condition for no-slip walls, or density continuity without that "
-              "condition and without renormalisation");
+    set_error("sph_step_vjp: only the standard SPH solver with no heat is differentiated: "
+              "density summation or continuity (without renormalisation), with or without "
+              "the wall boundary condition for no-slip walls and artificial viscosity");
     return SPH_ERR_ARG;
   }
   if (prm->kernel != SPH_KERNEL_QSK && prm->kernel != SPH_KERNEL_WC2K) {
@@ -605,11 +638,12 @@ extern "C" int sph_step_vjp(sph_ws* ws, int bank, const sph_solver_params* prm, 
                                                        nullptr));
   }
   const bool walls = prm->bc_trick != 0, cont = prm->density_evolution != 0;
+  const bool av = prm->artificial_alpha != 0.0f;
   if (prm->dim == 2)
     return prm->kernel == SPH_KERNEL_QSK
-               ? launch_adjoint_tvf<2, SPH_KERNEL_QSK>(ws, a, tvf, walls, cont, ready, st)
-               : launch_adjoint_tvf<2, SPH_KERNEL_WC2K>(ws, a, tvf, walls, cont, ready, st);
+               ? launch_adjoint_tvf<2, SPH_KERNEL_QSK>(ws, a, tvf, walls, cont, av, ready, st)
+               : launch_adjoint_tvf<2, SPH_KERNEL_WC2K>(ws, a, tvf, walls, cont, av, ready, st);
   return prm->kernel == SPH_KERNEL_QSK
-             ? launch_adjoint_tvf<3, SPH_KERNEL_QSK>(ws, a, tvf, walls, cont, ready, st)
-             : launch_adjoint_tvf<3, SPH_KERNEL_WC2K>(ws, a, tvf, walls, cont, ready, st);
+             ? launch_adjoint_tvf<3, SPH_KERNEL_QSK>(ws, a, tvf, walls, cont, av, ready, st)
+             : launch_adjoint_tvf<3, SPH_KERNEL_WC2K>(ws, a, tvf, walls, cont, av, ready, st);
 }

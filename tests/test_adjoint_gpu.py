@@ -314,7 +314,9 @@ def test_wall_bc_rollout_gradient_matches_oracle_fd(tvf, n_steps, k):
 
 
 @pytest.mark.parametrize("case,tvf,n_steps,k", [("tgv", 0.0, 1, 1), ("tgv", 0.0, 4, 2),
-                                                 ("tgv", 1.0, 3, 3), ("ldc", 0.0, 12, 4)])
+                                                 ("tgv", 1.0, 3, 3), ("ldc", 0.0, 12, 4),
+                                                 ("ldc_gwbc", 0.0, 12, 4),
+                                                 ("ldc_gwbc", 1.0, 12, 3)])
 def test_continuity_rollout_gradient_matches_oracle_fd(case, tvf, n_steps, k):
     """Density continuity (density_evolution, solver.py:24-30): the density
     is advanced from the one the step is given, so it is carried, and its
@@ -324,15 +326,19 @@ def test_continuity_rollout_gradient_matches_oracle_fd(case, tvf, n_steps, k):
     (fluid) and positions, against float64 oracle central differences, to
     1e-3 as above.  From two steps on, the differences of the same rollouts
     with density summation differ by more than 1% (the cavity, whose density
-    moves slowly, over 12 steps)."""
+    moves slowly, over 12 steps).  The cavity also runs with the wall
+    boundary condition (ldc_gwbc): the walls' rho^ is then carried to the
+    next step's continuity update of their fluid neighbours."""
     ov = {"case": {"dx": 0.05, "viscosity": 0.1}, "solver": {"t_end": 1.0, "tvf": tvf,
                                                              "density_evolution": True}}
+    gwbc = case == "ldc_gwbc"
+    case = "ldc" if gwbc else case
     if case == "tgv":
         ov["case"]["dim"] = 2
     else:
-        ov["solver"]["is_bc_trick"] = False
+        ov["solver"]["is_bc_trick"] = gwbc
     cfg, box, state, g, bc, eos, disp, shift, solver = G.setup(case, ov)
-    assert solver.is_rho_evol and not solver.is_bc_trick
+    assert solver.is_rho_evol and solver.is_bc_trick == gwbc
     dt = cfg.solver.dt
     cut = solver._kernel_fn.cutoff
     st = state
@@ -496,6 +502,120 @@ def test_body_force_rollout_gradient_matches_oracle_fd(case, n_steps, k, monkeyp
         assert abs(ad - fd) <= 1e-3 * scale, msg
         if case == "smooth" and n_steps >= 2 and trial == 0:
             assert abs(ad_no - fd) > 2e-3 * scale, msg
+
+
+@pytest.mark.parametrize("case,tvf,n_steps,k", [("tgv", 0.0, 1, 1), ("tgv", 0.0, 4, 2),
+                                                 ("tgv", 1.0, 3, 3), ("db", 0.0, 2, 1),
+                                                 ("db", 0.0, 4, 2)])
+def test_artificial_viscosity_rollout_gradient_matches_oracle_fd(case, tvf, n_steps, k):
+    """Artificial viscosity (solver.py:404-428) on fluid-fluid pairs.  The 2D
+    TGV (dx = 0.05, density summation, alpha = 0.5) and the dam break as its
+    case file runs it (cases/db.yaml: density continuity, the wall boundary
+    condition with gravity normal to the bottom wall, alpha = 0.1, the hook
+    holding the walls; dx = 0.1, 50 warm-up steps so the column moves).  TGV:
+    the gradient of 0.5 sum(u^2) + <w, r> after n steps in the fluid's
+    initial velocity and all positions; dam break: the gradient of <w, u>
+    after n steps in all initial positions, along position directions only
+    (so the derivative is the dynamics' alone: along velocity directions the
+    identity part of <w, u> hides it; after one step u' does not depend on
+    the positions).  Against float64 oracle central
+    differences, to 1e-3 as above.  From two steps on, the TGV differences
+    taken without the artificial viscosity differ by more than 1%, so its
+    transpose is exercised (in the dam break, whose velocities are still
+    nearly uniform after the warm-up, it moves them too little to test; there
+    the differences without the wall boundary condition differ by more than
+    1%).  GraphRollout equals the eager rollout bit for bit."""
+    if case == "tgv":
+        ov = {"case": {"dim": 2, "dx": 0.05, "viscosity": 0.01},
+              "solver": {"t_end": 1.0, "tvf": tvf, "artificial_alpha": 0.5}}
+    else:
+        ov = {"case": {"dx": 0.1}}
+    cfg, box, state, g, bc, eos, disp, shift, solver = G.setup(case, ov)
+    assert solver.artificial_alpha > 0
+    if case == "db":
+        assert solver.is_rho_evol and solver.is_bc_trick and bc.kind == 2
+    dt = cfg.solver.dt
+    cut = solver._kernel_fn.cutoff
+    st = state
+    bc_fn = bc if case == "db" else None
+    if case == "db":
+        warm = partition.neighbor_list(disp, box, r_cutoff=cut, mask_self=False,
+                                       format=partition.Sparse)
+        nb = warm.allocate(state["r"])
+        adv = integrator.si_euler(tvf, solver.forward_wrapper(), shift, bc, None)
+        for _ in range(50):
+            st, nb = adv(dt, st, nb)
+    else:
+        gen0 = torch.Generator().manual_seed(2)
+        n0 = st["r"].shape[0]
+        st = dict(st)
+        st["u"] = (st["u"].cpu() + 0.1 * torch.randn(n0, 2, generator=gen0)).cuda()
+        st["v"] = st["u"].clone()
+    fn = partition.neighbor_list(disp, box, r_cutoff=cut, capacity_multiplier=1.25,
+                                 mask_self=False, format=partition.Sparse, skin=0.0)
+    n = st["r"].shape[0]
+    gen = torch.Generator().manual_seed(9)
+    w = torch.randn(n, 2, generator=gen).cuda().double()
+    u0 = st["u"].clone().requires_grad_(True)
+    r0 = st["r"].clone().requires_grad_(True)
+    s_in = dict(st)
+    s_in["u"], s_in["r"] = u0, r0
+    lin = case == "db"
+
+    def loss(u, r):
+        if lin:
+            return (w * u.double()).sum() + 0.0 * r.sum()
+        return 0.5 * (u.double() ** 2).sum() + (w * r.double()).sum()
+
+    u, r, _ = adjoint.rollout(s_in, solver, fn, shift, dt, n_steps, tvf=tvf,
+                              checkpoint_every=k, bc_fn=bc_fn)
+    loss(u, r).backward()
+    gro = adjoint.GraphRollout(st, solver, fn, shift, dt, n_steps, tvf=tvf,
+                               checkpoint_every=k, bc_fn=bc_fn)
+    u2 = st["u"].clone().requires_grad_(True)
+    r2 = st["r"].clone().requires_grad_(True)
+    ug, rg = gro(u2, r2)
+    loss(ug, rg).backward()
+    assert torch.equal(ug, u) and torch.equal(rg, r)
+    assert torch.equal(u2.grad, u0.grad) and torch.equal(r2.grad, r0.grad)
+    fluid = (st["tag"] == 0).cpu().numpy()[:, None]
+    gu = u0.grad.double().cpu().numpy() * fluid
+    gr = r0.grad.double().cpu().numpy()
+    assert np.isfinite(gu).all() and np.isfinite(gr).all()
+    base = G.to_np(st)
+    wn = w.cpu().numpy()
+    off = {"artificial_alpha": 0.0} if case == "tgv" else {"is_bc_trick": False}
+    alt = G.setup(case, {**ov, "solver": {**ov.get("solver", {}), **off}})[-1]
+
+    def L(eps, du, dr, sv=solver):
+        s = G.to_f64(dict(base))
+        s["u"] = s["u"] + eps * du
+        s["v"] = s["u"].copy()
+        s["r"] = np.mod(s["r"] + eps * dr, box)
+        s["r"] = np.where(s["r"] >= box, s["r"] - box, s["r"])  # -tiny mod box rounds to box
+        out = _oracle_rollout(s, sv, box, dt, n_steps, tvf=tvf, bc_fn=bc_fn)
+        if lin:
+            return float((wn * out["u"]).sum())
+        return 0.5 * float((out["u"] ** 2).sum()) + float((wn * out["r"]).sum())
+
+    rng = np.random.default_rng(8)
+    eps = 1e-6
+    if lin:
+        gu = np.zeros_like(gu)
+    norm = np.sqrt((gu ** 2).sum() + (gr ** 2).sum())
+    dirs = [(gu / norm, gr / norm)] + [(rng.standard_normal(gu.shape) * fluid * (not lin),
+                                        rng.standard_normal(gr.shape)) for _ in range(2)]
+    for trial, (du, dr) in enumerate(dirs):
+        fd = (L(eps, du, dr) - L(-eps, du, dr)) / (2 * eps)
+        ad = float((gu * du).sum() + (gr * dr).sum())
+        scale = abs(fd) if trial == 0 else float(np.abs(gu * du).sum() + np.abs(gr * dr).sum())
+        print(f"av {case} tvf={tvf} n={n_steps} trial {trial}: ad {ad:.9g} fd {fd:.9g} "
+              f"scale {scale:.6g}", flush=True)
+        assert abs(ad - fd) <= 1e-3 * scale, (case, trial, ad, fd, scale)
+        if trial == 0 and n_steps >= 2:
+            fd_no = (L(eps, du, dr, alt) - L(-eps, du, dr, alt)) / (2 * eps)
+            print(f"  without {off}: fd {fd_no:.9g}", flush=True)
+            assert abs(fd_no - fd) > 0.01 * abs(fd), (fd, fd_no)
 
 
 def test_checkpoint_interval_does_not_change_the_gradient():
