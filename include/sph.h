@@ -245,7 +245,8 @@ typedef struct {
 } sph_cotangent_out;
 /* Vector-Jacobian product of one si_euler step (integrator.py:22-56), the
  * transpose that jax.grad builds for the reference's step.  Standard SPH
- * without heat; density summation or continuity (solver.py:24-30, without
+ * (heat conduction may be on: u', r' do not depend on the temperature, whose
+ * cotangent is not formed); density summation or continuity (solver.py:24-30, without
  * renormalisation); with or without artificial viscosity (solver.py:404-428)
  * and the generalized wall boundary condition (bc_trick, solver.py:431-530)
  * for no-slip walls; QSK or WC2K; with prm->tvf_stress the

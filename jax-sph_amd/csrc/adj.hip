@@ -2,10 +2,12 @@
 //
 // The reference differentiates its step with jax.grad (reverse mode through
 // XLA).  Here the transpose of the step is written out.  Scope: the standard
-// SPH solver without heat, density summation or continuity (sweep C), with or
+// SPH solver, density summation or continuity (sweep C), with or
 // without the transport-velocity formulation (tvf), artificial viscosity
 // (k_adj_forces) and the generalized wall boundary condition for no-slip
-// walls (below).
+// walls (below).  Heat conduction (solver.py:832-850) may be on: the
+// temperature enters neither the EOS nor the forces, so u' and r' do not
+// depend on it and the transpose below is the whole one for them.
 // State n -> n+1 (integrator.py:26-54, solver.py:176-178, 199-256, 801-802):
 //
 //   u' = u + dt a,   v' = u' + c b  (c = tvf dt / 2),   r' = wrap(r + dt v')
@@ -578,9 +580,9 @@ extern "C" int sph_step_vjp(sph_ws* ws, int bank, const sph_solver_params* prm, 
     set_error("sph_step_vjp: the r, u and a cotangent arrays are required");
     return SPH_ERR_ARG;
   }
-  if (prm->solver != 0 || prm->heat_conduction || (prm->bc_trick && prm->free_slip) ||
+  if (prm->solver != 0 || (prm->bc_trick && prm->free_slip) ||
       (prm->density_evolution && prm->rho_renorm)) {
-    set_error("sph_step_vjp: only the standard SPH solver with no heat is differentiated: "
+    set_error("sph_step_vjp: only the standard SPH solver is differentiated: "
               "density summation or continuity (without renormalisation), with or without "
               "the wall boundary condition for no-slip walls and artificial viscosity");
     return SPH_ERR_ARG;

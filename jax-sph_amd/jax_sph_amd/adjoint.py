@@ -16,8 +16,7 @@ positions.  :func:`rollout` wraps an ``n_steps`` rollout in a
 Memory is O(n_steps / k + k) states instead of O(n_steps).
 
 Differentiated configuration (other settings raise ``NotImplementedError``):
-standard SPH (``solver="SPH"``), no heat conduction, Quintic or Wendland-C2
-kernel, a constant body force or (with density summation, through
+standard SPH (``solver="SPH"``), Quintic or Wendland-C2 kernel, a constant body force or (with density summation, through
 :func:`rollout`) a position-dependent one, with or without the
 transport-velocity formulation (any ``tvf``: the stress ``A = rho u (v - u)``
 and the background-pressure acceleration, solver.py:199-256), with or without
@@ -26,7 +25,9 @@ continuity (``density_evolution``, solver.py:24-30; the density is then
 carried from step to step, and checkpointed; no renormalisation), each with
 or without the generalized wall boundary condition for no-slip walls
 (``is_bc_trick``, solver.py:431-530).  The dam break as its case file runs
-it (continuity, wall condition, artificial viscosity, gravity) is covered.  Wall
+it (continuity, wall condition, artificial viscosity, gravity) is covered.
+Heat conduction may be on: the temperature enters neither the EOS nor the
+forces (solver.py:832-850), so the outputs u and r do not depend on it.  Wall
 particles may be held by the case's boundary hook (``bc_fn``, a
 ``case_setup.CaseBC``: the lid-driven cavity as the reference's case file
 runs it, and as its gradient check runs it, notebooks/iclr24_grads.ipynb).
@@ -58,8 +59,6 @@ def _check(solver: WCSPH, tvf: float, bc_fn=None, graph: bool = False):
         why.append("density renormalisation is not differentiated")
     if solver.is_bc_trick and solver.is_free_slip:
         why.append("free-slip walls are not differentiated")
-    if solver.is_heat_conduction:
-        why.append("heat conduction is not differentiated")
     if solver._kernel_fn.name not in ("QSK", "WC2K"):
         why.append("kernel must be QSK or WC2K")
     if solver.g_const is None and graph:

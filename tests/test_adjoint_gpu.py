@@ -618,6 +618,64 @@ def test_artificial_viscosity_rollout_gradient_matches_oracle_fd(case, tvf, n_st
             assert abs(fd_no - fd) > 0.01 * abs(fd), (fd, fd_no)
 
 
+def test_heat_conduction_leaves_the_velocity_and_position_gradient():
+    """With heat conduction on (solver.py:832-850) the temperature evolves
+    beside the flow but enters neither the EOS nor the forces: the rollout's
+    u and r, and so their gradient, are those of the same run without it.
+    2D TGV (dx = 0.05) with a nonuniform temperature; the gradient of
+    0.5 sum(u^2) + <w, r> after 3 steps equals the one without heat to float32
+    rounding and matches float64 oracle central differences of the run with
+    heat to 1e-3."""
+    ov = {"case": {"dim": 2, "dx": 0.05, "viscosity": 0.1},
+          "solver": {"t_end": 1.0, "heat_conduction": True}}
+    cfg, box, state, g, bc, eos, disp, shift, solver = G.setup("tgv", ov)
+    plain = G.setup("tgv", {**ov, "solver": {"t_end": 1.0}})[-1]
+    assert solver.is_heat_conduction and not plain.is_heat_conduction
+    n = state["r"].shape[0]
+    gen = torch.Generator().manual_seed(4)
+    st = dict(state)
+    st["T"] = (1.0 + 0.2 * torch.rand(n, generator=gen)).cuda()
+    st["kappa"] = torch.full_like(st["T"], 0.5)    # the case's defaults are 0
+    st["Cp"] = torch.full_like(st["T"], 2.0)
+    st["u"] = (state["u"].cpu() + 0.05 * torch.randn(n, 2, generator=gen)).cuda()
+    st["v"] = st["u"].clone()
+    dt = cfg.solver.dt
+    fn = partition.neighbor_list(disp, box, r_cutoff=solver._kernel_fn.cutoff,
+                                 capacity_multiplier=1.25, mask_self=False,
+                                 format=partition.Sparse, skin=0.0)
+    w = torch.randn(n, 2, generator=gen).cuda().double()
+    grads = []
+    for sv in (solver, plain):
+        u0 = st["u"].clone().requires_grad_(True)
+        r0 = st["r"].clone().requires_grad_(True)
+        s_in = dict(st)
+        s_in["u"], s_in["r"] = u0, r0
+        u, r, _ = adjoint.rollout(s_in, sv, fn, shift, dt, 3, checkpoint_every=2)
+        (0.5 * (u.double() ** 2).sum() + (w * r.double()).sum()).backward()
+        grads.append((u0.grad.double().cpu().numpy(), r0.grad.double().cpu().numpy()))
+    (gu, gr), (gu0, gr0) = grads
+    scale = np.abs(gu).max() + np.abs(gr).max()
+    assert np.abs(gu - gu0).max() <= 1e-6 * scale and np.abs(gr - gr0).max() <= 1e-6 * scale
+    base = G.to_np(st)
+    wn = w.cpu().numpy()
+
+    def L(eps, du, dr):
+        s = G.to_f64(dict(base))
+        s["u"] = s["u"] + eps * du
+        s["v"] = s["u"].copy()
+        s["r"] = np.mod(s["r"] + eps * dr, box)
+        out = _oracle_rollout(s, solver, box, dt, 3)
+        assert np.ptp(out["T"]) > 0.1 and np.isfinite(out["T"]).all()   # a live temperature
+        return 0.5 * float((out["u"] ** 2).sum()) + float((wn * out["r"]).sum())
+
+    eps = 1e-6
+    norm = np.sqrt((gu ** 2).sum() + (gr ** 2).sum())
+    du, dr = gu / norm, gr / norm
+    fd = (L(eps, du, dr) - L(-eps, du, dr)) / (2 * eps)
+    ad = float((gu * du).sum() + (gr * dr).sum())
+    assert abs(ad - fd) <= 1e-3 * abs(fd), (ad, fd)
+
+
 def test_checkpoint_interval_does_not_change_the_gradient():
     cfg, box, st, solver, fn, shift, gen = _setup()
     n = st["r"].shape[0]
