@@ -249,7 +249,8 @@ typedef struct {
  * cotangent is not formed); density summation or continuity (solver.py:24-30, without
  * renormalisation); with or without artificial viscosity (solver.py:404-428)
  * and the generalized wall boundary condition (bc_trick, solver.py:431-530)
- * for no-slip walls; QSK or WC2K; with prm->tvf_stress the
+ * for no-slip or free-slip walls (free slip needs next->nw); QSK or WC2K;
+ * with prm->tvf_stress the
  * transport-velocity step
  * (solver.py:199-256: the A = rho u (v - u) stress and the background-pressure
  * dvdt), whose v' = u' + tvf_coef * dvdt (tvf_coef = tvf dt / 2, 0 without

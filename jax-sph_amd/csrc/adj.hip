@@ -34,7 +34,8 @@
 // gathers over the pair list of r' (two sweeps: the second needs the density
 // cotangent of the neighbours), with no atomics, like the forward sweeps.
 //
-// Generalized wall boundary condition (solver.py:431-530, no-slip, no heat):
+// Generalized wall boundary condition (solver.py:431-530; for free-slip walls
+// alpha and beta below take only their normal parts):
 // before the force, every wall particle i takes Shepard averages over its
 // fluid neighbours j (w_ij = W(d_ij), S_i = sum_j w_ij + eps):
 //   u^_i = 2 u_i - sum_j w_ij u_j / S_i   (v^ likewise)
@@ -282,8 +283,17 @@ __global__ __launch_bounds__(kAB) void k_adj_walls_prep(AdjArgs aa, float4* __re
   const float4 A0 = ab0[k];
   const float pbar = A0.w / eos.dp_drho(a.R.U2[k].w);  // rho^ = R(p^): R' = 1 / P'(rho^)
   const V3 vbh = TVF ? v3(ab2[k]) : V3{0.f, 0.f, 0.f};
-  wf0[k] = f4(inv * v3(A0), inv * pbar);
-  wf1[k] = f4(inv * vbh, phat);
+  // free-slip walls take only the normal part of the average (u^ = 2 u - n n^T
+  // ubar_w with n the wall normal, solver.py:505-510): the fluid side sees the
+  // projected cotangents
+  V3 ubp = v3(A0), vbp = vbh;
+  if (p.free_slip) {
+    const V3 nn = v3(a.R.NW[k]);
+    ubp = dot<DIM>(nn, ubp) * nn;
+    vbp = dot<DIM>(nn, vbp) * nn;
+  }
+  wf0[k] = f4(inv * ubp, inv * pbar);
+  wf1[k] = f4(inv * vbp, phat);
   wf2[k] = f4(inv * A, 0.0f);
   wf3[k] = f4(inv * B, 0.0f);
   ab0[k] = f4(2.0f * v3(A0), 0.0f);
@@ -340,17 +350,18 @@ __global__ __launch_bounds__(kAB) void k_adj_walls(AdjArgs aa, float4* __restric
     const float q = d2 * inv_d * p.one_over_h;
     const float w = K.w(q), wd = K.dw(q);
     if (fluid_k) {
-      // wall neighbour j: r_jk = -dr
+      // wall neighbour j: r_jk = -dr; its pressure extrapolation uses its own g
       const float4 G0 = wf0[j], G1 = wf1[j];
       const V3 al = v3(G0), be = v3(G1);
-      const float ga = G0.w, gr = -dot<DIM>(g, dr);
+      const V3 gj = g_ext_of<DIM>(a, a.perm[j]);
+      const float ga = G0.w, gr = -dot<DIM>(gj, dr);
       ub = ub - w * al;
       if (TVF) vb = vb - w * be;
       pacc += w * ga;
       racc += w * gr * ga;
       float L = -dot<DIM>(al, uk - v3(wf2[j])) + ga * (pk + rhok * gr - G1.w);
       if (TVF) L -= dot<DIM>(be, vk - v3(wf3[j]));
-      rb = rb + (L * wd * inv_d) * dr - (ga * rhok * w) * g;
+      rb = rb + (L * wd * inv_d) * dr - (ga * rhok * w) * gj;
     } else {
       // fluid neighbour j: r_kj = dr
       const float4 Uj = a.R.U1[j];
@@ -580,11 +591,14 @@ extern "C" int sph_step_vjp(sph_ws* ws, int bank, const sph_solver_params* prm, 
     set_error("sph_step_vjp: the r, u and a cotangent arrays are required");
     return SPH_ERR_ARG;
   }
-  if (prm->solver != 0 || (prm->bc_trick && prm->free_slip) ||
-      (prm->density_evolution && prm->rho_renorm)) {
+  if (prm->solver != 0 || (prm->density_evolution && prm->rho_renorm)) {
     set_error("sph_step_vjp: only the standard SPH solver is differentiated: "
               "density summation or continuity (without renormalisation), with or without "
-              "the wall boundary condition for no-slip walls and artificial viscosity");
+              "the wall boundary condition and artificial viscosity");
+    return SPH_ERR_ARG;
+  }
+  if (prm->bc_trick && prm->free_slip && !next->nw) {
+    set_error("sph_step_vjp: free-slip walls need the wall normals nw");
     return SPH_ERR_ARG;
   }
   if (prm->kernel != SPH_KERNEL_QSK && prm->kernel != SPH_KERNEL_WC2K) {

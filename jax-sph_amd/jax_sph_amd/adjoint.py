@@ -23,8 +23,9 @@ and the background-pressure acceleration, solver.py:199-256), with or without
 artificial viscosity (solver.py:404-428); density summation or density
 continuity (``density_evolution``, solver.py:24-30; the density is then
 carried from step to step, and checkpointed; no renormalisation), each with
-or without the generalized wall boundary condition for no-slip walls
-(``is_bc_trick``, solver.py:431-530).  The dam break as its case file runs
+or without the generalized wall boundary condition for no-slip or free-slip
+walls (``is_bc_trick``, solver.py:431-530; free-slip walls at rest, whose
+normals are those of the state).  The dam break as its case file runs
 it (continuity, wall condition, artificial viscosity, gravity) is covered.
 Heat conduction may be on: the temperature enters neither the EOS nor the
 forces (solver.py:832-850), so the outputs u and r do not depend on it.  Wall
@@ -45,20 +46,23 @@ from . import _lib
 from .integrator import _bc_spec, si_euler
 from .jax_md.partition import NeighborList
 from .solver import WCSPH
+from .utils import Tag
 
 _f = np.float32
 
 
-def _check(solver: WCSPH, tvf: float, bc_fn=None, graph: bool = False):
+def _check(solver: WCSPH, tvf: float, bc_fn=None, graph: bool = False, state=None):
     why = []
+    if solver.is_bc_trick and solver.is_free_slip and state is not None and \
+            bool((torch.as_tensor(state["tag"]) == int(Tag.MOVING_WALL)).any()):
+        why.append("free-slip moving walls (their normals follow them, case_setup.py:208-218) "
+                   "are not differentiated")
     if bc_fn is not None and _bc_spec(bc_fn) is None:
         why.append("bc_fn must be a case boundary hook (case_setup.CaseBC)")
     if solver.solver != "SPH":
         why.append("solver must be 'SPH'")
     if solver.is_rho_evol and solver.is_rho_renorm:
         why.append("density renormalisation is not differentiated")
-    if solver.is_bc_trick and solver.is_free_slip:
-        why.append("free-slip walls are not differentiated")
     if solver._kernel_fn.name not in ("QSK", "WC2K"):
         why.append("kernel must be QSK or WC2K")
     if solver.g_const is None and graph:
@@ -299,7 +303,7 @@ class GraphRollout:
                  dt: float, n_steps: int, tvf: float = 0.0, checkpoint_every: int = 10,
                  workspace_sets: Optional[int] = None, side_streams: int = 3, bc_fn=None,
                  list_margin: float = 1.5):
-        _check(solver, tvf, bc_fn, graph=True)
+        _check(solver, tvf, bc_fn, graph=True, state=state)
         if getattr(getattr(neighbor_fn, "search", None), "skin", 0.0):
             raise ValueError("the differentiable rollout needs a neighbor_list with skin=0")
         self.k = int(checkpoint_every)
@@ -540,7 +544,7 @@ def rollout(state: Dict[str, torch.Tensor], solver: WCSPH, neighbor_fn, shift_fn
     ``neighbor_fn`` must build exact lists (``skin=0``), so the backward pass
     can recompute the forward trajectory exactly.  ``bc_fn`` is the case's
     boundary hook (a ``case_setup.CaseBC``), applied after every step."""
-    _check(solver, tvf, bc_fn)
+    _check(solver, tvf, bc_fn, state=state)
     if getattr(getattr(neighbor_fn, "search", None), "skin", 0.0):
         raise ValueError("the differentiable rollout needs a neighbor_list with skin=0")
     runner = _Runner(solver, neighbor_fn, shift_fn, dt, n_steps, checkpoint_every, state, tvf,

@@ -618,6 +618,78 @@ def test_artificial_viscosity_rollout_gradient_matches_oracle_fd(case, tvf, n_st
             assert abs(fd_no - fd) > 0.01 * abs(fd), (fd, fd_no)
 
 
+@pytest.mark.parametrize("free,n_steps,k", [(False, 2, 1), (True, 2, 1), (True, 4, 2)])
+def test_channel_walls_gradient_matches_oracle_fd(free, n_steps, k):
+    """The Poiseuille channel (dx = 0.05, walls at rest, fluid velocities
+    perturbed) with no-slip and with free-slip walls (solver.py:505-510: the
+    walls take only the normal part of the Shepard average of their fluid
+    neighbours' velocity, so the cotangents they pass back are projected on
+    the normal).  Its body force is switched on between the walls, so the
+    walls' pressure extrapolation uses the walls' own g (zero), not their
+    fluid neighbours'.  The gradient of <w, u> after n steps in the initial
+    positions along position directions (the dynamics alone), against float64
+    oracle central differences to 1e-5 (two and four steps leave the float32
+    rollout's rounding far below that: measured 2e-8 to 1e-7); with
+    free-slip walls the differences with no-slip walls differ by more than
+    1e-4 (measured 0.25%), ten times that bound.  The cavity's free-slip
+    lid, whose normals follow it, raises."""
+    ov = {"case": {"dx": 0.05}, "solver": {"free_slip": free}}
+    cfg, box, state, g, bc, eos, disp, shift, solver = G.setup("pf", ov)
+    assert solver.is_bc_trick and solver.is_free_slip == free and solver.g_const is None
+    if free:
+        assert float(state["nw"].abs().sum()) > 0
+    noslip = G.setup("pf", {"case": {"dx": 0.05}})[-1]
+    dt = cfg.solver.dt
+    n = state["r"].shape[0]
+    gen = torch.Generator().manual_seed(12)
+    fluid_t = (state["tag"] == 0)[:, None]
+    st = dict(state)
+    st["u"] = torch.where(fluid_t, state["u"] + 0.2 * torch.randn(n, 2, generator=gen).cuda(),
+                          state["u"])
+    st["v"] = st["u"].clone()
+    fn = partition.neighbor_list(disp, box, r_cutoff=solver._kernel_fn.cutoff,
+                                 capacity_multiplier=1.25, mask_self=False,
+                                 format=partition.Sparse, skin=0.0)
+    w = torch.randn(n, 2, generator=gen).cuda().double()
+    r0 = st["r"].clone().requires_grad_(True)
+    s_in = dict(st)
+    s_in["r"] = r0
+    u, r, _ = adjoint.rollout(s_in, solver, fn, shift, dt, n_steps, checkpoint_every=k,
+                              bc_fn=bc)
+    ((w * u.double()).sum() + 0.0 * r.sum()).backward()
+    gr = r0.grad.double().cpu().numpy()
+    assert np.isfinite(gr).all() and np.abs(gr).max() > 0
+    base = G.to_np(st)
+    wn = w.cpu().numpy()
+
+    def L(eps, dr, sv=solver):
+        s = G.to_f64(dict(base))
+        s["r"] = np.mod(s["r"] + eps * dr, box)
+        out = _oracle_rollout(s, sv, box, dt, n_steps, bc_fn=bc)
+        return float((wn * out["u"]).sum())
+
+    rng = np.random.default_rng(13)
+    eps = 1e-6
+    dirs = [gr / np.sqrt((gr ** 2).sum())] + [rng.standard_normal(gr.shape) for _ in range(2)]
+    for trial, dr in enumerate(dirs):
+        fd = (L(eps, dr) - L(-eps, dr)) / (2 * eps)
+        ad = float((gr * dr).sum())
+        scale = abs(fd) if trial == 0 else float(np.abs(gr * dr).sum())
+        print(f"free slip {free} n={n_steps} trial {trial}: ad {ad:.9g} fd {fd:.9g} "
+              f"scale {scale:.6g}", flush=True)
+        assert abs(ad - fd) <= 1e-5 * scale, (trial, ad, fd, scale)
+        if trial == 0 and free:
+            fd_ns = (L(eps, dr, noslip) - L(-eps, dr, noslip)) / (2 * eps)
+            print(f"  no-slip walls: fd {fd_ns:.9g}", flush=True)
+            assert abs(fd_ns - fd) > 1e-4 * abs(fd), (fd, fd_ns)
+    lov = {"case": {"dx": 0.05}, "solver": {"free_slip": True}}
+    lcfg, lbox, lstate, _, lbc, _, ldisp, lshift, lsolver = G.setup("ldc", lov)
+    lfn = partition.neighbor_list(ldisp, lbox, r_cutoff=lsolver._kernel_fn.cutoff,
+                                  mask_self=False, format=partition.Sparse, skin=0.0)
+    with pytest.raises(NotImplementedError, match="moving walls"):
+        adjoint.rollout(lstate, lsolver, lfn, lshift, lcfg.solver.dt, 2, bc_fn=lbc)
+
+
 def test_heat_conduction_leaves_the_velocity_and_position_gradient():
     """With heat conduction on (solver.py:832-850) the temperature evolves
     beside the flow but enters neither the EOS nor the forces: the rollout's
