@@ -16,8 +16,8 @@ positions.  :func:`rollout` wraps an ``n_steps`` rollout in a
 Memory is O(n_steps / k + k) states instead of O(n_steps).
 
 Differentiated configuration (other settings raise ``NotImplementedError``):
-standard SPH (``solver="SPH"``), Quintic or Wendland-C2 kernel, a constant body force or (with density summation, through
-:func:`rollout`) a position-dependent one, with or without the
+standard SPH (``solver="SPH"``), Quintic or Wendland-C2 kernel, a constant body force or (through :func:`rollout`) a
+position-dependent one, with or without the
 transport-velocity formulation (any ``tvf``: the stress ``A = rho u (v - u)``
 and the background-pressure acceleration, solver.py:199-256), with or without
 artificial viscosity (solver.py:404-428); density summation or density
@@ -68,8 +68,6 @@ def _check(solver: WCSPH, tvf: float, bc_fn=None, graph: bool = False, state=Non
     if solver.g_const is None and graph:
         why.append("the graph-replayed rollout needs a constant body force (rollout() "
                    "differentiates a position-dependent one)")
-    if solver.g_const is None and solver.is_rho_evol:
-        why.append("a position-dependent body force is differentiated with density summation")
     if why:
         raise NotImplementedError("differentiable rollout: " + "; ".join(why))
 
