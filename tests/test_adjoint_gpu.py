@@ -693,6 +693,69 @@ def test_channel_walls_gradient_matches_oracle_fd(free, n_steps, k):
         adjoint.rollout(lstate, lsolver, lfn, lshift, lcfg.solver.dt, 2, bc_fn=lbc)
 
 
+@pytest.mark.parametrize("tvf", [0.0, 1.0])
+def test_wall_bc_3d_gradient_matches_oracle_fd(tvf):
+    """The 3D lid-driven cavity (dx = 0.05, periodic in z, 6,760 particles)
+    with the wall boundary condition and the hook: the 3D wall sweeps of the
+    transpose.  Fluid velocities perturbed; the gradient of <w, u> after 2
+    steps in the initial positions along position directions, against
+    float64 oracle central differences to 1e-5; the differences without the
+    wall boundary condition differ by more than 1e-4."""
+    ov = {"case": {"dim": 3, "dx": 0.05, "viscosity": 0.1},
+          "solver": {"t_end": 1.0, "tvf": tvf}}
+    cfg, box, state, g, bc, eos, disp, shift, solver = G.setup("ldc", ov)
+    assert solver.is_bc_trick and state["r"].shape[1] == 3
+    nobc = G.setup("ldc", {**ov, "solver": {**ov["solver"], "is_bc_trick": False}})[-1]
+    dt = cfg.solver.dt
+    n = state["r"].shape[0]
+    gen = torch.Generator().manual_seed(21)
+    fluid_t = (state["tag"] == 0)[:, None]
+    st = dict(state)
+    st["u"] = torch.where(fluid_t, state["u"] + 0.2 * torch.randn(n, 3, generator=gen).cuda(),
+                          state["u"])
+    st["v"] = st["u"].clone()
+    if tvf:   # a transport acceleration, so the stress terms act from the first step
+        st["dvdt"] = torch.where(fluid_t, torch.randn(n, 3, generator=gen).cuda(),
+                                 torch.zeros_like(st["u"]))
+    # the fluid sits n_walls dx up in z, past the z-periodic box (ldc.py:85): wrap it
+    st["r"] = torch.remainder(st["r"], torch.as_tensor(np.asarray(box, np.float32)).cuda())
+    fn = partition.neighbor_list(disp, box, r_cutoff=solver._kernel_fn.cutoff,
+                                 capacity_multiplier=1.25, mask_self=False,
+                                 format=partition.Sparse, skin=0.0)
+    w = torch.randn(n, 3, generator=gen).cuda().double()
+    r0 = st["r"].clone().requires_grad_(True)
+    s_in = dict(st)
+    s_in["r"] = r0
+    u, r, _ = adjoint.rollout(s_in, solver, fn, shift, dt, 2, tvf=tvf, checkpoint_every=1,
+                              bc_fn=bc)
+    ((w * u.double()).sum() + 0.0 * r.sum()).backward()
+    gr = r0.grad.double().cpu().numpy()
+    assert np.isfinite(gr).all() and np.abs(gr).max() > 0
+    base = G.to_np(st)
+    wn = w.cpu().numpy()
+
+    def L(eps, dr, sv=solver):
+        s = G.to_f64(dict(base))
+        s["r"] = np.mod(s["r"] + eps * dr, box)
+        out = _oracle_rollout(s, sv, box, dt, 2, tvf=tvf, bc_fn=bc)
+        return float((wn * out["u"]).sum())
+
+    rng = np.random.default_rng(22)
+    eps = 1e-6
+    dirs = [gr / np.sqrt((gr ** 2).sum()), rng.standard_normal(gr.shape)]
+    for trial, dr in enumerate(dirs):
+        fd = (L(eps, dr) - L(-eps, dr)) / (2 * eps)
+        ad = float((gr * dr).sum())
+        scale = abs(fd) if trial == 0 else float(np.abs(gr * dr).sum())
+        print(f"3D walls tvf={tvf} trial {trial}: ad {ad:.9g} fd {fd:.9g} scale {scale:.6g}",
+              flush=True)
+        assert abs(ad - fd) <= 1e-5 * scale, (trial, ad, fd, scale)
+        if trial == 0:
+            fd_nb = (L(eps, dr, nobc) - L(-eps, dr, nobc)) / (2 * eps)
+            print(f"  without the wall boundary condition: fd {fd_nb:.9g}", flush=True)
+            assert abs(fd_nb - fd) > 1e-4 * abs(fd), (fd, fd_nb)
+
+
 def test_heat_conduction_leaves_the_velocity_and_position_gradient():
     """With heat conduction on (solver.py:832-850) the temperature evolves
     beside the flow but enters neither the EOS nor the forces: the rollout's
