@@ -38,6 +38,8 @@ STATE = ("r", "u", "v", "rho")
     ("ldc", {"case": {"dx": 0.04}}),
     # 3D cavity (periodic in z): the wall boundary condition's 3D sweeps
     ("ldc", {"case": {"dim": 3, "dx": 0.05}}),
+    ("ldc", {"case": {"dim": 3, "dx": 0.05}, "solver": {"name": "RIE"},
+             "kernel": {"name": "WC2K"}}),
     ("db", {"case": {"dx": 0.04}, "solver": {"name": "RIE"}, "kernel": {"name": "WC2K"}}),
     ("ldc", {"case": {"dx": 0.04}, "solver": {"name": "DELTA", "density_evolution": True,
                                               "density_renormalize": True}}),
