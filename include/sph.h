@@ -244,7 +244,9 @@ typedef struct {
   float *r, *u, *a, *b, *rho;
 } sph_cotangent_out;
 /* Vector-Jacobian product of one si_euler step (integrator.py:22-56), the
- * transpose that jax.grad builds for the reference's step.  Standard SPH
+ * transpose that jax.grad builds for the reference's step.  The Riemann
+ * solver (solver.py:316-401) with density summation, no bc_trick and no
+ * tvf_stress; or standard SPH
  * (heat conduction may be on: u', r' do not depend on the temperature, whose
  * cotangent is not formed); density summation or continuity (solver.py:24-30, without
  * renormalisation); with or without artificial viscosity (solver.py:404-428)

@@ -1,8 +1,9 @@
 // Hand-written adjoint (vector-Jacobian product) of one si_euler step on gfx950.
 //
 // The reference differentiates its step with jax.grad (reverse mode through
-// XLA).  Here the transpose of the step is written out.  Scope: the standard
-// SPH solver, density summation or continuity (sweep C), with or
+// XLA).  Here the transpose of the step is written out.  Scope: the Riemann
+// solver with density summation, no walls and tvf = 0 (k_adj_forces_rie), and
+// the standard SPH solver, density summation or continuity (sweep C), with or
 // without the transport-velocity formulation (tvf), artificial viscosity
 // (k_adj_forces) and the generalized wall boundary condition for no-slip
 // walls (below).  Heat conduction (solver.py:832-850) may be on: the
@@ -230,6 +231,98 @@ __global__ __launch_bounds__(kAB) void k_adj_forces(AdjArgs aa, float4* __restri
   }
   ab1[k] = f4(rb, 0.0f);
   if (TVF) ab2[k] = f4(vb, 0.0f);
+}
+
+// Sweep A of the Riemann solver (solver.py:316-401, 574-588) for fluid pairs
+// with density summation, no wall condition, no transport velocity.  With
+// e = r_kj / d, D = u_L - u_R = -(u_k - u_j) . e, the limiter
+// beta = min(eta_lim max(D, 0), c_ref) (c_ref when eta_lim = -1),
+// P* = (p_k + p_j)/2 + (rho_k + rho_j)/4 D beta and K = 2 m_k m_j W'(d) /
+// (rho_k rho_j), the pair force times m_k is F_kj = K (-P* e + eta_kj du / d)
+// (the stress term A vanishes: v = u there), antisymmetric, so as for the
+// standard solver <abar', a'> = 1/2 sum_kj S_kj with S_kj = F_kj . l_kj
+// symmetric, l_kj = abar'_k / m_k - abar'_j / m_j, and particle k's
+// derivatives are sum_j dS_kj / dx_k.  AV: as in k_adj_forces.
+template <int DIM, int KIND, bool AV>
+__global__ __launch_bounds__(kAB) void k_adj_forces_rie(AdjArgs aa, float4* __restrict__ ab0,
+                                                        float4* __restrict__ ab1) {
+  constexpr int TPP = tpp_of(DIM);
+  const SweepArgs& a = aa.s;
+  const int tid = logical_block(a.remap) * blockDim.x + threadIdx.x;
+  const int k = tid / TPP, h = tid % TPP;
+  if (k >= a.n) return;  // whole lane groups leave together
+  const sph_solver_params& p = a.prm;
+  const KernelEval<KIND, DIM> K{p.kernel, p.sigma, p.one_over_h};
+  const Eos eos = make_eos(p);
+  const float4 Pk = a.R.P[k], Uk = a.R.U1[k], Qk = a.R.Q1[k], Lk4 = aa.la[k];
+  const float mk = Pk.w, rhok = Uk.w, pk = Qk.x, etak = Qk.y;
+  const float dPk = eos.dp_drho(rhok);
+  const float inv_mk = 1.0f / mk;
+  const V3 uk = v3(Uk), lk = v3(Lk4);
+  const bool fluid = (int)Qk.z == kFluid;
+  const float avK = AV ? p.artificial_alpha * p.av_h * 10.0f : 0.0f;
+  const float av_h2 = AV ? 0.01f * p.av_h * p.av_h : 0.0f;
+  float rhob = 0.0f;
+  V3 ub = {0.f, 0.f, 0.f}, rb = {0.f, 0.f, 0.f};
+  for_each_pair(a, k, h, [&](const int j) {
+    const float4 Pj = a.R.P[j];
+    const V3 dr = min_image<DIM>(Pk, Pj, a.geo);
+    const float d2 = dist2<DIM>(dr.x, dr.y, dr.z);
+    if (d2 == 0.0f) return;
+    if (!in_cutoff<DIM>(a, d2, Pk, Pj)) return;
+    const float4 Uj = a.R.U1[j], Qj = a.R.Q1[j], Lj4 = aa.la[j];
+    const float mj = Pj.w, rhoj = Uj.w, pj = Qj.x, etaj = Qj.y;
+    const float inv = rsq_fast(d2);
+    const float d = d2 * inv;
+    const float q = d * p.one_over_h;
+    const float w1 = K.dw(q), w2 = K.d2w(q);
+    const V3 e = inv * dr;
+    const V3 du = uk - v3(Uj);
+    const float s_ = dot<DIM>(du, e);
+    const float D = -s_;
+    float beta = p.c_ref, dbeta = 0.0f;
+    if (p.eta_limiter != -1.0f) {
+      const float t = p.eta_limiter * fmaxf(D, 0.0f);
+      beta = fminf(t, p.c_ref);
+      dbeta = (D > 0.0f && t < p.c_ref) ? p.eta_limiter : 0.0f;
+    }
+    const float rho_avg = 0.5f * (rhok + rhoj);
+    const float Ps = 0.5f * (pk + pj) + 0.5f * rho_avg * D * beta;
+    const float PsD = 0.5f * rho_avg * (beta + D * dbeta);   // d P* / d D
+    const float etakj = 2.0f * etak * etaj * rcp_fast(etak + etaj + kEps);
+    const V3 l = inv_mk * lk - rcp_fast(mj) * v3(Lj4);
+    const float el = dot<DIM>(e, l), ul = dot<DIM>(du, l);
+    const float k0 = 2.0f * mk * mj * rcp_fast(rhok * rhoj);
+    const float Kw = k0 * w1;
+    const float Q = -Ps * el + etakj * ul * inv;
+    const float S = Kw * Q;
+    ub = ub + Kw * ((el * PsD) * e + (etakj * inv) * l);
+    rhob += -S / rhok - Kw * el * (0.5f * dPk + 0.25f * D * beta);
+    const V3 dQ = ((el * PsD) * inv) * (du - s_ * e) - (Ps * inv) * (l - el * e) -
+                  (etakj * ul * inv * inv) * e;
+    rb = rb + (k0 * w2 * Q) * e + Kw * dQ;
+    if (AV && fluid && (int)Qj.z == kFluid) {
+      const V3 lkj = mj * lk - mk * v3(Lj4);
+      const float A0 = w1 * inv, Ad0 = (w2 - w1 * inv) * inv;
+      const float irs = rcp_fast(rhok + rhoj);
+      const float Dd = d2 + av_h2;
+      const float iD = rcp_fast(Dd);
+      const float irab = 2.0f * irs;
+      const float Kphi = avK * A0 * irab * iD;
+      const float eu = dot<DIM>(du, dr);
+      const float sl = dot<DIM>(dr, lkj);
+      const float dphi = (Ad0 * Dd - 2.0f * A0 * d) * irab * iD * iD * inv;
+      ub = ub + (Kphi * sl) * dr;
+      rhob -= Kphi * eu * sl * irs;
+      rb = rb + (avK * dphi * eu * sl) * dr + Kphi * (sl * du + eu * lkj);
+    }
+  });
+  rhob = lane_sum<TPP>(rhob);
+  ub = lane_sum<TPP>(ub);
+  rb = lane_sum<TPP>(rb);
+  if (h != 0) return;
+  ab0[k] = f4(ub, fluid ? mk * rhob : 0.0f);
+  ab1[k] = f4(rb, 0.0f);
 }
 
 // Sweep W1, wall particles: their Shepard sums recomputed as the forward
@@ -526,9 +619,15 @@ int launch_adjoint(sph_ws* ws, AdjArgs& a, bool records_ready, hipStream_t st) {
   }
   SPH_LAUNCH(ws, "adj_gather", st,
              (k_adj_gather<DIM, TVF><<<(unsigned)cdivl(n, kAB), kAB, 0, st>>>(a)));
-  SPH_LAUNCH(ws, "adj_forces", st,
-             (k_adj_forces<DIM, KIND, TVF, WALLS, CONT, AV><<<grid, kAB, 0, st>>>(
-                 a, ws->AB0, ws->AB1, ws->AB2)));
+  if (a.s.prm.solver == 1) {
+    // the Riemann solver: fluid pairs, summation, no wall condition, tvf = 0
+    SPH_LAUNCH(ws, "adj_forces", st,
+               (k_adj_forces_rie<DIM, KIND, AV><<<grid, kAB, 0, st>>>(a, ws->AB0, ws->AB1)));
+  } else {
+    SPH_LAUNCH(ws, "adj_forces", st,
+               (k_adj_forces<DIM, KIND, TVF, WALLS, CONT, AV><<<grid, kAB, 0, st>>>(
+                   a, ws->AB0, ws->AB1, ws->AB2)));
+  }
   if (WALLS) {
     // the delta-SPH records hold the per-wall factors (no delta-SPH here)
     SPH_LAUNCH(ws, "adj_walls_prep", st,
@@ -591,10 +690,16 @@ extern "C" int sph_step_vjp(sph_ws* ws, int bank, const sph_solver_params* prm, 
     set_error("sph_step_vjp: the r, u and a cotangent arrays are required");
     return SPH_ERR_ARG;
   }
-  if (prm->solver != 0 || (prm->density_evolution && prm->rho_renorm)) {
-    set_error("sph_step_vjp: only the standard SPH solver is differentiated: "
-              "density summation or continuity (without renormalisation), with or without "
-              "the wall boundary condition and artificial viscosity");
+  if ((prm->solver != 0 && prm->solver != 1) || (prm->density_evolution && prm->rho_renorm)) {
+    set_error("sph_step_vjp: the standard SPH solver is differentiated with density "
+              "summation or continuity (without renormalisation), with or without the wall "
+              "boundary condition and artificial viscosity; the Riemann solver with "
+              "summation only");
+    return SPH_ERR_ARG;
+  }
+  if (prm->solver == 1 && (prm->bc_trick || prm->density_evolution || prm->tvf_stress)) {
+    set_error("sph_step_vjp: the Riemann solver is differentiated with density summation, "
+              "without the wall boundary condition and without the transport velocity");
     return SPH_ERR_ARG;
   }
   if (prm->bc_trick && prm->free_slip && !next->nw) {

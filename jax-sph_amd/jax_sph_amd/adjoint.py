@@ -16,7 +16,9 @@ positions.  :func:`rollout` wraps an ``n_steps`` rollout in a
 Memory is O(n_steps / k + k) states instead of O(n_steps).
 
 Differentiated configuration (other settings raise ``NotImplementedError``):
-standard SPH (``solver="SPH"``), Quintic or Wendland-C2 kernel, a constant body force or (through :func:`rollout`) a
+standard SPH (``solver="SPH"``) as below, or the Riemann solver
+(``solver="RIE"``, solver.py:316-401) with density summation, no wall
+boundary condition and tvf = 0; Quintic or Wendland-C2 kernel, a constant body force or (through :func:`rollout`) a
 position-dependent one, with or without the
 transport-velocity formulation (any ``tvf``: the stress ``A = rho u (v - u)``
 and the background-pressure acceleration, solver.py:199-256), with or without
@@ -59,8 +61,11 @@ def _check(solver: WCSPH, tvf: float, bc_fn=None, graph: bool = False, state=Non
                    "are not differentiated")
     if bc_fn is not None and _bc_spec(bc_fn) is None:
         why.append("bc_fn must be a case boundary hook (case_setup.CaseBC)")
-    if solver.solver != "SPH":
-        why.append("solver must be 'SPH'")
+    if solver.solver not in ("SPH", "RIE"):
+        why.append("solver must be 'SPH' or 'RIE'")
+    if solver.solver == "RIE" and (solver.is_bc_trick or solver.is_rho_evol or tvf != 0.0):
+        why.append("the Riemann solver is differentiated with density summation, without "
+                   "the wall boundary condition and with tvf = 0")
     if solver.is_rho_evol and solver.is_rho_renorm:
         why.append("density renormalisation is not differentiated")
     if solver._kernel_fn.name not in ("QSK", "WC2K"):
