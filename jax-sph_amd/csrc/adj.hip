@@ -2,7 +2,8 @@
 //
 // The reference differentiates its step with jax.grad (reverse mode through
 // XLA).  Here the transpose of the step is written out.  Scope: the Riemann
-// solver with density summation, no walls and tvf = 0 (k_adj_forces_rie), and
+// solver with density summation and tvf = 0, walls held by the case hook
+// (k_adj_forces_rie, k_adj_rie_walls), and
 // the standard SPH solver, density summation or continuity (sweep C), with or
 // without the transport-velocity formulation (tvf), artificial viscosity
 // (k_adj_forces) and the generalized wall boundary condition for no-slip
@@ -233,19 +234,83 @@ __global__ __launch_bounds__(kAB) void k_adj_forces(AdjArgs aa, float4* __restri
   if (TVF) ab2[k] = f4(vb, 0.0f);
 }
 
-// Sweep A of the Riemann solver (solver.py:316-401, 574-588) for fluid pairs
-// with density summation, no wall condition, no transport velocity.  With
-// e = r_kj / d, D = u_L - u_R = -(u_k - u_j) . e, the limiter
-// beta = min(eta_lim max(D, 0), c_ref) (c_ref when eta_lim = -1),
-// P* = (p_k + p_j)/2 + (rho_k + rho_j)/4 D beta and K = 2 m_k m_j W'(d) /
-// (rho_k rho_j), the pair force times m_k is F_kj = K (-P* e + eta_kj du / d)
-// (the stress term A vanishes: v = u there), antisymmetric, so as for the
-// standard solver <abar', a'> = 1/2 sum_kj S_kj with S_kj = F_kj . l_kj
-// symmetric, l_kj = abar'_k / m_k - abar'_j / m_j, and particle k's
-// derivatives are sum_j dS_kj / dx_k.  AV: as in k_adj_forces.
+// The Riemann limiter (solver.py:574-588) and its derivative in D = u_L - u_R.
+__device__ __forceinline__ void rie_beta(const sph_solver_params& p, float D, float* beta,
+                                         float* dbeta) {
+  *beta = p.c_ref;
+  *dbeta = 0.0f;
+  if (p.eta_limiter != -1.0f) {
+    const float t = p.eta_limiter * fmaxf(D, 0.0f);
+    *beta = fminf(t, p.c_ref);
+    *dbeta = (D > 0.0f && t < p.c_ref) ? p.eta_limiter : 0.0f;
+  }
+}
+
+// Row term of fluid particle k and wall neighbour j in the Riemann
+// acceleration (solver.py:345-388): the wall enters through mirrored states,
+// u_L = -u_k . n_j, u_R = -u_L + 2 u_j . n_j, p_R = p_k - rho_k g_k . r_kj,
+// rho_R = R(p_R), and the viscous velocity u_d = 2 u_j - ut_j (ut_j the wall's
+// Shepard velocity with the wall condition, else u_j).  T = abar_k . f_kj with
+//   f_kj = -c9 P* e + c6 (u_k - u_d),  c9 = 2 m_j W'(d) / (rho_k rho_j),
+//   c6 = c9 eta_kj / d.
+// Returns dT/du_k, dT/drho_k (p_k = P(rho_k) included), dT/dr_kj and c6 (the
+// coefficient of ut_j).  The wall's own u, n, rho and m are constants here.
+template <int DIM>
+__device__ __forceinline__ void rie_wall_row(const sph_solver_params& p, const Eos& eos, V3 ak,
+                                             V3 uk, float rhok, float pk, float etak, V3 gk,
+                                             float mj, float rhoj, float etaj, V3 uj, V3 nj,
+                                             V3 utj, V3 dr, float d, float inv, float w1,
+                                             float w2, V3* dU, float* dRho, V3* dR,
+                                             float* c6o) {
+  const V3 e = inv * dr;
+  const float ae = dot<DIM>(ak, e);
+  const float gr = dot<DIM>(gk, dr);
+  const float pR = pk - rhok * gr;
+  const float rhoR = eos.rho_fn(pR);
+  const float dPR = eos.dp_drho(rhoR);  // P'(rho_R); R'(p_R) = 1 / P'(rho_R)
+  const float D = -2.0f * dot<DIM>(uk + uj, nj);
+  float beta, dbeta;
+  rie_beta(p, D, &beta, &dbeta);
+  const float rho_avg = 0.5f * (rhok + rhoR);
+  const float hDb = 0.5f * D * beta;
+  const float Ps = 0.5f * (pk + pR) + rho_avg * hDb;
+  const float PsD = 0.5f * rho_avg * (beta + D * dbeta);
+  const float etakj = 2.0f * etak * etaj * rcp_fast(etak + etaj + kEps);
+  const float irr = rcp_fast(rhok * rhoj);
+  const float c9 = 2.0f * mj * w1 * irr;
+  const float c6 = c9 * etakj * inv;
+  const V3 vk = uk - (2.0f * uj - utj);
+  const float av = dot<DIM>(ak, vk);
+  const float T = -c9 * Ps * ae + c6 * av;
+  *dU = (2.0f * c9 * ae * PsD) * nj + c6 * ak;
+  const float dPs_dp = 1.0f + 0.5f * hDb / dPR;
+  const float dPs_drho = -0.5f * gr + 0.5f * hDb * (1.0f - gr / dPR) + dPs_dp * eos.dp_drho(rhok);
+  *dRho = -T / rhok - c9 * ae * dPs_drho;
+  const float c9d = 2.0f * mj * w2 * irr;
+  const float c6d = 2.0f * mj * etakj * irr * (w2 * inv - w1 * inv * inv);
+  const V3 dPs_dr = (-0.5f * rhok * (1.0f + hDb / dPR)) * gk;
+  *dR = (-c9d * Ps * ae + c6d * av) * e - (c9 * ae) * dPs_dr - (c9 * Ps * inv) * (ak - ae * e);
+  *c6o = c6;
+}
+
+// Sweep A of the Riemann solver (solver.py:316-401, 574-588) with density
+// summation and tvf = 0.  Fluid pairs: with e = r_kj / d, D = u_L - u_R =
+// -(u_k - u_j) . e, the limiter beta = min(eta_lim max(D, 0), c_ref) (c_ref
+// when eta_lim = -1), P* = (p_k + p_j)/2 + (rho_k + rho_j)/4 D beta and
+// K = 2 m_k m_j W'(d) / (rho_k rho_j), the pair force times m_k is
+// F_kj = K (-P* e + eta_kj du / d) (the stress term A vanishes: v = u there),
+// antisymmetric, so as for the standard solver <abar', a'> = 1/2 sum_kj S_kj
+// with S_kj = F_kj . l_kj symmetric, l_kj = abar'_k / m_k - abar'_j / m_j, and
+// particle k's derivatives are sum_j dS_kj / dx_k.  Fluid-wall pairs: the
+// fluid's row term (rie_wall_row); the walls, held by the case hook, carry no
+// row of their own and gather the position derivative of their fluid
+// neighbours' rows, and (with the wall condition) the cotangent of their
+// Shepard velocity, passed on as wf = (ubar_t / S, S).  AV: as in
+// k_adj_forces.
 template <int DIM, int KIND, bool AV>
 __global__ __launch_bounds__(kAB) void k_adj_forces_rie(AdjArgs aa, float4* __restrict__ ab0,
-                                                        float4* __restrict__ ab1) {
+                                                        float4* __restrict__ ab1,
+                                                        float4* __restrict__ wf) {
   constexpr int TPP = tpp_of(DIM);
   const SweepArgs& a = aa.s;
   const int tid = logical_block(a.remap) * blockDim.x + threadIdx.x;
@@ -260,10 +325,16 @@ __global__ __launch_bounds__(kAB) void k_adj_forces_rie(AdjArgs aa, float4* __re
   const float inv_mk = 1.0f / mk;
   const V3 uk = v3(Uk), lk = v3(Lk4);
   const bool fluid = (int)Qk.z == kFluid;
+  const bool wall_k = is_wall((int)Qk.z);
+  const V3 gk = g_ext_of<DIM>(a, a.perm[k]);
+  const V3 nk = wall_k ? v3(a.R.NW[k]) : V3{0.f, 0.f, 0.f};
+  // the wall's velocity in u_d: its Shepard velocity with the wall condition,
+  // else its own (solver.py:378, 547-552)
+  const V3 utk = wall_k ? (p.bc_trick ? v3(a.R.UT[k]) : uk) : V3{0.f, 0.f, 0.f};
   const float avK = AV ? p.artificial_alpha * p.av_h * 10.0f : 0.0f;
   const float av_h2 = AV ? 0.01f * p.av_h * p.av_h : 0.0f;
-  float rhob = 0.0f;
-  V3 ub = {0.f, 0.f, 0.f}, rb = {0.f, 0.f, 0.f};
+  float rhob = 0.0f, Sw = 0.0f;
+  V3 ub = {0.f, 0.f, 0.f}, rb = {0.f, 0.f, 0.f}, utb = {0.f, 0.f, 0.f};
   for_each_pair(a, k, h, [&](const int j) {
     const float4 Pj = a.R.P[j];
     const V3 dr = min_image<DIM>(Pk, Pj, a.geo);
@@ -271,21 +342,44 @@ __global__ __launch_bounds__(kAB) void k_adj_forces_rie(AdjArgs aa, float4* __re
     if (d2 == 0.0f) return;
     if (!in_cutoff<DIM>(a, d2, Pk, Pj)) return;
     const float4 Uj = a.R.U1[j], Qj = a.R.Q1[j], Lj4 = aa.la[j];
+    const int tag_j = (int)Qj.z;
     const float mj = Pj.w, rhoj = Uj.w, pj = Qj.x, etaj = Qj.y;
     const float inv = rsq_fast(d2);
     const float d = d2 * inv;
     const float q = d * p.one_over_h;
     const float w1 = K.dw(q), w2 = K.d2w(q);
+    if (!fluid) {
+      // a wall (held by the hook): the rows of its fluid neighbours
+      if (!wall_k || tag_j != kFluid) return;
+      V3 dU, dR;
+      float dRho, c6;
+      rie_wall_row<DIM>(p, eos, v3(Lj4), v3(Uj), rhoj, pj, etaj, g_ext_of<DIM>(a, a.perm[j]),
+                        mk, rhok, etak, uk, nk, utk, (-1.0f) * dr, d, inv, w1, w2, &dU, &dRho,
+                        &dR, &c6);
+      rb = rb - dR;  // the row depends on r_jk = r_j - r_k
+      if (p.bc_trick) {
+        utb = utb + c6 * v3(Lj4);
+        Sw += K.w(q);
+      }
+      return;
+    }
+    if (is_wall(tag_j)) {
+      V3 dU, dR;
+      float dRho, c6;
+      const V3 utj = p.bc_trick ? v3(a.R.UT[j]) : v3(Uj);
+      rie_wall_row<DIM>(p, eos, lk, uk, rhok, pk, etak, gk, mj, rhoj, etaj, v3(Uj),
+                        v3(a.R.NW[j]), utj, dr, d, inv, w1, w2, &dU, &dRho, &dR, &c6);
+      ub = ub + dU;
+      rhob += dRho;
+      rb = rb + dR;
+      return;
+    }
     const V3 e = inv * dr;
     const V3 du = uk - v3(Uj);
     const float s_ = dot<DIM>(du, e);
     const float D = -s_;
-    float beta = p.c_ref, dbeta = 0.0f;
-    if (p.eta_limiter != -1.0f) {
-      const float t = p.eta_limiter * fmaxf(D, 0.0f);
-      beta = fminf(t, p.c_ref);
-      dbeta = (D > 0.0f && t < p.c_ref) ? p.eta_limiter : 0.0f;
-    }
+    float beta, dbeta;
+    rie_beta(p, D, &beta, &dbeta);
     const float rho_avg = 0.5f * (rhok + rhoj);
     const float Ps = 0.5f * (pk + pj) + 0.5f * rho_avg * D * beta;
     const float PsD = 0.5f * rho_avg * (beta + D * dbeta);   // d P* / d D
@@ -301,7 +395,7 @@ __global__ __launch_bounds__(kAB) void k_adj_forces_rie(AdjArgs aa, float4* __re
     const V3 dQ = ((el * PsD) * inv) * (du - s_ * e) - (Ps * inv) * (l - el * e) -
                   (etakj * ul * inv * inv) * e;
     rb = rb + (k0 * w2 * Q) * e + Kw * dQ;
-    if (AV && fluid && (int)Qj.z == kFluid) {
+    if (AV && tag_j == kFluid) {
       const V3 lkj = mj * lk - mk * v3(Lj4);
       const float A0 = w1 * inv, Ad0 = (w2 - w1 * inv) * inv;
       const float irs = rcp_fast(rhok + rhoj);
@@ -320,9 +414,66 @@ __global__ __launch_bounds__(kAB) void k_adj_forces_rie(AdjArgs aa, float4* __re
   rhob = lane_sum<TPP>(rhob);
   ub = lane_sum<TPP>(ub);
   rb = lane_sum<TPP>(rb);
+  utb = lane_sum<TPP>(utb);
+  Sw = lane_sum<TPP>(Sw);
   if (h != 0) return;
   ab0[k] = f4(ub, fluid ? mk * rhob : 0.0f);
   ab1[k] = f4(rb, 0.0f);
+  if (wall_k) wf[k] = f4((1.0f / (Sw + kEps)) * utb, Sw);
+}
+
+// Sweep W of the Riemann solver's wall condition (solver.py:547-552): the
+// walls' Shepard velocity ut_j = sum_k w_jk u_k / (S_j + eps) over fluid k;
+// with alpha_j = ubar_t_j / (S_j + eps) (wf), fluid k gathers
+// ubar_k += w_jk alpha_j, and both sides take the position derivative
+// alpha_j . (u_k - ut_j) dw_jk.
+template <int DIM, int KIND>
+__global__ __launch_bounds__(kAB) void k_adj_rie_walls(AdjArgs aa, float4* __restrict__ ab0,
+                                                       float4* __restrict__ ab1,
+                                                       const float4* __restrict__ wf) {
+  constexpr int TPP = tpp_of(DIM);
+  const SweepArgs& a = aa.s;
+  const int tid = logical_block(a.remap) * blockDim.x + threadIdx.x;
+  const int k = tid / TPP, h = tid % TPP;
+  if (k >= a.n) return;
+  const sph_solver_params& p = a.prm;
+  const KernelEval<KIND, DIM> K{p.kernel, p.sigma, p.one_over_h};
+  const int tag_k = (int)a.R.Q1[k].z;
+  const bool fluid_k = tag_k == kFluid;
+  if (!fluid_k && !is_wall(tag_k)) return;
+  const float4 Pk = a.R.P[k];
+  const V3 uk = v3(a.R.U1[k]);
+  const V3 alk = fluid_k ? V3{0.f, 0.f, 0.f} : v3(wf[k]);
+  const V3 utk = fluid_k ? V3{0.f, 0.f, 0.f} : v3(a.R.UT[k]);
+  V3 ub = {0.f, 0.f, 0.f}, rb = {0.f, 0.f, 0.f};
+  for_each_pair(a, k, h, [&](const int j) {
+    const int tag_j = (int)a.R.Q1[j].z;
+    if (fluid_k ? !is_wall(tag_j) : tag_j != kFluid) return;
+    const float4 Pj = a.R.P[j];
+    const V3 dr = min_image<DIM>(Pk, Pj, a.geo);
+    const float d2 = dist2<DIM>(dr.x, dr.y, dr.z);
+    if (d2 == 0.0f) return;
+    if (!in_cutoff<DIM>(a, d2, Pk, Pj)) return;
+    const float inv = rsq_fast(d2);
+    const float q = d2 * inv * p.one_over_h;
+    const float w = K.w(q), wd = K.dw(q);
+    if (fluid_k) {
+      const V3 al = v3(wf[j]);
+      ub = ub + w * al;
+      rb = rb + (dot<DIM>(al, uk - v3(a.R.UT[j])) * wd * inv) * dr;
+    } else {
+      rb = rb + (dot<DIM>(alk, v3(a.R.U1[j]) - utk) * wd * inv) * dr;
+    }
+  });
+  rb = lane_sum<TPP>(rb);
+  if (fluid_k) ub = lane_sum<TPP>(ub);
+  if (h != 0) return;
+  const float4 B1 = ab1[k];
+  ab1[k] = f4(v3(B1) + rb, B1.w);
+  if (fluid_k) {
+    const float4 B0 = ab0[k];
+    ab0[k] = f4(v3(B0) + ub, B0.w);
+  }
 }
 
 // Sweep W1, wall particles: their Shepard sums recomputed as the forward
@@ -608,8 +759,32 @@ __global__ __launch_bounds__(kAB) void k_adj_density(AdjArgs aa, const float4* _
   if (aa.bb_out) store_vec<DIM>(aa.bb_out, i, aa.c_tvf * vbt);
 }
 
+// The Riemann solver with density summation and tvf = 0: the density sweep
+// (its records, and the walls' Shepard velocity with the wall condition),
+// the force transpose, the Shepard velocity's transpose, then sweep B.
+template <int DIM, int KIND, bool AV>
+int launch_adjoint_rie(sph_ws* ws, AdjArgs& a, bool records_ready, hipStream_t st) {
+  const int n = a.s.n;
+  dim3 grid(xcd_grid(cdivl((int64_t)n * tpp_of(DIM), kAB), a.s.remap));
+  if (!records_ready)
+    SPH_LAUNCH(ws, "density", st, (k_density<DIM, KIND, 0><<<grid, kAB, 0, st>>>(a.s)));
+  SPH_LAUNCH(ws, "adj_gather", st,
+             (k_adj_gather<DIM, false><<<(unsigned)cdivl(n, kAB), kAB, 0, st>>>(a)));
+  SPH_LAUNCH(ws, "adj_forces", st,
+             (k_adj_forces_rie<DIM, KIND, AV><<<grid, kAB, 0, st>>>(a, ws->AB0, ws->AB1,
+                                                                     ws->DL0)));
+  if (a.s.prm.bc_trick)
+    SPH_LAUNCH(ws, "adj_rie_walls", st,
+               (k_adj_rie_walls<DIM, KIND><<<grid, kAB, 0, st>>>(a, ws->AB0, ws->AB1, ws->DL0)));
+  SPH_LAUNCH(ws, "adj_density", st,
+             (k_adj_density<DIM, KIND, false><<<grid, kAB, 0, st>>>(a, ws->AB0, ws->AB1,
+                                                                     ws->AB2)));
+  return SPH_OK;
+}
+
 template <int DIM, int KIND, bool TVF, bool WALLS, bool CONT, bool AV>
 int launch_adjoint(sph_ws* ws, AdjArgs& a, bool records_ready, hipStream_t st) {
+  if (a.s.prm.solver == 1) return launch_adjoint_rie<DIM, KIND, AV>(ws, a, records_ready, st);
   const int n = a.s.n;
   dim3 grid(xcd_grid(cdivl((int64_t)n * tpp_of(DIM), kAB), a.s.remap));
   if (!records_ready) {
@@ -619,15 +794,9 @@ int launch_adjoint(sph_ws* ws, AdjArgs& a, bool records_ready, hipStream_t st) {
   }
   SPH_LAUNCH(ws, "adj_gather", st,
              (k_adj_gather<DIM, TVF><<<(unsigned)cdivl(n, kAB), kAB, 0, st>>>(a)));
-  if (a.s.prm.solver == 1) {
-    // the Riemann solver: fluid pairs, summation, no wall condition, tvf = 0
-    SPH_LAUNCH(ws, "adj_forces", st,
-               (k_adj_forces_rie<DIM, KIND, AV><<<grid, kAB, 0, st>>>(a, ws->AB0, ws->AB1)));
-  } else {
-    SPH_LAUNCH(ws, "adj_forces", st,
-               (k_adj_forces<DIM, KIND, TVF, WALLS, CONT, AV><<<grid, kAB, 0, st>>>(
-                   a, ws->AB0, ws->AB1, ws->AB2)));
-  }
+  SPH_LAUNCH(ws, "adj_forces", st,
+             (k_adj_forces<DIM, KIND, TVF, WALLS, CONT, AV><<<grid, kAB, 0, st>>>(
+                 a, ws->AB0, ws->AB1, ws->AB2)));
   if (WALLS) {
     // the delta-SPH records hold the per-wall factors (no delta-SPH here)
     SPH_LAUNCH(ws, "adj_walls_prep", st,
@@ -697,9 +866,9 @@ extern "C" int sph_step_vjp(sph_ws* ws, int bank, const sph_solver_params* prm, 
               "summation only");
     return SPH_ERR_ARG;
   }
-  if (prm->solver == 1 && (prm->bc_trick || prm->density_evolution || prm->tvf_stress)) {
+  if (prm->solver == 1 && (prm->density_evolution || prm->tvf_stress || prm->free_slip)) {
     set_error("sph_step_vjp: the Riemann solver is differentiated with density summation, "
-              "without the wall boundary condition and without the transport velocity");
+              "no-slip walls and without the transport velocity");
     return SPH_ERR_ARG;
   }
   if (prm->bc_trick && prm->free_slip && !next->nw) {

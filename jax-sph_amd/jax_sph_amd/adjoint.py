@@ -17,8 +17,9 @@ Memory is O(n_steps / k + k) states instead of O(n_steps).
 
 Differentiated configuration (other settings raise ``NotImplementedError``):
 standard SPH (``solver="SPH"``) as below, or the Riemann solver
-(``solver="RIE"``, solver.py:316-401) with density summation, no wall
-boundary condition and tvf = 0; Quintic or Wendland-C2 kernel, a constant body force or (through :func:`rollout`) a
+(``solver="RIE"``, solver.py:316-401) with density summation and tvf = 0,
+with or without its no-slip wall condition, its walls held by the case hook;
+Quintic or Wendland-C2 kernel, a constant body force or (through :func:`rollout`) a
 position-dependent one, with or without the
 transport-velocity formulation (any ``tvf``: the stress ``A = rho u (v - u)``
 and the background-pressure acceleration, solver.py:199-256), with or without
@@ -48,7 +49,7 @@ from . import _lib
 from .integrator import _bc_spec, si_euler
 from .jax_md.partition import NeighborList
 from .solver import WCSPH
-from .utils import Tag
+from .utils import Tag, wall_tags
 
 _f = np.float32
 
@@ -63,9 +64,22 @@ def _check(solver: WCSPH, tvf: float, bc_fn=None, graph: bool = False, state=Non
         why.append("bc_fn must be a case boundary hook (case_setup.CaseBC)")
     if solver.solver not in ("SPH", "RIE"):
         why.append("solver must be 'SPH' or 'RIE'")
-    if solver.solver == "RIE" and (solver.is_bc_trick or solver.is_rho_evol or tvf != 0.0):
-        why.append("the Riemann solver is differentiated with density summation, without "
-                   "the wall boundary condition and with tvf = 0")
+    if solver.solver == "RIE" and (solver.is_rho_evol or tvf != 0.0 or
+                                   (solver.is_bc_trick and solver.is_free_slip)):
+        why.append("the Riemann solver is differentiated with density summation, no-slip "
+                   "walls and tvf = 0")
+    if solver.solver == "RIE" and state is not None:
+        # the wall rows of the Riemann acceleration are not transposed: every
+        # wall must be held by the case hook (its accelerations are constants)
+        tag = torch.as_tensor(state["tag"])
+        walls = torch.isin(tag, torch.as_tensor(wall_tags, device=tag.device))
+        kind = getattr(bc_fn, "kind", 0) if bc_fn is not None else 0
+        held = {1: (int(Tag.SOLID_WALL), int(Tag.MOVING_WALL)), 2: (int(Tag.SOLID_WALL),),
+                3: tuple(int(t) for t in wall_tags)}.get(kind, ())
+        if bool(walls.any()) and not bool(
+                torch.isin(tag[walls], torch.as_tensor(held, dtype=tag.dtype,
+                                                       device=tag.device)).all()):
+            why.append("the Riemann solver's walls must be held by the case hook")
     if solver.is_rho_evol and solver.is_rho_renorm:
         why.append("density renormalisation is not differentiated")
     if solver._kernel_fn.name not in ("QSK", "WC2K"):

@@ -831,6 +831,76 @@ def test_riemann_solver_gradient_matches_oracle_fd(kernel, limiter, alpha, n_ste
             assert abs(fd_s - fd) > 0.01 * abs(fd), (fd, fd_s)
 
 
+@pytest.mark.parametrize("case,gwbc,kernel", [("pf", True, "QSK"), ("ldc", True, "WC2K"),
+                                            ("ldc", False, "QSK")])
+def test_riemann_walls_gradient_matches_oracle_fd(case, gwbc, kernel):
+    """The Riemann solver next to walls (solver.py:345-388): a fluid particle
+    sees a wall through mirrored states (u_R from the wall's velocity and
+    normal, p_R = p_L + rho_L g . (-r), rho_R = R(p_R)) and, with the wall
+    condition, the wall's Shepard velocity in the viscous term
+    (solver.py:547-552), which passes cotangents on to the fluid around the
+    wall.  The Poiseuille channel (the reference's own test runs it with this
+    solver, tests/test_pf2d.py) and the lid-driven cavity, dx = 0.05, fluid
+    velocities perturbed, the hook holding the walls; the gradient of <w, u>
+    after 2 steps in the initial positions along position directions,
+    against float64 oracle central differences to 1e-5; with the wall
+    condition, the differences without it differ by more than 1e-4."""
+    ov = {"case": {"dx": 0.05, "viscosity": 0.1} if case == "ldc" else {"dx": 0.05},
+          "kernel": {"name": kernel},
+          "solver": {"name": "RIE", "is_bc_trick": gwbc, "t_end": 1.0}}
+    cfg, box, state, g, bc, eos, disp, shift, solver = G.setup(case, ov)
+    assert solver.solver == "RIE" and solver.is_bc_trick == gwbc and bc.kind == 1
+    plain = G.setup(case, {**ov, "solver": {**ov["solver"], "is_bc_trick": False}})[-1]
+    dt = cfg.solver.dt
+    n = state["r"].shape[0]
+    gen = torch.Generator().manual_seed(41)
+    fluid_t = (state["tag"] == 0)[:, None]
+    st = dict(state)
+    st["u"] = torch.where(fluid_t, state["u"] + 0.2 * torch.randn(n, 2, generator=gen).cuda(),
+                          state["u"])
+    st["v"] = st["u"].clone()
+    fn = partition.neighbor_list(disp, box, r_cutoff=solver._kernel_fn.cutoff,
+                                 capacity_multiplier=1.25, mask_self=False,
+                                 format=partition.Sparse, skin=0.0)
+    w = torch.randn(n, 2, generator=gen).cuda().double()
+    r0 = st["r"].clone().requires_grad_(True)
+    s_in = dict(st)
+    s_in["r"] = r0
+    u, r, _ = adjoint.rollout(s_in, solver, fn, shift, dt, 2, checkpoint_every=1, bc_fn=bc)
+    ((w * u.double()).sum() + 0.0 * r.sum()).backward()
+    gr = r0.grad.double().cpu().numpy()
+    assert np.isfinite(gr).all() and np.abs(gr).max() > 0
+    if solver.g_const is not None:
+        gro = adjoint.GraphRollout(st, solver, fn, shift, dt, 2, checkpoint_every=1, bc_fn=bc)
+        r2 = st["r"].clone().requires_grad_(True)
+        ug, rg = gro(st["u"].clone(), r2)
+        ((w * ug.double()).sum() + 0.0 * rg.sum()).backward()
+        assert torch.equal(ug, u) and torch.equal(r2.grad, r0.grad)
+    base = G.to_np(st)
+    wn = w.cpu().numpy()
+
+    def L(eps, dr, sv=solver):
+        s = G.to_f64(dict(base))
+        s["r"] = np.mod(s["r"] + eps * dr, box)
+        out = _oracle_rollout(s, sv, box, dt, 2, bc_fn=bc)
+        return float((wn * out["u"]).sum())
+
+    rng = np.random.default_rng(42)
+    eps = 1e-6
+    dirs = [gr / np.sqrt((gr ** 2).sum()), rng.standard_normal(gr.shape)]
+    for trial, dr in enumerate(dirs):
+        fd = (L(eps, dr) - L(-eps, dr)) / (2 * eps)
+        ad = float((gr * dr).sum())
+        scale = abs(fd) if trial == 0 else float(np.abs(gr * dr).sum())
+        print(f"RIE walls {case} gwbc={gwbc} trial {trial}: ad {ad:.9g} fd {fd:.9g} "
+              f"scale {scale:.6g}", flush=True)
+        assert abs(ad - fd) <= 1e-5 * scale, (trial, ad, fd, scale)
+        if trial == 0 and gwbc:
+            fd_p = (L(eps, dr, plain) - L(-eps, dr, plain)) / (2 * eps)
+            print(f"  without the wall condition: fd {fd_p:.9g}", flush=True)
+            assert abs(fd_p - fd) > 1e-4 * abs(fd), (fd, fd_p)
+
+
 def test_heat_conduction_leaves_the_velocity_and_position_gradient():
     """With heat conduction on (solver.py:832-850) the temperature evolves
     beside the flow but enters neither the EOS nor the forces: the rollout's
