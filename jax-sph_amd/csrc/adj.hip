@@ -2,8 +2,8 @@
 //
 // The reference differentiates its step with jax.grad (reverse mode through
 // XLA).  Here the transpose of the step is written out.  Scope: the Riemann
-// solver with density summation and tvf = 0, walls held by the case hook
-// (k_adj_forces_rie, k_adj_rie_walls), and
+// solver with density summation or continuity and tvf = 0, walls held by the
+// case hook (k_adj_forces_rie, k_adj_rie_walls, k_adj_rie_continuity), and
 // the standard SPH solver, density summation or continuity (sweep C), with or
 // without the transport-velocity formulation (tvf), artificial viscosity
 // (k_adj_forces) and the generalized wall boundary condition for no-slip
@@ -307,7 +307,7 @@ __device__ __forceinline__ void rie_wall_row(const sph_solver_params& p, const E
 // neighbours' rows, and (with the wall condition) the cotangent of their
 // Shepard velocity, passed on as wf = (ubar_t / S, S).  AV: as in
 // k_adj_forces.
-template <int DIM, int KIND, bool AV>
+template <int DIM, int KIND, bool AV, bool CONT>
 __global__ __launch_bounds__(kAB) void k_adj_forces_rie(AdjArgs aa, float4* __restrict__ ab0,
                                                         float4* __restrict__ ab1,
                                                         float4* __restrict__ wf) {
@@ -417,9 +417,145 @@ __global__ __launch_bounds__(kAB) void k_adj_forces_rie(AdjArgs aa, float4* __re
   utb = lane_sum<TPP>(utb);
   Sw = lane_sum<TPP>(Sw);
   if (h != 0) return;
-  ab0[k] = f4(ub, fluid ? mk * rhob : 0.0f);
+  if (CONT)   // the whole cotangent of rho' (the next step's included)
+    ab0[k] = f4(ub, rhob + (aa.rhob_in ? aa.rhob_in[a.perm[k]] : 0.0f));
+  else
+    ab0[k] = f4(ub, fluid ? mk * rhob : 0.0f);
   ab1[k] = f4(rb, 0.0f);
   if (wall_k) wf[k] = f4((1.0f / (Sw + kEps)) * utb, Sw);
+}
+
+// Row of fluid particle k and wall neighbour j in the Riemann continuity
+// (solver.py:108-164): with mirrored states the bracket of
+// R_kj = (rho_k m_j / rho_j) W'(d) [s + rho_k G / (rho_avg c)], s = (u_k - u_j) . e,
+// G = g_k . r_kj, rho_avg = (rho_k + R(p_k - rho_k G)) / 2.  Returns dR/du_k
+// (a multiple of e: *cu), dR/drho_k, dR/dp_k and dR/dr_kj.
+template <int DIM>
+__device__ __forceinline__ void rie_cont_wall_row(const sph_solver_params& p, const Eos& eos,
+                                                  V3 uk, float rhok, float pk, V3 gk, float mj,
+                                                  float rhoj, V3 uj, V3 dr, float inv, float w1,
+                                                  float w2, float* cu, float* dRho, float* dP,
+                                                  V3* dR) {
+  const V3 e = inv * dr;
+  const V3 du = uk - uj;
+  const float s_ = dot<DIM>(du, e);
+  const float G = dot<DIM>(gk, dr);
+  const float rhoR = eos.rho_fn(pk - rhok * G);
+  const float dPR = eos.dp_drho(rhoR);
+  const float ra = 0.5f * (rhok + rhoR);
+  const float ic = 1.0f / (ra * p.c_ref);
+  const float Y = s_ + rhok * G * ic;
+  const float al = rhok * mj * rcp_fast(rhoj);
+  const float R = al * w1 * Y;
+  const float t2 = rhok * G * ic / ra;  // rho_k G / (rho_avg^2 c)
+  *cu = al * w1;
+  *dRho = R / rhok + al * w1 * (G * ic - t2 * 0.5f * (1.0f - G / dPR));
+  *dP = -al * w1 * t2 * 0.5f / dPR;
+  *dR = (al * w2 * Y) * e +
+        (al * w1) * (inv * (du - s_ * e) + (rhok * ic + t2 * 0.5f * rhok / dPR) * gk);
+}
+
+// Sweep C of the Riemann continuity (solver.py:108-164, 772-793): the
+// transpose of rho'_k = rho_k + dt drhodt_k (fluid k), with
+//   drhodt_k = sum_j (rho_k m_j / rho_j) W'(d) [s + (p_k - p_j) / (rho_avg c)]
+// for fluid j (s = (u'_k - u'_j) . e, rho_avg = (rho_k + rho_j) / 2, u' the
+// kicked velocity, rho and p those of the state the step starts from) and the
+// mirrored row of rie_cont_wall_row for wall j.  B = ab0.w (the cotangent of
+// rho').  Particle k gathers its own rows (times B_k) and its fluid
+// neighbours' rows (times B_j); p = P(rho) of the state turns the pressure
+// cotangent into one of rho.  Walls (rho' = rho) pass B on and gather the
+// position derivative of their fluid neighbours' rows.  Then the kinematic
+// transpose of sweep B.
+template <int DIM, int KIND>
+__global__ __launch_bounds__(kAB) void k_adj_rie_continuity(AdjArgs aa,
+                                                            const float4* __restrict__ ab0,
+                                                            const float4* __restrict__ ab1) {
+  constexpr int TPP = tpp_of(DIM);
+  const SweepArgs& a = aa.s;
+  const int tid = logical_block(a.remap) * blockDim.x + threadIdx.x;
+  const int k = tid / TPP, h = tid % TPP;
+  if (k >= a.n) return;
+  const sph_solver_params& p = a.prm;
+  const KernelEval<KIND, DIM> K{p.kernel, p.sigma, p.one_over_h};
+  const Eos eos = make_eos(p);
+  const float4 Pk = a.R.P[k], Uk = a.R.U0[k], Qk = a.R.Q0[k];
+  const float mk = Pk.w, rhok = Uk.w, pk = Qk.x;
+  const V3 uk = v3(Uk);
+  const bool fluid = (int)Qk.z == kFluid, wall_k = is_wall((int)Qk.z);
+  const V3 gk = g_ext_of<DIM>(a, a.perm[k]);
+  const float Bk = ab0[k].w;
+  float rho_c = 0.f, p_c = 0.f;
+  V3 ub = {0.f, 0.f, 0.f}, rb = {0.f, 0.f, 0.f};
+  for_each_pair(a, k, h, [&](const int j) {
+    const float4 Pj = a.R.P[j];
+    const V3 dr = min_image<DIM>(Pk, Pj, a.geo);
+    const float d2 = dist2<DIM>(dr.x, dr.y, dr.z);
+    if (d2 == 0.0f) return;
+    if (!in_cutoff<DIM>(a, d2, Pk, Pj)) return;
+    const float4 Uj = a.R.U0[j], Qj = a.R.Q0[j];
+    const int tag_j = (int)Qj.z;
+    const float inv = rsq_fast(d2);
+    const float q = d2 * inv * p.one_over_h;
+    const float w1 = K.dw(q), w2 = K.d2w(q);
+    const float Bj = ab0[j].w;
+    if (!fluid) {
+      if (!wall_k || tag_j != kFluid) return;
+      float cu, dRho, dP;
+      V3 dR;
+      rie_cont_wall_row<DIM>(p, eos, v3(Uj), Uj.w, Qj.x, g_ext_of<DIM>(a, a.perm[j]), mk,
+                             rhok, uk, (-1.0f) * dr, inv, w1, w2, &cu, &dRho, &dP, &dR);
+      rb = rb - Bj * dR;
+      return;
+    }
+    if (is_wall(tag_j)) {
+      float cu, dRho, dP;
+      V3 dR;
+      rie_cont_wall_row<DIM>(p, eos, uk, rhok, pk, gk, Pj.w, Uj.w, v3(Uj), dr, inv, w1, w2,
+                             &cu, &dRho, &dP, &dR);
+      ub = ub + (Bk * cu * inv) * dr;
+      rho_c += Bk * dRho;
+      p_c += Bk * dP;
+      rb = rb + Bk * dR;
+      return;
+    }
+    if (tag_j != kFluid) return;
+    const V3 e = inv * dr;
+    const V3 du = uk - v3(Uj);
+    const float s_ = dot<DIM>(du, e);
+    const float dp_ = pk - Qj.x;
+    const float ra = 0.5f * (rhok + Uj.w);
+    const float ic = 1.0f / (ra * p.c_ref);
+    const float X = s_ + dp_ * ic, Xn = s_ - dp_ * ic;
+    const float al = rhok * Pj.w * rcp_fast(Uj.w);     // own row rho_k m_j / rho_j
+    const float be = Uj.w * mk * rcp_fast(rhok);       // neighbour's row rho_j m_k / rho_k
+    const float ca = Bk * al, cb = Bj * be;
+    ub = ub + ((ca + cb) * w1) * e;
+    rho_c += ca * w1 * X / rhok - cb * w1 * Xn / rhok +
+             (cb - ca) * w1 * dp_ * ic / (2.0f * ra);
+    p_c += (ca - cb) * w1 * ic;
+    rb = rb + (w2 * (ca * X + cb * Xn)) * e + ((ca + cb) * w1 * inv) * (du - s_ * e);
+  });
+  rho_c = lane_sum<TPP>(rho_c);
+  p_c = lane_sum<TPP>(p_c);
+  ub = lane_sum<TPP>(ub);
+  rb = lane_sum<TPP>(rb);
+  if (h != 0) return;
+  const float dt = aa.dt;
+  const int i = a.perm[k];
+  const float4 A0r = ab0[k], A1 = ab1[k];
+  const V3 rbt = load_vec<DIM>(aa.rb_in, i) + v3(A1) + dt * rb;
+  V3 vbt = dt * rbt;  // r' = r + dt v'
+  if (aa.vb_in) vbt = vbt + load_vec<DIM>(aa.vb_in, i);
+  V3 ubt = load_vec<DIM>(aa.ub_in, i) + v3(A0r) + dt * ub + vbt;
+  if (bc_holds(aa.bc_kind, a.in.tag[i])) {
+    ubt = V3{0.f, 0.f, 0.f};
+    vbt = ubt;
+  }
+  store_vec<DIM>(aa.rb_out, i, rbt);
+  store_vec<DIM>(aa.ub_out, i, ubt);
+  store_vec<DIM>(aa.ab_out, i, dt * ubt);  // u' = u + dt a
+  if (aa.bb_out) store_vec<DIM>(aa.bb_out, i, aa.c_tvf * vbt);
+  if (aa.rhob_out) aa.rhob_out[i] = Bk + (fluid ? dt * (rho_c + eos.dp_drho(rhok) * p_c) : 0.0f);
 }
 
 // Sweep W of the Riemann solver's wall condition (solver.py:547-552): the
@@ -762,20 +898,26 @@ __global__ __launch_bounds__(kAB) void k_adj_density(AdjArgs aa, const float4* _
 // The Riemann solver with density summation and tvf = 0: the density sweep
 // (its records, and the walls' Shepard velocity with the wall condition),
 // the force transpose, the Shepard velocity's transpose, then sweep B.
-template <int DIM, int KIND, bool AV>
+template <int DIM, int KIND, bool AV, bool CONT>
 int launch_adjoint_rie(sph_ws* ws, AdjArgs& a, bool records_ready, hipStream_t st) {
   const int n = a.s.n;
   dim3 grid(xcd_grid(cdivl((int64_t)n * tpp_of(DIM), kAB), a.s.remap));
   if (!records_ready)
-    SPH_LAUNCH(ws, "density", st, (k_density<DIM, KIND, 0><<<grid, kAB, 0, st>>>(a.s)));
+    SPH_LAUNCH(ws, "density", st,
+               (k_density<DIM, KIND, CONT ? 2 : 0><<<grid, kAB, 0, st>>>(a.s)));
   SPH_LAUNCH(ws, "adj_gather", st,
              (k_adj_gather<DIM, false><<<(unsigned)cdivl(n, kAB), kAB, 0, st>>>(a)));
   SPH_LAUNCH(ws, "adj_forces", st,
-             (k_adj_forces_rie<DIM, KIND, AV><<<grid, kAB, 0, st>>>(a, ws->AB0, ws->AB1,
-                                                                     ws->DL0)));
+             (k_adj_forces_rie<DIM, KIND, AV, CONT><<<grid, kAB, 0, st>>>(a, ws->AB0, ws->AB1,
+                                                                           ws->DL0)));
   if (a.s.prm.bc_trick)
     SPH_LAUNCH(ws, "adj_rie_walls", st,
                (k_adj_rie_walls<DIM, KIND><<<grid, kAB, 0, st>>>(a, ws->AB0, ws->AB1, ws->DL0)));
+  if (CONT) {
+    SPH_LAUNCH(ws, "adj_continuity", st,
+               (k_adj_rie_continuity<DIM, KIND><<<grid, kAB, 0, st>>>(a, ws->AB0, ws->AB1)));
+    return SPH_OK;
+  }
   SPH_LAUNCH(ws, "adj_density", st,
              (k_adj_density<DIM, KIND, false><<<grid, kAB, 0, st>>>(a, ws->AB0, ws->AB1,
                                                                      ws->AB2)));
@@ -784,7 +926,7 @@ int launch_adjoint_rie(sph_ws* ws, AdjArgs& a, bool records_ready, hipStream_t s
 
 template <int DIM, int KIND, bool TVF, bool WALLS, bool CONT, bool AV>
 int launch_adjoint(sph_ws* ws, AdjArgs& a, bool records_ready, hipStream_t st) {
-  if (a.s.prm.solver == 1) return launch_adjoint_rie<DIM, KIND, AV>(ws, a, records_ready, st);
+  if (a.s.prm.solver == 1) return launch_adjoint_rie<DIM, KIND, AV, CONT>(ws, a, records_ready, st);
   const int n = a.s.n;
   dim3 grid(xcd_grid(cdivl((int64_t)n * tpp_of(DIM), kAB), a.s.remap));
   if (!records_ready) {
@@ -866,9 +1008,9 @@ extern "C" int sph_step_vjp(sph_ws* ws, int bank, const sph_solver_params* prm, 
               "summation only");
     return SPH_ERR_ARG;
   }
-  if (prm->solver == 1 && (prm->density_evolution || prm->tvf_stress || prm->free_slip)) {
-    set_error("sph_step_vjp: the Riemann solver is differentiated with density summation, "
-              "no-slip walls and without the transport velocity");
+  if (prm->solver == 1 && (prm->tvf_stress || prm->free_slip)) {
+    set_error("sph_step_vjp: the Riemann solver is differentiated with no-slip walls and "
+              "without the transport velocity");
     return SPH_ERR_ARG;
   }
   if (prm->bc_trick && prm->free_slip && !next->nw) {
