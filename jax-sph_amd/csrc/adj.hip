@@ -307,7 +307,7 @@ __device__ __forceinline__ void rie_wall_row(const sph_solver_params& p, const E
 // neighbours' rows, and (with the wall condition) the cotangent of their
 // Shepard velocity, passed on as wf = (ubar_t / S, S).  AV: as in
 // k_adj_forces.
-template <int DIM, int KIND, bool AV, bool CONT>
+template <int DIM, int KIND, bool AV, bool CONT, bool TVF>
 __global__ __launch_bounds__(kAB) void k_adj_forces_rie(AdjArgs aa, float4* __restrict__ ab0,
                                                         float4* __restrict__ ab1,
                                                         float4* __restrict__ wf) {
@@ -328,9 +328,15 @@ __global__ __launch_bounds__(kAB) void k_adj_forces_rie(AdjArgs aa, float4* __re
   const bool wall_k = is_wall((int)Qk.z);
   const V3 gk = g_ext_of<DIM>(a, a.perm[k]);
   const V3 nk = wall_k ? v3(a.R.NW[k]) : V3{0.f, 0.f, 0.f};
-  // the wall's velocity in u_d: its Shepard velocity with the wall condition,
-  // else its own (solver.py:378, 547-552)
+  // the wall's velocity in u_d: its Shepard velocity with the no-slip wall
+  // condition, else its own (solver.py:378, 547-552); the records hold that
+  const bool shepard = p.bc_trick && !p.free_slip;
   const V3 utk = wall_k ? (p.bc_trick ? v3(a.R.UT[k]) : uk) : V3{0.f, 0.f, 0.f};
+  // transport velocity: the background-pressure acceleration G of
+  // solver.py:199-213, the same pair sum as the standard solver's (the
+  // Riemann stress term vanishes)
+  const V3 bk = TVF ? v3(aa.lb[k]) : V3{0.f, 0.f, 0.f};
+  const float pbg = TVF ? eos.p_fn(0.0f) : 0.0f;
   const float avK = AV ? p.artificial_alpha * p.av_h * 10.0f : 0.0f;
   const float av_h2 = AV ? 0.01f * p.av_h * p.av_h : 0.0f;
   float rhob = 0.0f, Sw = 0.0f;
@@ -348,6 +354,17 @@ __global__ __launch_bounds__(kAB) void k_adj_forces_rie(AdjArgs aa, float4* __re
     const float d = d2 * inv;
     const float q = d * p.one_over_h;
     const float w1 = K.dw(q), w2 = K.d2w(q);
+    if (TVF) {
+      const float Vk = mk / rhok, Vj = mj * rcp_fast(rhoj);
+      const float Vk2 = Vk * Vk, Vs = Vk2 + Vj * Vj;
+      const float A0 = w1 * inv, Ad0 = (w2 - w1 * inv) * inv;
+      const float imj = rcp_fast(mj);
+      const V3 bj = v3(aa.lb[j]);
+      const V3 nA = (A0 * inv_mk) * bk - (A0 * imj) * bj;
+      const V3 nAd = (Ad0 * inv_mk) * bk - (Ad0 * imj) * bj;
+      rhob += (-2.0f * Vk2 / rhok) * pbg * dot<DIM>(nA, dr);
+      rb = rb + (Vs * pbg) * ((dot<DIM>(nAd, dr) * inv) * dr + nA);
+    }
     if (!fluid) {
       // a wall (held by the hook): the rows of its fluid neighbours
       if (!wall_k || tag_j != kFluid) return;
@@ -357,7 +374,7 @@ __global__ __launch_bounds__(kAB) void k_adj_forces_rie(AdjArgs aa, float4* __re
                         mk, rhok, etak, uk, nk, utk, (-1.0f) * dr, d, inv, w1, w2, &dU, &dRho,
                         &dR, &c6);
       rb = rb - dR;  // the row depends on r_jk = r_j - r_k
-      if (p.bc_trick) {
+      if (shepard) {
         utb = utb + c6 * v3(Lj4);
         Sw += K.w(q);
       }
@@ -898,7 +915,7 @@ __global__ __launch_bounds__(kAB) void k_adj_density(AdjArgs aa, const float4* _
 // The Riemann solver with density summation and tvf = 0: the density sweep
 // (its records, and the walls' Shepard velocity with the wall condition),
 // the force transpose, the Shepard velocity's transpose, then sweep B.
-template <int DIM, int KIND, bool AV, bool CONT>
+template <int DIM, int KIND, bool AV, bool CONT, bool TVF>
 int launch_adjoint_rie(sph_ws* ws, AdjArgs& a, bool records_ready, hipStream_t st) {
   const int n = a.s.n;
   dim3 grid(xcd_grid(cdivl((int64_t)n * tpp_of(DIM), kAB), a.s.remap));
@@ -906,11 +923,11 @@ int launch_adjoint_rie(sph_ws* ws, AdjArgs& a, bool records_ready, hipStream_t s
     SPH_LAUNCH(ws, "density", st,
                (k_density<DIM, KIND, CONT ? 2 : 0><<<grid, kAB, 0, st>>>(a.s)));
   SPH_LAUNCH(ws, "adj_gather", st,
-             (k_adj_gather<DIM, false><<<(unsigned)cdivl(n, kAB), kAB, 0, st>>>(a)));
+             (k_adj_gather<DIM, TVF><<<(unsigned)cdivl(n, kAB), kAB, 0, st>>>(a)));
   SPH_LAUNCH(ws, "adj_forces", st,
-             (k_adj_forces_rie<DIM, KIND, AV, CONT><<<grid, kAB, 0, st>>>(a, ws->AB0, ws->AB1,
-                                                                           ws->DL0)));
-  if (a.s.prm.bc_trick)
+             (k_adj_forces_rie<DIM, KIND, AV, CONT, TVF><<<grid, kAB, 0, st>>>(
+                 a, ws->AB0, ws->AB1, ws->DL0)));
+  if (a.s.prm.bc_trick && !a.s.prm.free_slip)
     SPH_LAUNCH(ws, "adj_rie_walls", st,
                (k_adj_rie_walls<DIM, KIND><<<grid, kAB, 0, st>>>(a, ws->AB0, ws->AB1, ws->DL0)));
   if (CONT) {
@@ -926,7 +943,8 @@ int launch_adjoint_rie(sph_ws* ws, AdjArgs& a, bool records_ready, hipStream_t s
 
 template <int DIM, int KIND, bool TVF, bool WALLS, bool CONT, bool AV>
 int launch_adjoint(sph_ws* ws, AdjArgs& a, bool records_ready, hipStream_t st) {
-  if (a.s.prm.solver == 1) return launch_adjoint_rie<DIM, KIND, AV, CONT>(ws, a, records_ready, st);
+  if (a.s.prm.solver == 1)
+    return launch_adjoint_rie<DIM, KIND, AV, CONT, TVF>(ws, a, records_ready, st);
   const int n = a.s.n;
   dim3 grid(xcd_grid(cdivl((int64_t)n * tpp_of(DIM), kAB), a.s.remap));
   if (!records_ready) {
@@ -1006,11 +1024,6 @@ extern "C" int sph_step_vjp(sph_ws* ws, int bank, const sph_solver_params* prm, 
               "summation or continuity (without renormalisation), with or without the wall "
               "boundary condition and artificial viscosity; the Riemann solver with "
               "summation only");
-    return SPH_ERR_ARG;
-  }
-  if (prm->solver == 1 && (prm->tvf_stress || prm->free_slip)) {
-    set_error("sph_step_vjp: the Riemann solver is differentiated with no-slip walls and "
-              "without the transport velocity");
     return SPH_ERR_ARG;
   }
   if (prm->bc_trick && prm->free_slip && !next->nw) {

@@ -18,8 +18,9 @@ Memory is O(n_steps / k + k) states instead of O(n_steps).
 Differentiated configuration (other settings raise ``NotImplementedError``):
 standard SPH (``solver="SPH"``) as below, or the Riemann solver
 (``solver="RIE"``, solver.py:108-164, 316-401) with density summation or its
-continuity and tvf = 0, with or without its no-slip wall condition, its walls
-held by the case hook (the dam break as the survey's benchmark runs it);
+continuity, any ``tvf``, with or without its wall condition (no-slip or
+free-slip), its walls held by the case hook (the dam break as the survey's
+benchmark runs it);
 Quintic or Wendland-C2 kernel, a constant body force or (through :func:`rollout`) a
 position-dependent one, with or without the
 transport-velocity formulation (any ``tvf``: the stress ``A = rho u (v - u)``
@@ -65,8 +66,6 @@ def _check(solver: WCSPH, tvf: float, bc_fn=None, graph: bool = False, state=Non
         why.append("bc_fn must be a case boundary hook (case_setup.CaseBC)")
     if solver.solver not in ("SPH", "RIE"):
         why.append("solver must be 'SPH' or 'RIE'")
-    if solver.solver == "RIE" and (tvf != 0.0 or (solver.is_bc_trick and solver.is_free_slip)):
-        why.append("the Riemann solver is differentiated with no-slip walls and tvf = 0")
     if solver.solver == "RIE" and state is not None:
         # the wall rows of the Riemann acceleration are not transposed: every
         # wall must be held by the case hook (its accelerations are constants)

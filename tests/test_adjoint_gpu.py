@@ -831,10 +831,12 @@ def test_riemann_solver_gradient_matches_oracle_fd(kernel, limiter, alpha, n_ste
             assert abs(fd_s - fd) > 0.01 * abs(fd), (fd, fd_s)
 
 
-@pytest.mark.parametrize("case,gwbc,kernel,cont", [
-    ("pf", True, "QSK", False), ("ldc", True, "WC2K", False), ("ldc", False, "QSK", False),
-    ("ldc", True, "WC2K", True), ("db", True, "WC2K", True)])
-def test_riemann_walls_gradient_matches_oracle_fd(case, gwbc, kernel, cont):
+@pytest.mark.parametrize("case,gwbc,kernel,cont,tvf,free", [
+    ("pf", True, "QSK", False, 0.0, False), ("ldc", True, "WC2K", False, 0.0, False),
+    ("ldc", False, "QSK", False, 0.0, False), ("ldc", True, "WC2K", True, 0.0, False),
+    ("db", True, "WC2K", True, 0.0, False), ("ldc", True, "QSK", True, 1.0, False),
+    ("pf", True, "QSK", False, 0.0, True)])
+def test_riemann_walls_gradient_matches_oracle_fd(case, gwbc, kernel, cont, tvf, free):
     """The Riemann solver next to walls (solver.py:345-388): a fluid particle
     sees a wall through mirrored states (u_R from the wall's velocity and
     normal, p_R = p_L + rho_L g . (-r), rho_R = R(p_R)) and, with the wall
@@ -858,12 +860,14 @@ def test_riemann_walls_gradient_matches_oracle_fd(case, gwbc, kernel, cont):
           {"dx": 0.1 if case == "db" else 0.05},
           "kernel": {"name": kernel},
           "solver": {"name": "RIE", "is_bc_trick": gwbc, "t_end": 1.0,
-                     "density_evolution": cont}}
+                     "density_evolution": cont, "tvf": tvf, "free_slip": free}}
     cfg, box, state, g, bc, eos, disp, shift, solver = G.setup(case, ov)
     assert solver.solver == "RIE" and solver.is_bc_trick == gwbc
     assert bc.kind == (2 if case == "db" else 1) and solver.is_rho_evol == cont
-    plain = G.setup(case, {**ov, "solver": {**ov["solver"], "is_bc_trick": False}})[-1]
+    plain = G.setup(case, {**ov, "solver": {**ov["solver"], "is_bc_trick": False,
+                                            "free_slip": False}})[-1]
     summ = G.setup(case, {**ov, "solver": {**ov["solver"], "density_evolution": False}})[-1]
+    noslip = G.setup(case, {**ov, "solver": {**ov["solver"], "free_slip": False}})[-1]
     if case == "db":
         assert solver.artificial_alpha > 0
         warm = partition.neighbor_list(disp, box, r_cutoff=solver._kernel_fn.cutoff,
@@ -880,6 +884,9 @@ def test_riemann_walls_gradient_matches_oracle_fd(case, gwbc, kernel, cont):
     st["u"] = torch.where(fluid_t, state["u"] + 0.2 * torch.randn(n, 2, generator=gen).cuda(),
                           state["u"])
     st["v"] = st["u"].clone()
+    if tvf:   # a transport acceleration, so the background-pressure term acts at once
+        st["dvdt"] = torch.where(fluid_t, torch.randn(n, 2, generator=gen).cuda(),
+                                 torch.zeros_like(st["u"]))
     fn = partition.neighbor_list(disp, box, r_cutoff=solver._kernel_fn.cutoff,
                                  capacity_multiplier=1.25, mask_self=False,
                                  format=partition.Sparse, skin=0.0)
@@ -887,12 +894,14 @@ def test_riemann_walls_gradient_matches_oracle_fd(case, gwbc, kernel, cont):
     r0 = st["r"].clone().requires_grad_(True)
     s_in = dict(st)
     s_in["r"] = r0
-    u, r, _ = adjoint.rollout(s_in, solver, fn, shift, dt, 2, checkpoint_every=1, bc_fn=bc)
+    u, r, _ = adjoint.rollout(s_in, solver, fn, shift, dt, 2, tvf=tvf, checkpoint_every=1,
+                              bc_fn=bc)
     ((w * u.double()).sum() + 0.0 * r.sum()).backward()
     gr = r0.grad.double().cpu().numpy()
     assert np.isfinite(gr).all() and np.abs(gr).max() > 0
     if solver.g_const is not None:
-        gro = adjoint.GraphRollout(st, solver, fn, shift, dt, 2, checkpoint_every=1, bc_fn=bc)
+        gro = adjoint.GraphRollout(st, solver, fn, shift, dt, 2, tvf=tvf, checkpoint_every=1,
+                                   bc_fn=bc)
         r2 = st["r"].clone().requires_grad_(True)
         ug, rg = gro(st["u"].clone(), r2)
         ((w * ug.double()).sum() + 0.0 * rg.sum()).backward()
@@ -903,7 +912,7 @@ def test_riemann_walls_gradient_matches_oracle_fd(case, gwbc, kernel, cont):
     def L(eps, dr, sv=solver):
         s = G.to_f64(dict(base))
         s["r"] = np.mod(s["r"] + eps * dr, box)
-        out = _oracle_rollout(s, sv, box, dt, 2, bc_fn=bc)
+        out = _oracle_rollout(s, sv, box, dt, 2, tvf=tvf, bc_fn=bc)
         return float((wn * out["u"]).sum())
 
     rng = np.random.default_rng(42)
@@ -913,10 +922,14 @@ def test_riemann_walls_gradient_matches_oracle_fd(case, gwbc, kernel, cont):
         fd = (L(eps, dr) - L(-eps, dr)) / (2 * eps)
         ad = float((gr * dr).sum())
         scale = abs(fd) if trial == 0 else float(np.abs(gr * dr).sum())
-        print(f"RIE walls {case} gwbc={gwbc} cont={cont} trial {trial}: ad {ad:.9g} "
-              f"fd {fd:.9g} scale {scale:.6g}", flush=True)
+        print(f"RIE walls {case} gwbc={gwbc} cont={cont} tvf={tvf} free={free} trial "
+              f"{trial}: ad {ad:.9g} fd {fd:.9g} scale {scale:.6g}", flush=True)
         assert abs(ad - fd) <= 1e-5 * scale, (trial, ad, fd, scale)
-        if trial == 0 and gwbc and case != "db":
+        if trial == 0 and free:
+            fd_n = (L(eps, dr, noslip) - L(-eps, dr, noslip)) / (2 * eps)
+            print(f"  no-slip walls: fd {fd_n:.9g}", flush=True)
+            assert abs(fd_n - fd) > 1e-4 * abs(fd), (fd, fd_n)
+        if trial == 0 and gwbc and case != "db" and not free:
             # (the dam break's viscosity, 5e-5, leaves the wall condition's
             # viscous term too small to see over two steps)
             fd_p = (L(eps, dr, plain) - L(-eps, dr, plain)) / (2 * eps)
@@ -1003,9 +1016,6 @@ def test_checkpoint_interval_does_not_change_the_gradient():
 
 def test_unsupported_configurations_raise():
     cfg, box, st, solver, fn, shift, gen = _setup()
-    rie = G.setup("tgv", {"case": {"dim": 2, "dx": 0.05}, "solver": {"name": "RIE"}})[-1]
-    with pytest.raises(NotImplementedError, match="Riemann"):
-        adjoint.rollout(st, rie, fn, shift, cfg.solver.dt, 2, tvf=1.0)
     delta = G.setup("tgv", {"case": {"dim": 2, "dx": 0.05}, "solver": {"name": "DELTA"}})[-1]
     with pytest.raises(NotImplementedError):
         adjoint.rollout(st, delta, fn, shift, cfg.solver.dt, 2)
