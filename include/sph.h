@@ -248,8 +248,8 @@ typedef struct {
  * solver (solver.py:108-164, 316-401) with density summation or its
  * continuity, walls if any held by `bc`; or standard SPH
  * (heat conduction may be on: u', r' do not depend on the temperature, whose
- * cotangent is not formed); density summation or continuity (solver.py:24-30, without
- * renormalisation); with or without artificial viscosity (solver.py:404-428)
+ * cotangent is not formed); density summation or continuity (solver.py:24-30,
+ * with or without renormalisation); with or without artificial viscosity (solver.py:404-428)
  * and the generalized wall boundary condition (bc_trick, solver.py:431-530)
  * for no-slip or free-slip walls (free slip needs next->nw); QSK or WC2K;
  * with prm->tvf_stress the

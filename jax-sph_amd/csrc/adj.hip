@@ -81,6 +81,8 @@ struct AdjArgs {
   float dt;
   float c_tvf;               // v' = u' + c_tvf b
   int bc_kind;               // case boundary hook of the step (sph_bc_spec::kind)
+  const float4* rho_c;       // with renormalisation: records whose .w is the continuity
+                             // density rho' (the sweeps then read the renormalised ones)
 };
 
 // Whether the case boundary hook (integrator.py:54; ldc.py:130-144, db.py:299-307,
@@ -261,7 +263,7 @@ __device__ __forceinline__ void rie_wall_row(const sph_solver_params& p, const E
                                              float mj, float rhoj, float etaj, V3 uj, V3 nj,
                                              V3 utj, V3 dr, float d, float inv, float w1,
                                              float w2, V3* dU, float* dRho, V3* dR,
-                                             float* c6o) {
+                                             float* c6o, float* To) {
   const V3 e = inv * dr;
   const float ae = dot<DIM>(ak, e);
   const float gr = dot<DIM>(gk, dr);
@@ -291,6 +293,7 @@ __device__ __forceinline__ void rie_wall_row(const sph_solver_params& p, const E
   const V3 dPs_dr = (-0.5f * rhok * (1.0f + hDb / dPR)) * gk;
   *dR = (-c9d * Ps * ae + c6d * av) * e - (c9 * ae) * dPs_dr - (c9 * Ps * inv) * (ak - ae * e);
   *c6o = c6;
+  *To = T;  // dT/drho_j = -T / rho_j (the wall's density, when it changes)
 }
 
 // Sweep A of the Riemann solver (solver.py:316-401, 574-588) with density
@@ -369,11 +372,12 @@ __global__ __launch_bounds__(kAB) void k_adj_forces_rie(AdjArgs aa, float4* __re
       // a wall (held by the hook): the rows of its fluid neighbours
       if (!wall_k || tag_j != kFluid) return;
       V3 dU, dR;
-      float dRho, c6;
+      float dRho, c6, T;
       rie_wall_row<DIM>(p, eos, v3(Lj4), v3(Uj), rhoj, pj, etaj, g_ext_of<DIM>(a, a.perm[j]),
                         mk, rhok, etak, uk, nk, utk, (-1.0f) * dr, d, inv, w1, w2, &dU, &dRho,
-                        &dR, &c6);
+                        &dR, &c6, &T);
       rb = rb - dR;  // the row depends on r_jk = r_j - r_k
+      rhob -= T / rhok;  // and on the wall's density through 1 / rho_k
       if (shepard) {
         utb = utb + c6 * v3(Lj4);
         Sw += K.w(q);
@@ -382,10 +386,10 @@ __global__ __launch_bounds__(kAB) void k_adj_forces_rie(AdjArgs aa, float4* __re
     }
     if (is_wall(tag_j)) {
       V3 dU, dR;
-      float dRho, c6;
+      float dRho, c6, T;
       const V3 utj = p.bc_trick ? v3(a.R.UT[j]) : v3(Uj);
       rie_wall_row<DIM>(p, eos, lk, uk, rhok, pk, etak, gk, mj, rhoj, etaj, v3(Uj),
-                        v3(a.R.NW[j]), utj, dr, d, inv, w1, w2, &dU, &dRho, &dR, &c6);
+                        v3(a.R.NW[j]), utj, dr, d, inv, w1, w2, &dU, &dRho, &dR, &c6, &T);
       ub = ub + dU;
       rhob += dRho;
       rb = rb + dR;
@@ -452,7 +456,7 @@ __device__ __forceinline__ void rie_cont_wall_row(const sph_solver_params& p, co
                                                   V3 uk, float rhok, float pk, V3 gk, float mj,
                                                   float rhoj, V3 uj, V3 dr, float inv, float w1,
                                                   float w2, float* cu, float* dRho, float* dP,
-                                                  V3* dR) {
+                                                  V3* dR, float* Ro) {
   const V3 e = inv * dr;
   const V3 du = uk - uj;
   const float s_ = dot<DIM>(du, e);
@@ -470,6 +474,7 @@ __device__ __forceinline__ void rie_cont_wall_row(const sph_solver_params& p, co
   *dP = -al * w1 * t2 * 0.5f / dPR;
   *dR = (al * w2 * Y) * e +
         (al * w1) * (inv * (du - s_ * e) + (rhok * ic + t2 * 0.5f * rhok / dPR) * gk);
+  *Ro = R;  // dR/drho_j = -R / rho_j (the wall's density, when it changes)
 }
 
 // Sweep C of the Riemann continuity (solver.py:108-164, 772-793): the
@@ -517,18 +522,19 @@ __global__ __launch_bounds__(kAB) void k_adj_rie_continuity(AdjArgs aa,
     const float Bj = ab0[j].w;
     if (!fluid) {
       if (!wall_k || tag_j != kFluid) return;
-      float cu, dRho, dP;
+      float cu, dRho, dP, R;
       V3 dR;
       rie_cont_wall_row<DIM>(p, eos, v3(Uj), Uj.w, Qj.x, g_ext_of<DIM>(a, a.perm[j]), mk,
-                             rhok, uk, (-1.0f) * dr, inv, w1, w2, &cu, &dRho, &dP, &dR);
+                             rhok, uk, (-1.0f) * dr, inv, w1, w2, &cu, &dRho, &dP, &dR, &R);
       rb = rb - Bj * dR;
+      rho_c -= Bj * R / rhok;
       return;
     }
     if (is_wall(tag_j)) {
-      float cu, dRho, dP;
+      float cu, dRho, dP, R;
       V3 dR;
       rie_cont_wall_row<DIM>(p, eos, uk, rhok, pk, gk, Pj.w, Uj.w, v3(Uj), dr, inv, w1, w2,
-                             &cu, &dRho, &dP, &dR);
+                             &cu, &dRho, &dP, &dR, &R);
       ub = ub + (Bk * cu * inv) * dr;
       rho_c += Bk * dRho;
       p_c += Bk * dP;
@@ -572,7 +578,9 @@ __global__ __launch_bounds__(kAB) void k_adj_rie_continuity(AdjArgs aa,
   store_vec<DIM>(aa.ub_out, i, ubt);
   store_vec<DIM>(aa.ab_out, i, dt * ubt);  // u' = u + dt a
   if (aa.bb_out) store_vec<DIM>(aa.bb_out, i, aa.c_tvf * vbt);
-  if (aa.rhob_out) aa.rhob_out[i] = Bk + (fluid ? dt * (rho_c + eos.dp_drho(rhok) * p_c) : 0.0f);
+  // (a wall's density is constant unless renormalised; its cotangent is kept either way)
+  if (aa.rhob_out)
+    aa.rhob_out[i] = Bk + dt * (fluid ? rho_c + eos.dp_drho(rhok) * p_c : rho_c);
 }
 
 // Sweep W of the Riemann solver's wall condition (solver.py:547-552): the
@@ -864,6 +872,85 @@ __global__ __launch_bounds__(kAB) void k_adj_continuity(AdjArgs aa,
   if (aa.rhob_out) aa.rhob_out[i] = Bk * (1.0f + dt * accD) - dt * (mk / (rhok * rhok)) * accR;
 }
 
+// Sweeps N1/N2: the transpose of the density renormalisation after continuity
+// (solver.py:167-173), rho~_k = N_k / min(D_k, 1) with N_k = sum_j m_j W_kj and
+// D_k = sum_j (m_j / rho'_j) W_kj over all pairs (the self pair included).
+// Given B~ = ab0.w, the cotangent of rho~, N1 stores a_k = B~_k / min(D_k, 1)
+// and b_k = -B~_k N_k / D_k^2 (0 where D_k > 1 is clamped); N2 gathers
+//   rhobar'_k = -(m_k / rho'_k^2) sum_j b_j W_jk  (into ab0.w: continuity's B),
+//   rbar_k   += sum_j (a_k m_j + a_j m_k + b_k V_j + b_j V_k) W'(d) r_kj / d,
+// V = m / rho'.
+template <int DIM, int KIND>
+__global__ __launch_bounds__(kAB) void k_adj_renorm_prep(AdjArgs aa,
+                                                         const float4* __restrict__ ab0,
+                                                         float4* __restrict__ nb) {
+  constexpr int TPP = tpp_of(DIM);
+  const SweepArgs& a = aa.s;
+  const int tid = logical_block(a.remap) * blockDim.x + threadIdx.x;
+  const int k = tid / TPP, h = tid % TPP;
+  if (k >= a.n) return;
+  const sph_solver_params& p = a.prm;
+  const KernelEval<KIND, DIM> K{p.kernel, p.sigma, p.one_over_h};
+  const float4 Pk = a.R.P[k];
+  float nom = 0.f, den = 0.f;
+  for_each_pair(a, k, h, [&](const int j) {
+    const float4 Pj = a.R.P[j];
+    const V3 dr = min_image<DIM>(Pk, Pj, a.geo);
+    const float d2 = dist2<DIM>(dr.x, dr.y, dr.z);
+    if (!in_cutoff<DIM>(a, d2, Pk, Pj)) return;
+    const float w = K.w(sqrt_fast(d2) * p.one_over_h);
+    nom += Pj.w * w;
+    den += (Pj.w * rcp_fast(aa.rho_c[j].w)) * w;
+  });
+  nom = lane_sum<TPP>(nom);
+  den = lane_sum<TPP>(den);
+  if (h != 0) return;
+  const float Bt = ab0[k].w;
+  const float Dc = den > 1.0f ? 1.0f : den;
+  nb[k] = make_float4(Bt / Dc, den > 1.0f ? 0.0f : -Bt * nom / (den * den), 0.0f, 0.0f);
+}
+
+template <int DIM, int KIND>
+__global__ __launch_bounds__(kAB) void k_adj_renorm(AdjArgs aa, float4* __restrict__ ab0,
+                                                    float4* __restrict__ ab1,
+                                                    const float4* __restrict__ nb) {
+  constexpr int TPP = tpp_of(DIM);
+  const SweepArgs& a = aa.s;
+  const int tid = logical_block(a.remap) * blockDim.x + threadIdx.x;
+  const int k = tid / TPP, h = tid % TPP;
+  if (k >= a.n) return;
+  const sph_solver_params& p = a.prm;
+  const KernelEval<KIND, DIM> K{p.kernel, p.sigma, p.one_over_h};
+  const float4 Pk = a.R.P[k];
+  const float mk = Pk.w, rk = aa.rho_c[k].w, Vk = mk / rk;
+  const float4 Nk = nb[k];
+  float bw = 0.f;
+  V3 rb = {0.f, 0.f, 0.f};
+  for_each_pair(a, k, h, [&](const int j) {
+    const float4 Pj = a.R.P[j];
+    const V3 dr = min_image<DIM>(Pk, Pj, a.geo);
+    const float d2 = dist2<DIM>(dr.x, dr.y, dr.z);
+    if (!in_cutoff<DIM>(a, d2, Pk, Pj)) return;
+    const float4 Nj = nb[j];
+    if (d2 == 0.0f) {  // the self pair: W(0), no position derivative
+      bw += Nj.y * K.w(0.0f);
+      return;
+    }
+    const float inv = rsq_fast(d2);
+    const float q = d2 * inv * p.one_over_h;
+    bw += Nj.y * K.w(q);
+    const float mj = Pj.w, Vj = mj * rcp_fast(aa.rho_c[j].w);
+    const float c = Nk.x * mj + Nj.x * mk + Nk.y * Vj + Nj.y * Vk;
+    rb = rb + (c * K.dw(q) * inv) * dr;
+  });
+  bw = lane_sum<TPP>(bw);
+  rb = lane_sum<TPP>(rb);
+  if (h != 0) return;
+  const float4 A0 = ab0[k], A1 = ab1[k];
+  ab0[k] = f4(v3(A0), -(mk / (rk * rk)) * bw);
+  ab1[k] = f4(v3(A1) + rb, A1.w);
+}
+
 // Sweep B: the r' cotangent through the density sums, then the kinematic
 // transpose of the kick, transport velocity and drift, written in original
 // order.
@@ -912,6 +999,17 @@ __global__ __launch_bounds__(kAB) void k_adj_density(AdjArgs aa, const float4* _
   if (aa.bb_out) store_vec<DIM>(aa.bb_out, i, aa.c_tvf * vbt);
 }
 
+// The renormalisation's transpose (sweeps N1, N2), between the force and
+// wall transposes and the continuity transpose.
+template <int DIM, int KIND>
+int launch_adj_renorm(sph_ws* ws, AdjArgs& a, dim3 grid, hipStream_t st) {
+  SPH_LAUNCH(ws, "adj_renorm_prep", st,
+             (k_adj_renorm_prep<DIM, KIND><<<grid, kAB, 0, st>>>(a, ws->AB0, ws->DG1)));
+  SPH_LAUNCH(ws, "adj_renorm", st,
+             (k_adj_renorm<DIM, KIND><<<grid, kAB, 0, st>>>(a, ws->AB0, ws->AB1, ws->DG1)));
+  return SPH_OK;
+}
+
 // The Riemann solver with density summation and tvf = 0: the density sweep
 // (its records, and the walls' Shepard velocity with the wall condition),
 // the force transpose, the Shepard velocity's transpose, then sweep B.
@@ -931,6 +1029,10 @@ int launch_adjoint_rie(sph_ws* ws, AdjArgs& a, bool records_ready, hipStream_t s
     SPH_LAUNCH(ws, "adj_rie_walls", st,
                (k_adj_rie_walls<DIM, KIND><<<grid, kAB, 0, st>>>(a, ws->AB0, ws->AB1, ws->DL0)));
   if (CONT) {
+    if (a.s.prm.rho_renorm) {
+      const int rc = launch_adj_renorm<DIM, KIND>(ws, a, grid, st);
+      if (rc) return rc;
+    }
     SPH_LAUNCH(ws, "adj_continuity", st,
                (k_adj_rie_continuity<DIM, KIND><<<grid, kAB, 0, st>>>(a, ws->AB0, ws->AB1)));
     return SPH_OK;
@@ -968,6 +1070,10 @@ int launch_adjoint(sph_ws* ws, AdjArgs& a, bool records_ready, hipStream_t st) {
   }
   if (CONT) {
     // the walls' rho^ cotangent is spent above (their rho' is overwritten by rho^)
+    if (a.s.prm.rho_renorm) {
+      const int rc = launch_adj_renorm<DIM, KIND>(ws, a, grid, st);
+      if (rc) return rc;
+    }
     SPH_LAUNCH(ws, "adj_continuity", st,
                (k_adj_continuity<DIM, KIND, TVF><<<grid, kAB, 0, st>>>(a, ws->AB0, ws->AB1,
                                                                         ws->AB2)));
@@ -1019,11 +1125,9 @@ extern "C" int sph_step_vjp(sph_ws* ws, int bank, const sph_solver_params* prm, 
     set_error("sph_step_vjp: the r, u and a cotangent arrays are required");
     return SPH_ERR_ARG;
   }
-  if ((prm->solver != 0 && prm->solver != 1) || (prm->density_evolution && prm->rho_renorm)) {
-    set_error("sph_step_vjp: the standard SPH solver is differentiated with density "
-              "summation or continuity (without renormalisation), with or without the wall "
-              "boundary condition and artificial viscosity; the Riemann solver with "
-              "summation only");
+  if (prm->solver != 0 && prm->solver != 1) {
+    set_error("sph_step_vjp: the standard SPH and Riemann solvers are differentiated "
+              "(not delta-SPH)");
     return SPH_ERR_ARG;
   }
   if (prm->bc_trick && prm->free_slip && !next->nw) {
@@ -1056,6 +1160,13 @@ extern "C" int sph_step_vjp(sph_ws* ws, int bank, const sph_solver_params* prm, 
   a.rhob_out = bar->rho;
   a.la = ws->LA;
   a.lb = ws->LB;
+  a.rho_c = ws->U1;
+  if (prm->density_evolution && prm->rho_renorm) {
+    // the sweeps read the renormalised density and pressure (as the forward
+    // force and wall sweeps do); rho_c keeps the continuity density
+    a.s.R.U1 = ws->U3;
+    a.s.R.Q1 = ws->Q3;
+  }
   a.dt = dt;
   a.c_tvf = tvf_coef;
   a.bc_kind = bc ? bc->kind : 0;

@@ -27,7 +27,8 @@ transport-velocity formulation (any ``tvf``: the stress ``A = rho u (v - u)``
 and the background-pressure acceleration, solver.py:199-256), with or without
 artificial viscosity (solver.py:404-428); density summation or density
 continuity (``density_evolution``, solver.py:24-30; the density is then
-carried from step to step, and checkpointed; no renormalisation), each with
+carried from step to step, and checkpointed; with or without the density
+renormalisation of solver.py:167-173), each with
 or without the generalized wall boundary condition for no-slip or free-slip
 walls (``is_bc_trick``, solver.py:431-530; free-slip walls at rest, whose
 normals are those of the state).  The dam break as its case file runs
@@ -78,8 +79,6 @@ def _check(solver: WCSPH, tvf: float, bc_fn=None, graph: bool = False, state=Non
                 torch.isin(tag[walls], torch.as_tensor(held, dtype=tag.dtype,
                                                        device=tag.device)).all()):
             why.append("the Riemann solver's walls must be held by the case hook")
-    if solver.is_rho_evol and solver.is_rho_renorm:
-        why.append("density renormalisation is not differentiated")
     if solver._kernel_fn.name not in ("QSK", "WC2K"):
         why.append("kernel must be QSK or WC2K")
     if solver.g_const is None and graph:
