@@ -1,7 +1,7 @@
 """Case initialisation (host side, once per run) for the cases on the SPH step.
 
 Mirrors ``jax_sph/case_setup.py:44-231``, ``jax_sph/defaults.py:6-148`` and the
-case files ``cases/{tgv,ldc,db,pf,cf}.{py,yaml}``: the same defaults, CFL time
+case files ``cases/{tgv,ldc,db,pf,cf,rpf,ut,cw,ht}.{py,yaml}``: the same defaults, CFL time
 step, equation of state, lattice and wall positions (float32 rounded like the
 reference), tags, initial velocities, wall normals (nearest point of a 5x
 finer wall layer, ``utils.py:169-194``) and the 16-key state dict, returned as
@@ -9,7 +9,9 @@ device tensors.  ``initialize()`` returns the reference's 10-tuple.
 
 Per-step hooks are objects, not closures, so the fused step can run them on
 the device: ``CaseBC`` for ``bc_fn`` and ``solver.GravityField`` for constant
-body forces.  Random draws (position noise) use a seeded ``torch.Generator``
+body forces.  The heat-transfer case's hook (``HeatTransferBC``) and the position-
+dependent forces (pf, rpf, ht) are torch callables, applied between the HIP
+calls of the step.  Random draws (position noise) use a seeded ``torch.Generator``
 instead of a JAX PRNG key, so noisy inputs are not bitwise the reference's.
 """
 from __future__ import annotations
@@ -93,6 +95,25 @@ CASE_YAML = {
                                  "viscosity": 100.0, "u_ref": 1.25,
                                  "special": {"L": 0.4, "H": 1.0, "u_x_wall": 1.25}},
            "solver": {"dt": 0.0000005, "t_end": 0.01, "is_bc_trick": True}},
+    "rpf": {"seed": 123, "case": {"source": "rpf.py", "dim": 2, "dx": 0.025, "viscosity": 0.1,
+                                  "g_ext_magnitude": 1.0},
+            "solver": {"dt": 0.0005, "t_end": 100.0}, "eos": {"p_bg_factor": 0.05}},
+    "ut": {"case": {"source": "ut.py", "dim": 2, "dx": 0.02, "g_ext_magnitude": 1.0,
+                    "u_ref": 1.0,
+                    "special": {"L_wall": 1.0, "H_wall": 1.0, "L": 0.5, "H": 0.5,
+                                "cube_offset": [0.25, 0.25]}},
+           "solver": {"density_evolution": True, "t_end": 1.0}},
+    "cw": {"case": {"source": "cw.py", "dim": 2, "dx": 0.05, "g_ext_magnitude": 1.0,
+                    "u_ref": 1.0,
+                    "special": {"L_wall": 1.0, "H_wall": 1.0, "L": 0.3, "H": 0.3,
+                                "cube_offset": [0.5, 0.5], "u_init": [0.0, -0.5]}},
+           "solver": {"is_bc_trick": True, "density_evolution": True, "t_end": 1.0}},
+    "ht": {"case": {"source": "ht.py", "dim": 2, "dx": 0.02, "r0_noise_factor": 0.05,
+                    "g_ext_magnitude": 2.3, "kappa_ref": 7.313, "Cp_ref": 305.27,
+                    "special": {"hot_wall_temperature": 1.23, "hot_wall_half_width": 0.25,
+                                "L": 1.0, "H": 0.2}},
+           "solver": {"t_end": 1.5, "heat_conduction": True, "is_bc_trick": True},
+           "eos": {"p_bg_factor": 0.05}},
 }
 
 
@@ -202,6 +223,40 @@ class CaseBC:
         return state
 
 
+class HeatTransferBC:
+    """The heat-transfer case's ``bc_fn`` (ht.py:152-190), applied after the
+    RHS: fluid entering at x < n_walls dx takes ``T_ref``; the heated wall
+    (``Tag.DIRICHLET_WALL``) holds ``hot_T`` and the other walls ``T_ref``,
+    with dT/dt = 0 there; the walls stay at rest; fluid leaving at
+    x > ``x_len`` - n_walls dx has dT/dt = 0.  Thresholds and values are
+    float32, as the reference compares float32 positions with them."""
+
+    def __init__(self, n_walls: int, dx: float, T_ref: float, hot_T: float, x_len: float):
+        self.x_in = float(np.float32(n_walls * dx))
+        self.x_out = float(np.float32(x_len - n_walls * dx))
+        self.T_ref, self.hot_T = float(np.float32(T_ref)), float(np.float32(hot_T))
+
+    def __call__(self, state):
+        state = dict(state)
+        tag = torch.as_tensor(state["tag"])
+        x = state["r"][:, 0]
+        T, dTdt = state["T"], state["dTdt"]
+        zero = torch.zeros((), dtype=T.dtype, device=T.device)
+        fluid = tag == Tag.FLUID
+        hot, solid = tag == Tag.DIRICHLET_WALL, tag == Tag.SOLID_WALL
+        for m, value in ((fluid & (x < self.x_in), self.T_ref), (hot, self.hot_T),
+                         (solid, self.T_ref)):
+            T = torch.where(m, torch.full_like(T, value), T)
+            dTdt = torch.where(m, zero, dTdt)
+        static = (hot | solid)[:, None]
+        for k in ("u", "v", "dudt", "dvdt"):
+            state[k] = torch.where(static, torch.zeros((), dtype=state[k].dtype,
+                                                       device=state[k].device), state[k])
+        dTdt = torch.where(fluid & (x > self.x_out), zero, dTdt)
+        state["T"], state["dTdt"] = T, dTdt
+        return state
+
+
 def _compute_nws(r, tag, dx, n_walls, offset_vec, wall_part_fn):
     """utils.py:169-194: normal = unit vector to the nearest point of a fine wall layer."""
     from scipy.spatial import KDTree
@@ -259,7 +314,7 @@ def _wall_layer(dx, n_walls, offset_vec, wall_part_fn):
 
 
 class SimulationSetup:
-    """case_setup.py:29-231 for one of tgv / ldc / db / pf / cf."""
+    """case_setup.py:29-231 for one of tgv / ldc / db / pf / cf / rpf / ut / cw / ht."""
 
     def __init__(self, cfg: Cfg):
         self.cfg = cfg
@@ -282,6 +337,9 @@ class SimulationSetup:
         # relaxation (case_setup.py:356-404): the case's own geometry at rest,
         # no body force, every wall at rest
         self.relax = cfg.case.mode == "rlx"
+        if self.name in ("ut", "cw") and (self.relax or cfg.case.r0_type == "relaxed"):
+            # ut.py:19-20, cw.py:25-26
+            raise NotImplementedError(f"Relaxation not implemented for {self.name.upper()}")
 
     # ---------------------------------------------------------------- geometry
     def _box_size(self, dim, n_walls):
@@ -298,6 +356,17 @@ class SimulationSetup:
         if name in ("pf", "cf"):
             dx2n = dx * n_walls * 2
             return np.array([sp.L, sp.H + dx2n] if dim == 2 else [sp.L, sp.H + dx2n, 0.4])
+        if name == "rpf":                                               # rpf.py:25-29
+            return np.array([1.0, 2.0] if dim == 2 else [1.0, 2.0, 0.5])
+        if name == "ut":                                                # ut.py:22-26
+            return np.array([sp.L_wall, sp.H_wall] if dim == 2 else [sp.L_wall] * 3)
+        if name == "cw":                                                # cw.py:28-36
+            dx2n = 2 * n_walls * dx
+            return np.array([sp.L_wall + dx2n, sp.H_wall + dx2n] if dim == 2
+                            else [sp.L_wall + dx2n, sp.H_wall + dx2n, 1.0 + dx2n])
+        if name == "ht":                                                # ht.py:28-36
+            dx2n = dx * n_walls * 2
+            return np.array([sp.L, sp.H + dx2n] if dim == 2 else [sp.L, sp.H + dx2n, 0.5])
         raise ValueError(name)
 
     def _walls(self, dx, n_walls, dim):
@@ -308,13 +377,17 @@ class SimulationSetup:
         if name == "db":
             return (pos_box_2d(np.array([sp.L_wall, sp.H_wall]), dx, n_walls) if dim == 2
                     else pos_box_3d(np.array([sp.L_wall, sp.H_wall, 1.0]), dx, n_walls))
-        if name in ("pf", "cf"):
+        if name == "cw":                                                # cw.py:38-46
+            return (pos_box_2d(np.array([sp.L_wall, sp.H_wall]), dx, n_walls) if dim == 2
+                    else pos_box_3d(np.array([sp.L_wall, sp.H_wall, 1.0]), dx, n_walls, False))
+        if name in ("pf", "cf", "ht"):                                  # ht.py:38-70
             dxn = dx * n_walls
+            depth = 0.5 if name == "ht" else 0.4
             if dim == 2:
                 horiz = pos_init_cartesian_2d(np.array([sp.L, dxn]), dx)
                 top =(horiz + np.array([0.0, sp.H + dxn]).astype(np.float32)).astype(np.float32)
             else:
-                horiz = pos_init_cartesian_3d(np.array([sp.L, dxn, 0.4]), dx)
+                horiz = pos_init_cartesian_3d(np.array([sp.L, dxn, depth]), dx)
                 top = (horiz + np.array([0.0, sp.H + dxn, 0.0]).astype(np.float32)).astype(
                     np.float32)
             return np.concatenate([horiz, top])
@@ -324,8 +397,10 @@ class SimulationSetup:
         nw, dx = self.cfg.solver.n_walls, self.case.dx
         if self.name in ("ldc", "db"):
             return np.ones(dim) * nw * dx if dim == 2 else np.array([1.0, 1.0, 0.0]) * nw * dx
-        if self.name in ("pf", "cf"):
+        if self.name in ("pf", "cf", "ht"):                             # ht.py:126-132
             return np.array([0.0, 1.0] if dim == 2 else [0.0, 1.0, 0.0]) * nw * dx
+        if self.name == "cw":                                           # cw.py:22
+            return np.ones(dim) * nw * dx
         return np.zeros(dim)
 
     def relaxed_path(self) -> str:
@@ -354,23 +429,35 @@ class SimulationSetup:
         name, sp = self.name, self.special
         f32 = np.float32
         if self.case.r0_type == "relaxed" and name != "db":
-            # tgv/ldc/pf/cf take every particle and tag from the file
-            # (``_load_only_fluid = False``, cases/tgv.py:20-23, ldc.py:36-39)
+            # tgv/ldc/pf/cf/rpf/ht take every particle and tag from the file
+            # (``_load_only_fluid = False``, cases/tgv.py:20-23, ldc.py:36-39,
+            # rpf.py:19-22, ht.py:23-26)
             return self._relaxed()
-        if name == "tgv":
+        if name in ("tgv", "rpf"):              # the whole box (case_setup.py:241-249)
             box = self._box_size(dim, n_walls)
             r = pos_init_cartesian_2d(box, dx) if dim == 2 else pos_init_cartesian_3d(box, dx)
+            return r, np.full(len(r), Tag.FLUID, np.int32)
+        if name == "ut":                        # ut.py:34-44: a fluid cube, no walls
+            cube = np.array([sp.L, sp.H] if dim == 2 else [sp.L, sp.L, sp.H])
+            lat = pos_init_cartesian_2d(cube, dx) if dim == 2 else pos_init_cartesian_3d(cube, dx)
+            r = (lat + self._cube_offset(dim)).astype(f32)
             return r, np.full(len(r), Tag.FLUID, np.int32)
         if name == "ldc":
             fluid_box = np.ones(2) if dim == 2 else np.array([1.0, 1.0, 0.5])
         elif name == "db":
             fluid_box = np.array([sp.L, sp.H] if dim == 2 else [sp.L, sp.H, 1.0])
+        elif name == "cw":
+            fluid_box = np.array([sp.L, sp.H] if dim == 2 else [sp.L, sp.H, 0.3])
+        elif name == "ht":
+            fluid_box = np.array([sp.L, sp.H] if dim == 2 else [sp.L, sp.H, 0.5])
         else:
             fluid_box = np.array([sp.L, sp.H] if dim == 2 else [sp.L, sp.H, 0.4])
         lat = pos_init_cartesian_2d(fluid_box, dx) if dim == 2 else pos_init_cartesian_3d(
             fluid_box, dx)
         if name == "ldc" or (name == "db" and dim == 2):
             off = np.full(dim, n_walls * dx)               # ldc.py:65,85, db.py:241
+        elif name == "cw":                                 # cw.py:57-58: one float32 add
+            off = n_walls * dx + self._cube_offset(dim, np.float64)
         else:
             off = self.offset_vec(dim)
         r_f = (lat + off.astype(f32)).astype(f32)
@@ -387,7 +474,22 @@ class SimulationSetup:
             box = self._box_size(dim, n_walls)
             lid = r[:, 1] > (box[1] - n_walls * dx)                    # ldc.py:79
             tag = np.where(lid, np.int32(Tag.MOVING_WALL), tag).astype(np.int32)
+        if name == "ht":                       # ht.py:83-91: the heated part of the bottom
+            mid = f32(self._box_size(dim, n_walls)[0] / 2)
+            hw = f32(sp.hot_wall_half_width)
+            hot = ((r[:, 1] < f32(dx * n_walls)) & (r[:, 0] < f32(mid + hw))
+                   & (r[:, 0] > f32(mid - hw)))
+            tag = np.where(hot, np.int32(Tag.DIRICHLET_WALL), tag).astype(np.int32)
         return r, tag
+
+    def _cube_offset(self, dim, dtype=np.float32):
+        """The cube's offset (ut.py:17, cw.py:58); its length must be the
+        dimension (the 2-entry default offsets only a 2D cube)."""
+        off = np.asarray(self.special.cube_offset, dtype)
+        if off.shape != (dim,):
+            raise ValueError(f"special.cube_offset {list(self.special.cube_offset)} does not "
+                             f"match dim={dim}")
+        return off
 
     def _velocity(self, r):
         dim = r.shape[1]
@@ -405,24 +507,34 @@ class SimulationSetup:
                 zt = np.cos(z)
                 v[:, 0] = np.sin(x) * np.cos(y) * zt
                 v[:, 1] = -np.cos(x) * np.sin(y) * zt
+        if self.name == "cw" and dim == 2:                              # cw.py:83-85
+            v[:] = np.asarray(self.special.u_init, np.float32)
         return v.astype(np.float32)
 
     def g_ext_fn(self, dim):
         name, g = self.name, self.case.g_ext_magnitude
         if self.relax:                                                  # case_setup.py:390-391
             return GravityField(np.zeros(dim, np.float32))
-        if name == "db":
+        if name in ("db", "ut", "cw"):            # db.py, ut.py:52-55, cw.py:90-93
             return GravityField([0.0, -g] + ([0.0] if dim == 3 else []))
-        if name == "pf":
+        if name in ("pf", "ht"):
             nw, dx = self.cfg.solver.n_walls, self.case.dx
             box = self._box_size(dim, nw)
 
-            def pf_force(r, box=box, dxn=nw * dx, g=g):             # pf.py:134-143
+            def pf_force(r, box=box, dxn=nw * dx, g=g):     # pf.py:134-143, ht.py:140-150
                 res = torch.zeros_like(r)
                 fm = (r[:, 1] < float(box[1] - dxn)) & (r[:, 1] > float(dxn))
                 res[:, 0] = torch.where(fm, torch.ones_like(r[:, 0]), torch.zeros_like(r[:, 0]))
                 return res * float(np.float32(g))
             return pf_force
+        if name == "rpf":
+
+            def rpf_force(r, g=g):                                      # rpf.py:43-48
+                res = torch.zeros_like(r)
+                res[:, 0] = torch.where(r[:, 1] > 1.0, -torch.ones_like(r[:, 0]),
+                                        torch.ones_like(r[:, 0]))
+                return res * float(np.float32(g))
+            return rpf_force
         return GravityField(np.zeros(dim, np.float32))
 
     def bc_fn(self, dim):
@@ -432,10 +544,14 @@ class SimulationSetup:
             return CaseBC(1, [self.special.u_x_lid, 0.0, 0.0])
         if self.name == "cf":
             return CaseBC(1, [self.special.u_x_wall, 0.0, 0.0])
-        if self.name == "db":
+        if self.name in ("db", "cw"):                                   # cw.py:95-103
             return CaseBC(2)
         if self.name == "pf":
             return CaseBC(1)
+        if self.name == "ht":
+            return HeatTransferBC(self.cfg.solver.n_walls, self.case.dx, self.case.T_ref,
+                                  self.special.hot_wall_temperature,
+                                  float(self._box_size(dim, self.cfg.solver.n_walls)[0]))
         return CaseBC(0)
 
     # ----------------------------------------------------------------- main

@@ -43,7 +43,57 @@ def oracle_solver(s: "js.solver.WCSPH", box):
                     diff_alpha=s.diff_alpha, is_rho_renorm=s.is_rho_renorm)
 
 
-def oracle_bc(bc: "cs.CaseBC", dim):
+def oracle_ht_bc(cfg, box):
+    """ht.py:152-190 on numpy states (float32 thresholds and values, as the
+    reference compares float32 positions): inflow fluid (x < n_walls dx) and
+    solid walls take T_ref, the heated wall its temperature, all three with
+    dT/dt = 0; walls at rest; outflow fluid (x > box_x - n_walls dx) dT/dt = 0."""
+    f32 = np.float32
+    dxn = cfg.solver.n_walls * cfg.case.dx
+    x_in, x_out = f32(dxn), f32(float(box[0]) - dxn)
+    T_ref, hot_T = f32(cfg.case.T_ref), f32(cfg.case.special.hot_wall_temperature)
+
+    def f(state):
+        st = dict(state)
+        tag, x = st["tag"], st["r"][:, 0].astype(f32)
+        t = st["T"].dtype
+        fluid, hot, solid = tag == 0, tag == 3, tag == 1
+        T, dT = st["T"].copy(), st["dTdt"].copy()
+        for m, val in ((fluid & (x < x_in), T_ref), (hot, hot_T), (solid, T_ref)):
+            T[m], dT[m] = t.type(val), t.type(0)
+        dT[fluid & (x > x_out)] = t.type(0)
+        for k in ("u", "v", "dudt", "dvdt"):
+            st[k] = st[k].copy()
+            st[k][hot | solid] = 0
+        st["T"], st["dTdt"] = T, dT
+        return st
+    return f
+
+
+def case_force_np(cfg, box):
+    """The position-dependent body forces restated in numpy: pf.py:134-143 and
+    ht.py:140-150 (g in x inside the band n_walls dx < y < box_y - n_walls dx),
+    rpf.py:43-48 (+g in x for y <= 1, -g above).  Comparisons in float32."""
+    name = cfg.case.source.split(".")[0]
+    f32 = np.float32
+    g = f32(cfg.case.g_ext_magnitude)
+    dxn = cfg.solver.n_walls * cfg.case.dx
+
+    def f(r):
+        y = np.asarray(r)[:, 1].astype(f32)
+        res = np.zeros_like(r)
+        if name == "rpf":
+            res[:, 0] = np.where(y > f32(1.0), -g, g)
+        else:
+            res[:, 0] = np.where((y < f32(float(box[1]) - dxn)) & (y > f32(dxn)), g, f32(0))
+        return res
+    return f
+
+
+def oracle_bc(bc: "cs.CaseBC", dim, cfg=None, box=None):
+    if isinstance(bc, cs.HeatTransferBC):
+        return oracle_ht_bc(cfg, box)
+
     def f(state):
         st = dict(state)
         tag = st["tag"]
@@ -67,6 +117,8 @@ def setup(case, overrides=None):
     cfg = cs.load_config(case, overrides)
     Case = cs.load_case(case)
     (cfg, box, state, g_ext_fn, bc_fn, nw_fn, eos, key, disp, shift) = Case(cfg).initialize()
+    if case in ("pf", "rpf", "ht") and cfg.case.mode == "sim":
+        g_ext_fn.np = case_force_np(cfg, box)      # the oracle's own twin of the force
     solver = cs.make_solver(cfg, disp, eos, g_ext_fn)
     return cfg, box, state, g_ext_fn, bc_fn, eos, disp, shift, solver
 
@@ -145,7 +197,7 @@ class OracleRef:
         self.ns = nlo.NeighborSearch(box, solver._kernel_fn.cutoff, mask_self=False)
         self.ns64 = nlo.NeighborSearch(box, solver._kernel_fn.cutoff, mask_self=False,
                                        dtype=np.float64)
-        bcf = oracle_bc(bc, cfg.case.dim)
+        bcf = oracle_bc(bc, cfg.case.dim, cfg, box)
         self.adv = so.si_euler(cfg.solver.tvf, oracle_solver(solver, box).forward, box, bcf,
                                self.ns.update)
         self.adv64 = so.si_euler(cfg.solver.tvf, oracle_solver64(solver, box).forward, box,

@@ -30,6 +30,10 @@ def _init(case, ov):
     ("ldc", {"case": {"dx": 0.02}}, 56 * 56),          # 50^2 fluid + 3 wall layers
     ("ldc", {"case": {"dx": 0.005}}, 206 * 206),       # the largest ensemble member
     ("tgv", {"case": {"dim": 3, "dx": 2 * np.pi / 16}}, 16 ** 3),
+    ("rpf", {}, 40 * 80),                              # the whole [1, 2] box
+    ("ut", {}, 25 * 25),                               # a 0.5^2 cube, no walls
+    ("cw", {}, 6 * 6 + 2 * 3 * 26 + 2 * 20 * 3),       # 0.3^2 cube in a 1^2 box
+    ("ht", {}, 50 * 10 + 2 * 50 * 3),                  # channel and two walls
 ])
 def test_particle_counts(case, ov, n):
     cfg, box, state, *_ = _init(case, ov)
@@ -195,3 +199,84 @@ def test_moving_wall_normals_follow_the_walls():
     want = so.wall_normals(r.numpy(), state["tag"].numpy(), nw_fn.layer, nw_fn.offset, box)
     np.testing.assert_allclose(got, want, atol=1e-6)
     assert not np.allclose(got, state["nw"].numpy(), atol=1e-3)  # the normals did change
+
+
+def test_falling_cube_cases():
+    """ut.py:34-44 and cw.py:48-93: the cube's lattice plus its offset in one
+    float32 add (cw adds n_walls dx to the offset first); cw's fluid starts
+    at u_init and its walls at rest with p = 0 (its hook is the dam break's,
+    db.py:299-307); gravity -g in y for both."""
+    f = np.float32
+    cfg, box, st, g, bc, *_ = _init("ut", {})
+    r = st["r"].numpy()
+    assert r[0, 0] == f(f(f(0.5) * f(0.02)) + f(0.25))
+    assert np.all(st["tag"].numpy() == Tag.FLUID) and list(box) == [1.0, 1.0]
+    np.testing.assert_array_equal(g.g, np.array([0.0, -1.0], np.float32))
+    assert isinstance(bc, cs.CaseBC) and bc.kind == 0
+    cfg, box, st, g, bc, *_ = _init("cw", {})
+    r, tag = st["r"].numpy(), st["tag"].numpy()
+    fl = tag == Tag.FLUID
+    assert r[fl][0, 0] == f(f(f(0.5) * f(0.05)) + f(3 * 0.05 + 0.5))
+    np.testing.assert_array_equal(st["u"][fl].numpy(), np.tile([0.0, -0.5], (fl.sum(), 1)))
+    assert np.all(st["u"][~fl].numpy() == 0) and np.all(st["v"][~fl].numpy() == 0)
+    assert np.all(st["p"][~fl].numpy() == 0)
+    assert isinstance(bc, cs.CaseBC) and bc.kind == 2
+    np.testing.assert_allclose(box, [1.3, 1.3])
+    with pytest.raises(ValueError, match="cube_offset"):
+        _init("ut", {"case": {"dim": 3}})
+    cfg, box, st, *_ = _init("cw", {"case": {"dim": 3, "special": {"cube_offset": [.5, .5, .5]}}})
+    assert (st["tag"] == Tag.FLUID).sum() == 6 ** 3                # walls on all six sides
+    for case in ("ut", "cw"):
+        with pytest.raises(NotImplementedError):
+            _init(case, {"case": {"mode": "rlx"}})
+
+
+def test_reverse_poiseuille_force():
+    """rpf.py:43-48: +g in x below y = 1, -g above (y = 1 itself is below)."""
+    cfg, box, st, g, *_ = _init("rpf", {"case": {"g_ext_magnitude": 2.0}})
+    r = torch.tensor([[0.5, 0.2], [0.5, 1.0], [0.5, 1.0000001], [0.1, 1.9]])
+    np.testing.assert_array_equal(g(r).numpy(), [[2, 0], [2, 0], [-2, 0], [-2, 0]])
+    assert list(box) == [1.0, 2.0] and np.all(st["u"].numpy() == 0)
+
+
+def test_heat_transfer_setup_and_hook():
+    """ht.py: the bottom wall is heated (Tag.DIRICHLET_WALL) for |x - L/2| <
+    half width, strictly; the force drives the channel's fluid band in x; the
+    hook (ht.py:152-190) sets T and dT/dt at the inflow, the walls and the
+    outflow and holds the walls at rest, leaving the rest of the state as it was."""
+    cfg, box, st, g, bc, *_ = _init("ht", {"case": {"r0_noise_factor": 0.0}})
+    r, tag = st["r"].numpy(), st["tag"].numpy()
+    hot = tag == Tag.DIRICHLET_WALL
+    # columns 13..36 of the 50 (x = 0.25 and 0.75 round to the bounds), 3 layers
+    assert hot.sum() == 24 * 3
+    assert r[hot, 0].min() > 0.25 and r[hot, 0].max() < 0.75 and r[hot, 1].max() < 0.06
+    assert np.all(st["T"][hot].numpy() == np.float32(1.23))
+    assert np.all(st["T"][~hot].numpy() == 1.0)
+    f = g(torch.tensor([[0.5, 0.03], [0.5, 0.1], [0.5, 0.29]]))
+    np.testing.assert_array_equal(f.numpy(), [[0, 0], [np.float32(2.3), 0], [0, 0]])
+    # the hook on a perturbed state
+    gen = torch.Generator().manual_seed(5)
+    s = {k: v.clone() for k, v in st.items()}
+    for k in ("u", "v", "dudt", "dvdt", "T", "dTdt"):
+        s[k] = torch.rand(s[k].shape, generator=gen) + 0.5
+    before = {k: v.clone() for k, v in s.items()}
+    out = bc(s)
+    for k in s:                                                    # the input is left as it was
+        assert torch.equal(s[k], before[k])
+    x = s["r"][:, 0].numpy()
+    fl = tag == Tag.FLUID
+    inflow, outflow = fl & (x < np.float32(0.06)), fl & (x > np.float32(1.0 - 0.06))
+    solid = tag == Tag.SOLID_WALL
+    assert inflow.sum() > 0 and outflow.sum() > 0
+    T, dT = out["T"].numpy(), out["dTdt"].numpy()
+    assert np.all(T[inflow] == 1.0) and np.all(dT[inflow] == 0)
+    assert np.all(T[hot] == np.float32(1.23)) and np.all(dT[hot] == 0)
+    assert np.all(T[solid] == 1.0) and np.all(dT[solid] == 0)
+    assert np.all(dT[outflow] == 0)
+    np.testing.assert_array_equal(T[outflow], s["T"].numpy()[outflow])
+    rest = fl & ~inflow & ~outflow
+    np.testing.assert_array_equal(T[rest], s["T"].numpy()[rest])
+    np.testing.assert_array_equal(dT[rest], s["dTdt"].numpy()[rest])
+    for k in ("u", "v", "dudt", "dvdt"):
+        assert np.all(out[k][~fl].numpy() == 0)
+        np.testing.assert_array_equal(out[k][fl].numpy(), s[k][fl].numpy())
