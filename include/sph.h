@@ -246,7 +246,9 @@ typedef struct {
 /* Vector-Jacobian product of one si_euler step (integrator.py:22-56), the
  * transpose that jax.grad builds for the reference's step.  The Riemann
  * solver (solver.py:108-164, 316-401) with density summation or its
- * continuity, walls if any held by `bc`; or standard SPH
+ * continuity, walls if any held by `bc`; delta-SPH with density summation
+ * (standard SPH plus its velocity diffusion, solver.py:259-311; its density
+ * diffusion, with continuity, is refused); or standard SPH
  * (heat conduction may be on: u', r' do not depend on the temperature, whose
  * cotangent is not formed); density summation or continuity (solver.py:24-30,
  * with or without renormalisation); with or without artificial viscosity (solver.py:404-428)

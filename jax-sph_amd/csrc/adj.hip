@@ -7,7 +7,8 @@
 // the standard SPH solver, density summation or continuity (sweep C), with or
 // without the transport-velocity formulation (tvf), artificial viscosity
 // (k_adj_forces) and the generalized wall boundary condition for no-slip
-// walls (below).  Heat conduction (solver.py:832-850) may be on: the
+// walls (below); delta-SPH with density summation (the standard solver plus
+// its velocity diffusion, in k_adj_forces).  Heat conduction (solver.py:832-850) may be on: the
 // temperature enters neither the EOS nor the forces, so u' and r' do not
 // depend on it and the transpose below is the whole one for them.
 // State n -> n+1 (integrator.py:26-54, solver.py:176-178, 199-256, 801-802):
@@ -118,6 +119,10 @@ __global__ void k_adj_gather(AdjArgs a) {
 // K = alpha h c_ab, rho_ab = (rho_k + rho_j) / 2, du = u_k - u_j; its
 // cotangent-weighted sum is 1/2 sum_kj K phi (du . r)(r . l^AV_kj) with
 // l^AV_kj = m_j abar'_k - m_k abar'_j, symmetric in (k, j) like S_kj.
+// The delta-SPH velocity diffusion (solver.py:259-311, solver == DELTA):
+//   a_k += C m_j psi_kj (du . r_kj) r_kj for fluid j,  psi = W'(d) / (rho_k rho_j d^3),
+// C = alpha h c_ref rho_ref; the same transpose with l = m_j [j fluid] abar'_k -
+// m_k [k fluid] abar'_j, and d psi / d rho_k = -psi / rho_k.
 template <int DIM, int KIND, bool TVF, bool WALLS, bool CONT, bool AV>
 __global__ __launch_bounds__(kAB) void k_adj_forces(AdjArgs aa, float4* __restrict__ ab0,
                                                     float4* __restrict__ ab1,
@@ -147,6 +152,7 @@ __global__ __launch_bounds__(kAB) void k_adj_forces(AdjArgs aa, float4* __restri
   const bool fluid = (int)Qk.z == kFluid;
   const float avK = AV ? p.artificial_alpha * p.av_h * 10.0f : 0.0f;  // alpha h c_ab
   const float av_h2 = AV ? 0.01f * p.av_h * p.av_h : 0.0f;
+  const float dK = p.solver == 2 ? p.diff_alpha * p.support * p.c_ref * p.rho_ref : 0.0f;
   float rhob = 0.0f;
   V3 ub = {0.f, 0.f, 0.f}, rb = {0.f, 0.f, 0.f}, vb = {0.f, 0.f, 0.f};
   for_each_pair(a, k, h, [&](const int j) {
@@ -219,6 +225,20 @@ __global__ __launch_bounds__(kAB) void k_adj_forces(AdjArgs aa, float4* __restri
       ub = ub + (Kphi * sl) * dr;
       rhob -= Kphi * e * sl * irs;                // d phi / d rho_k = -phi / (rho_k + rho_j)
       rb = rb + (avK * dphi * e * sl) * dr + Kphi * (sl * du + e * l);
+    }
+    const bool fluid_j = (int)Qj.z == kFluid;
+    if (dK != 0.0f && (fluid || fluid_j)) {
+      const float ipp = rcp_fast(rhok * rhoj);
+      const float psi = A0 * inv * inv * ipp;                      // W' / (rho_k rho_j d^3)
+      const float dpsi = (Ad0 * d - 2.0f * A0) * (inv * inv) * (inv * inv) * ipp;  // psi' / d
+      const V3 du = uk - uj;
+      const float e = dot<DIM>(du, dr);
+      const V3 l = (fluid_j ? mj : 0.0f) * lk - (fluid ? mk : 0.0f) * lj;
+      const float sl = dot<DIM>(dr, l);
+      const float Kpsi = dK * psi;
+      ub = ub + (Kpsi * sl) * dr;
+      rhob -= Kpsi * e * sl / rhok;
+      rb = rb + (dK * dpsi * e * sl) * dr + Kpsi * (sl * du + e * l);
     }
   });
   rhob = lane_sum<TPP>(rhob);
@@ -1060,7 +1080,8 @@ int launch_adjoint(sph_ws* ws, AdjArgs& a, bool records_ready, hipStream_t st) {
              (k_adj_forces<DIM, KIND, TVF, WALLS, CONT, AV><<<grid, kAB, 0, st>>>(
                  a, ws->AB0, ws->AB1, ws->AB2)));
   if (WALLS) {
-    // the delta-SPH records hold the per-wall factors (no delta-SPH here)
+    // the per-wall factors go to the delta-SPH continuity's buffers (that
+    // continuity is not differentiated, so they are free)
     SPH_LAUNCH(ws, "adj_walls_prep", st,
                (k_adj_walls_prep<DIM, KIND, TVF><<<grid, kAB, 0, st>>>(
                    a, ws->AB0, ws->AB2, ws->DL0, ws->DL1, ws->DL2, ws->DG0)));
@@ -1125,9 +1146,13 @@ extern "C" int sph_step_vjp(sph_ws* ws, int bank, const sph_solver_params* prm, 
     set_error("sph_step_vjp: the r, u and a cotangent arrays are required");
     return SPH_ERR_ARG;
   }
-  if (prm->solver != 0 && prm->solver != 1) {
-    set_error("sph_step_vjp: the standard SPH and Riemann solvers are differentiated "
-              "(not delta-SPH)");
+  if (prm->solver != 0 && prm->solver != 1 && prm->solver != 2) {
+    set_error("sph_step_vjp: unknown solver");
+    return SPH_ERR_ARG;
+  }
+  if (prm->solver == 2 && prm->density_evolution) {
+    set_error("sph_step_vjp: delta-SPH is differentiated with density summation only "
+              "(not its density diffusion)");
     return SPH_ERR_ARG;
   }
   if (prm->bc_trick && prm->free_slip && !next->nw) {

@@ -20,7 +20,8 @@ standard SPH (``solver="SPH"``) as below, or the Riemann solver
 (``solver="RIE"``, solver.py:108-164, 316-401) with density summation or its
 continuity, any ``tvf``, with or without its wall condition (no-slip or
 free-slip), its walls held by the case hook (the dam break as the survey's
-benchmark runs it);
+benchmark runs it), or delta-SPH (``solver="DELTA"``) with density summation:
+standard SPH plus its velocity diffusion (solver.py:259-311);
 Quintic or Wendland-C2 kernel, a constant body force or (through :func:`rollout`) a
 position-dependent one, with or without the
 transport-velocity formulation (any ``tvf``: the stress ``A = rho u (v - u)``
@@ -65,8 +66,11 @@ def _check(solver: WCSPH, tvf: float, bc_fn=None, graph: bool = False, state=Non
                    "are not differentiated")
     if bc_fn is not None and _bc_spec(bc_fn) is None:
         why.append("bc_fn must be a case boundary hook (case_setup.CaseBC)")
-    if solver.solver not in ("SPH", "RIE"):
-        why.append("solver must be 'SPH' or 'RIE'")
+    if solver.solver not in ("SPH", "RIE", "DELTA"):
+        why.append("solver must be 'SPH', 'RIE' or 'DELTA'")
+    if solver.solver == "DELTA" and solver.is_rho_evol:
+        why.append("delta-SPH is differentiated with density summation only (not the "
+                   "density diffusion of its continuity, solver.py:33-105)")
     if solver.solver == "RIE" and state is not None:
         # the wall rows of the Riemann acceleration are not transposed: every
         # wall must be held by the case hook (its accelerations are constants)
