@@ -47,6 +47,9 @@ STATE = ("r", "u", "v", "rho")
     ("ut", {"case": {"dx": 0.05}}),
     ("cw", {}),
     ("cw", {"solver": {"name": "RIE"}, "kernel": {"name": "WC2K"}}),
+    ("ut", {"case": {"dim": 3, "dx": 0.05, "special": {"cube_offset": [0.25, 0.25, 0.25]}}}),
+    # 3D: walls on all six sides (cw.py:44)
+    ("cw", {"case": {"dim": 3, "dx": 0.1, "special": {"cube_offset": [0.5, 0.5, 0.5]}}}),
 ])
 def test_fused_steps_match_oracle(case, ov, skin):
     cfg, state, nb, adv, R, fn, *_ = _make(case, ov, skin=skin)
@@ -81,6 +84,7 @@ def test_fused_steps_match_oracle(case, ov, skin):
     ("rpf", {"case": {"dim": 3, "dx": 0.05}}),
     ("ht", {"case": {"dx": 0.04}}),
     ("ht", {"case": {"dx": 0.04}, "solver": {"name": "RIE"}, "kernel": {"name": "WC2K"}}),
+    ("ht", {"case": {"dim": 3, "dx": 0.05}}),
 ])
 def test_position_dependent_cases_match_oracle(case, ov):
     """Reverse Poiseuille flow (rpf.py: a force that flips sign at y = 1) and
