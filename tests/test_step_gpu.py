@@ -41,6 +41,11 @@ STATE = ("r", "u", "v", "rho")
     ("ldc", {"case": {"dim": 3, "dx": 0.05}, "solver": {"name": "RIE"},
              "kernel": {"name": "WC2K"}}),
     ("db", {"case": {"dx": 0.04}, "solver": {"name": "RIE"}, "kernel": {"name": "WC2K"}}),
+    # the dam break as its case file runs it (continuity, AV, walls, gravity),
+    # and with validation/db2d.sh's Riemann settings (free-slip walls, no viscosity)
+    ("db", {"case": {"dx": 0.04}}),
+    ("db", {"case": {"dx": 0.04, "viscosity": 0.0, "u_ref": 2.0},
+            "solver": {"name": "RIE", "free_slip": True, "artificial_alpha": 0.0}}),
     ("ldc", {"case": {"dx": 0.04}, "solver": {"name": "DELTA", "density_evolution": True,
                                               "density_renormalize": True}}),
     # a fluid cube falling in a periodic box (ut.py), and in a walled box (cw.py)
